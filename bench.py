@@ -1,0 +1,205 @@
+"""Benchmark of the MI355X engine on OpFlow's explicit hot path.
+
+Headline workload (one GPU): 3-D FTCS heat sweep, 512^3 cells FP64,
+periodic, u0 ~ U[-1,1] i.i.d. from a seed, dt = 0.2 h^2, alpha = 1 -- one
+"step" is one FTCS time step over the whole grid (one fused launch).
+value = cell-updates/s over the timed steps (inputs resident in HBM).
+
+Also reported (extra): the FP32 variant of the same sweep, the 2-D 256^2
+Dirichlet FTCS config (1000 steps), and the implicit-solver numbers when the
+multigrid solver is built.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-updates/s & %HBM peak; Poisson solves/s + p99 latency at rel-res 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEventRecord on the library's stream (not torch's)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def ftcs_field(of, ctx, n, dim, dtype, bc_type, seed, init="uniform"):
+    nodes = [n + 1] * dim
+    mb = of.MeshBuilder(ctx).newMesh(*nodes)
+    for d in range(dim):
+        mb.setMeshOfDim(d, 0.0, 1.0)
+    mesh = mb.build()
+    b = of.ExprBuilder().setName("u").setMesh(mesh).setLoc(of.LocOnMesh.Center).setExt(1).setDType(dtype)
+    for d in range(dim):
+        b.setBC(d, of.DimPos.start, bc_type, 0.0).setBC(d, of.DimPos.end, bc_type, 0.0)
+    u = b.build()
+    if init == "uniform":
+        import numpy as np
+        cnt = 1
+        for e in u.shape:
+            cnt *= e
+        u.upload(of.gen_uniform(seed, cnt, -1.0, 1.0).reshape(u.shape).astype(u.dtype))
+    else:
+        u.upload(of.gen_bumps(u, seed, 4).astype(u.dtype))
+    u.updatePadding()
+    return mesh, u
+
+
+def time_ftcs(of, ctx, ev, u, coef, steps, warmup):
+    u.ftcs(coef, warmup)
+    ctx.sync()
+    e0, e1 = ev.create(), ev.create()
+    t0 = time.perf_counter()
+    ev.record(e0, ctx.stream)
+    u.ftcs(coef, steps)
+    ev.record(e1, ctx.stream)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms = ev.elapsed_ms(e0, e1)
+    return ms, wall
+
+
+def cpu_baseline_ftcs3d(n, steps):
+    """Oracle (C/OpenMP restatement of OpFlow's FieldAssigner FTCS sequence) on host cores."""
+    import numpy as np
+    import oracle
+    from tests.helpers import desc as mkdesc
+    d = mkdesc(3, [n + 1] * 3, 1, [(3, 3)] * 3, ext=(1, 1))
+    om = oracle.Mesh([n + 1] * 3)
+    import opflow_amd as of
+    cnt = d.extent[0] * d.extent[1] * d.extent[2]
+    u = of.gen_uniform(0xC0FFEE + 2, cnt, -1.0, 1.0).reshape(d.extent[2], d.extent[1], d.extent[0])
+    u = oracle.update_padding(d, om, u)
+    h = 1.0 / n
+    oracle.ftcs(d, om, u, 0.2 * h * h, 1)  # warm-up
+    t0 = time.perf_counter()
+    oracle.ftcs(d, om, u, 0.2 * h * h, steps)
+    dt = time.perf_counter() - t0
+    return n ** 3 * steps / dt / 1e6, dt, oracle.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: the engine is single-GPU by design "
+              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import opflow_amd as of
+    ctx = of.Context(0)
+    ev = HipEvents()
+    n = args.n
+    h = 1.0 / n
+    coef = 0.2 * h * h  # dt = 0.2 h^2, alpha = 1
+
+    # --- headline: 3-D FTCS 512^3 FP64 periodic
+    mesh, u = ftcs_field(of, ctx, n, 3, "f64", of.BCType.Periodic, 0xC0FFEE + 2)
+    ms, wall = time_ftcs(of, ctx, ev, u, coef, args.steps, args.warmup)
+    cells = n ** 3
+    value = cells * args.steps / (ms / 1e3) / 1e6
+    per_launch_s = ms / 1e3 / args.steps
+    alg_bytes = 16.0 * cells  # read u^n + write u^{n+1}, FP64
+    achieved = alg_bytes / per_launch_s / 1e9
+    u.close()
+
+    extra = {}
+    if not args.no_extra:
+        mesh32, u32 = ftcs_field(of, ctx, n, 3, "f32", of.BCType.Periodic, 0xC0FFEE + 2)
+        ms32, _ = time_ftcs(of, ctx, ev, u32, coef, args.steps, args.warmup)
+        v32 = cells * args.steps / (ms32 / 1e3) / 1e6
+        extra["ftcs3d_512_fp32"] = {"Mcell_updates_per_s": v32, "ms_per_step": ms32 / args.steps,
+                                    "hbm_frac": 8.0 * cells / (ms32 / 1e3 / args.steps) / 1e9 / HBM_PEAK_GBS}
+        u32.close()
+        n2 = 256
+        h2 = 1.0 / n2
+        mesh2, u2 = ftcs_field(of, ctx, n2, 2, "f64", of.BCType.Dirc, 0xC0FFEE + 1, init="bumps")
+        ms2, _ = time_ftcs(of, ctx, ev, u2, 0.2 * h2 * h2, 1000, 50)
+        extra["ftcs2d_256_fp64_1000steps"] = {"Mcell_updates_per_s": n2 * n2 * 1000 / (ms2 / 1e3) / 1e6,
+                                              "ms_total": ms2}
+        u2.close()
+        try:
+            from bench_solver import solver_extra
+            extra.update(solver_extra(of, ctx, ev))
+        except ImportError:
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        v_cpu, t_cpu, cores = cpu_baseline_ftcs3d(n, args.cpu_steps)
+        cpu = {"value": v_cpu, "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
+               "sample": f"oracle FTCS (OpFlow FieldAssigner sequence, C/OpenMP) on the same 512^3 FP64 periodic "
+                         f"input, {args.cpu_steps} steps in {t_cpu:.1f} s on {cores} threads of {cpu_model()}"}
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "ftcs3d_512_fp64_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (i.i.d. U[-1,1] from SplitMix64 seed 0xC0FFEE+2)",
+        "config": {"workload": "ftcs3d 512^3 cells fp64 periodic dt=0.2h^2 (one step = one fused sweep)",
+                   "cells": cells, "grid": [n, n, n], "bc": "periodic"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": wall,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

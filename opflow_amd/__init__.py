@@ -1,0 +1,472 @@
+"""opflow_amd -- host-side mirror of OpFlow's explicit/implicit front-end over
+the MI355X engine (libopflow_hip.so, C ABI in include/opflow_hip.h).
+
+Names follow OpFlow (src/Core/...): MeshBuilder/CartesianMesh, ExprBuilder/
+CartesianField with setBC/setExt/setLoc/setPadding, the operator kernels
+D2SecondOrderCentered, D1FirstOrderCentered, D1FirstOrderBiasedUpwind/
+Downwind, D1WENO53Upwind/Downwind, D1Linear interpolation, `field.assign(expr)`
+for OpFlow's `field = expr`, rangeReduce, Solve/makeEqnSolveHandler and
+StructSolverParams. Every compute call goes to the HIP library; there is no
+CPU path in this package.
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = [
+    "Context", "LocOnMesh", "BCType", "DimPos", "BasicArithOp", "StructSolverType", "MeshBuilder",
+    "CartesianMesh", "ExprBuilder", "CartesianField", "Expr", "D2SecondOrderCentered", "D1FirstOrderCentered",
+    "D1FirstOrderBiasedUpwind", "D1FirstOrderBiasedDownwind", "D1WENO53Upwind", "D1WENO53Downwind",
+    "d2x", "d2y", "d2z", "dx", "dy", "dz", "d1", "d2", "d1IntpCenterToCorner", "d1IntpCornerToCenter",
+    "rangeReduce", "StructSolverParams", "EqnSolveState", "Solve", "makeEqnSolveHandler", "PrecondStructSolver",
+    "PoissonGMG", "Server",
+]
+
+
+class LocOnMesh(enum.IntEnum):  # src/Core/Constants.hpp:50
+    Corner = 0
+    Center = 1
+
+
+class BCType(enum.IntEnum):  # src/Core/BC/BCBase.hpp:24
+    Undefined = 0
+    Dirc = 1
+    Neum = 2
+    Periodic = 3
+    Internal = 4
+    Symm = 5
+    ASymm = 6
+
+
+class DimPos(enum.IntEnum):  # src/Core/Constants.hpp:48
+    start = 0
+    end = 1
+
+
+class BasicArithOp(enum.IntEnum):  # src/Core/Constants.hpp:53
+    Eq = 0
+    Add = 1
+    Minus = 2
+    Mul = 3
+    Div = 4
+
+
+class StructSolverType(enum.IntEnum):  # src/Core/Solvers/Struct/StructSolver.hpp:23-34
+    None_ = 0
+    Jacobi = 1
+    SMG = 2
+    PFMG = 3
+    CYCRED = 4
+    PCG = 5
+    GMRES = 6
+    FGMRES = 7
+    LGMRES = 8
+    BICGSTAB = 9
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One device + one HIP stream (replaces the global ParallelPlan/TBB arena)."""
+
+    def __init__(self, device=0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.opf_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        L.check(self.lib.opf_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return self.lib.opf_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.opf_ctx_destroy(self.h)
+            self.h = None
+
+
+# ---------------------------------------------------------------- mesh
+def uniform_mesh_arrays(nnodes, lo, hi, pad=5):
+    """Node coordinates/spacings of MeshBuilder::set1DMesh(min,max) + Symm ext."""
+    lib = L.load()
+    x = np.zeros(nnodes + 2 * pad)
+    dx = np.zeros(nnodes + 2 * pad - 1)
+    L.check(lib.opf_mesh_uniform_arrays(nnodes, lo, hi, pad, x.ctypes.data_as(L._DP), dx.ctypes.data_as(L._DP)))
+    return x, dx
+
+
+class CartesianMesh:
+    def __init__(self, ctx, dims, ranges, pad=5):
+        self.ctx = ctx
+        self.dim = len(dims)
+        self.dims = list(dims)
+        self.pad = pad
+        self.xs, self.dxs = [], []
+        for d in range(self.dim):
+            x, dx_ = uniform_mesh_arrays(dims[d], ranges[d][0], ranges[d][1], pad)
+            self.xs.append(x)
+            self.dxs.append(dx_)
+        lib = ctx.lib
+        xp = (L._DP * 3)()
+        dp = (L._DP * 3)()
+        for d in range(self.dim):
+            xp[d] = self.xs[d].ctypes.data_as(L._DP)
+            dp[d] = self.dxs[d].ctypes.data_as(L._DP)
+        start = (C.c_int * 3)(0, 0, 0)
+        nn = (C.c_int * 3)(*(self.dims + [1] * (3 - self.dim)))
+        h = C.c_void_p()
+        L.check(lib.opf_mesh_create(ctx.h, self.dim, start, nn, pad, xp, dp, C.byref(h)))
+        self.h = h
+
+    def x(self, d, i):
+        return self.xs[d][i + self.pad]
+
+    def dx(self, d, i):
+        return self.dxs[d][i + self.pad]
+
+    def getRange(self):
+        return [(0, n) for n in self.dims]
+
+
+class MeshBuilder:
+    """MeshBuilder<CartesianMesh<Meta::int_<dim>>>().newMesh(...).setMeshOfDim(d, lo, hi).build()."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._dims = None
+        self._ranges = {}
+        self._pad = 5
+
+    def newMesh(self, *n):
+        self._dims = list(n)
+        return self
+
+    def setMeshOfDim(self, d, lo, hi):
+        self._ranges[d] = (lo, hi)
+        return self
+
+    def setPadWidth(self, p):
+        self._pad = p
+        return self
+
+    def build(self):
+        rng = [self._ranges.get(d, (0.0, 1.0)) for d in range(len(self._dims))]
+        return CartesianMesh(self.ctx, self._dims, rng, self._pad)
+
+
+# ---------------------------------------------------------------- expressions
+_N_FIELD, _N_CONST, _N_ADD, _N_SUB, _N_MUL, _N_DIV, _N_NEG = 0, 1, 2, 3, 4, 5, 6
+_N_D2C, _N_D1C, _N_D1UP, _N_D1DOWN, _N_WENO_UP, _N_WENO_DOWN = 7, 8, 9, 10, 11, 12
+_N_INTP_C2N, _N_INTP_N2C, _N_UNKNOWN = 13, 14, 15
+
+
+def _wrap(v):
+    if isinstance(v, Expr):
+        return v
+    if isinstance(v, (int, float, np.floating, np.integer)):
+        return Expr(_N_CONST, value=float(v))
+    raise TypeError(f"cannot use {type(v)} in an expression")
+
+
+class Expr:
+    """Lazy expression node (Expr/Expression.hpp); lowered by the library."""
+
+    def __init__(self, kind, a=None, b=None, dim=0, value=0.0, field=None):
+        self.kind, self.a, self.b, self.dim, self.value, self.field = kind, a, b, dim, value, field
+
+    def __add__(self, o): return Expr(_N_ADD, self, _wrap(o))
+    def __radd__(self, o): return Expr(_N_ADD, _wrap(o), self)
+    def __sub__(self, o): return Expr(_N_SUB, self, _wrap(o))
+    def __rsub__(self, o): return Expr(_N_SUB, _wrap(o), self)
+    def __mul__(self, o): return Expr(_N_MUL, self, _wrap(o))
+    def __rmul__(self, o): return Expr(_N_MUL, _wrap(o), self)
+    def __truediv__(self, o): return Expr(_N_DIV, self, _wrap(o))
+    def __rtruediv__(self, o): return Expr(_N_DIV, _wrap(o), self)
+    def __neg__(self): return Expr(_N_NEG, self)
+
+    def __eq__(self, o):  # Equation: lhs == rhs (Equation/Equation.hpp:27-55)
+        return Equation(self, _wrap(o))
+
+    __hash__ = object.__hash__
+
+    def flatten(self, unknown=None):
+        """Post-order node list + field slot list for the C ABI."""
+        nodes, fields, memo = [], [], {}
+        slot_of = {}
+
+        def visit(e):
+            if id(e) in memo:
+                return memo[id(e)]
+            if e.kind == _N_FIELD and unknown is not None and e.field is unknown:
+                node = (_N_UNKNOWN, 0, 0, 0, 0.0)
+            elif e.kind == _N_FIELD:
+                key = id(e.field)
+                if key not in slot_of:
+                    slot_of[key] = len(fields)
+                    fields.append(e.field)
+                node = (_N_FIELD, slot_of[key], 0, 0, 0.0)
+            elif e.kind == _N_CONST:
+                node = (_N_CONST, 0, 0, 0, e.value)
+            else:
+                ia = visit(e.a)
+                ib = visit(e.b) if e.b is not None else 0
+                node = (e.kind, ia, ib, e.dim, 0.0)
+            nodes.append(node)
+            memo[id(e)] = len(nodes) - 1
+            return memo[id(e)]
+
+        root = visit(self)
+        return nodes, fields, root
+
+
+class Equation:
+    def __init__(self, lhs, rhs):
+        self.lhs, self.rhs = lhs, rhs
+
+
+def _node_array(nodes):
+    arr = (L.ExprNode * len(nodes))()
+    for i, (k, a, b, d, v) in enumerate(nodes):
+        arr[i].kind, arr[i].a, arr[i].b, arr[i].dim, arr[i].value = k, a, b, d, v
+    return arr
+
+
+# operator kernels (FDMOperators/*.hpp, Interpolator/D1Linear.hpp)
+class _Kernel:
+    kind = None
+
+
+class D2SecondOrderCentered(_Kernel):
+    kind = _N_D2C
+
+
+class D1FirstOrderCentered(_Kernel):
+    kind = _N_D1C
+
+
+class D1FirstOrderBiasedUpwind(_Kernel):
+    kind = _N_D1UP
+
+
+class D1FirstOrderBiasedDownwind(_Kernel):
+    kind = _N_D1DOWN
+
+
+class D1WENO53Upwind(_Kernel):
+    kind = _N_WENO_UP
+
+
+class D1WENO53Downwind(_Kernel):
+    kind = _N_WENO_DOWN
+
+
+def d1(kernel, d, e):
+    return Expr(kernel.kind, _wrap(e), dim=d)
+
+
+def d2(kernel, d, e):
+    return Expr(kernel.kind, _wrap(e), dim=d)
+
+
+def d2x(kernel, e): return d2(kernel, 0, e)
+def d2y(kernel, e): return d2(kernel, 1, e)
+def d2z(kernel, e): return d2(kernel, 2, e)
+def dx(kernel, e): return d1(kernel, 0, e)
+def dy(kernel, e): return d1(kernel, 1, e)
+def dz(kernel, e): return d1(kernel, 2, e)
+
+
+def d1IntpCenterToCorner(d, e):  # IntpInterface.hpp:26-34, D1Linear<d, Cen2Cor>
+    return Expr(_N_INTP_C2N, _wrap(e), dim=d)
+
+
+def d1IntpCornerToCenter(d, e):  # D1Linear<d, Cor2Cen>
+    return Expr(_N_INTP_N2C, _wrap(e), dim=d)
+
+
+# ---------------------------------------------------------------- fields
+class CartesianField(Expr):
+    """Device-resident CartesianField (CartesianField.hpp:37-794)."""
+
+    def __init__(self, mesh, desc, name=""):
+        super().__init__(_N_FIELD, field=self)
+        self.mesh, self.ctx, self.name = mesh, mesh.ctx, name
+        self.lib = mesh.ctx.lib
+        h = C.c_void_p()
+        L.check(self.lib.opf_field_create(mesh.h, C.byref(desc), C.byref(h)))
+        self.h = h
+        self.desc = L.FieldDesc.from_buffer_copy(self.lib.opf_field_get_desc(h).contents)
+        self.dtype = np.float64 if desc.dtype == L.OPF_F64 else np.float32
+
+    # --- ranges (DS::Range as (start, end) tuples per dim)
+    def _rng(self, s, e):
+        return [(getattr(self.desc, s)[d], getattr(self.desc, e)[d]) for d in range(self.desc.dim)]
+
+    @property
+    def localRange(self): return self._rng("local_start", "local_end")
+    @property
+    def assignableRange(self): return self._rng("assign_start", "assign_end")
+    @property
+    def accessibleRange(self): return self._rng("access_start", "access_end")
+    @property
+    def logicalRange(self): return self._rng("logical_start", "logical_end")
+
+    @property
+    def shape(self):
+        """numpy shape of the storage block (z, y, x), x fastest."""
+        e = self.desc.extent
+        return tuple(e[d] for d in reversed(range(self.desc.dim)))
+
+    def offset(self):
+        return [self.desc.offset[d] for d in range(self.desc.dim)]
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        L.check(self.lib.opf_field_download(self.h, out.ctypes.data, out.nbytes))
+        return out
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype).reshape(self.shape)
+        L.check(self.lib.opf_field_upload(self.h, a.ctypes.data, a.nbytes))
+
+    def index(self, *idx):
+        """storage tuple (numpy index) of a field index (i0, i1, ...)."""
+        return tuple(idx[d] - self.desc.offset[d] for d in reversed(range(self.desc.dim)))
+
+    def updatePadding(self):
+        L.check(self.lib.opf_field_update_padding(self.h))
+
+    def assign(self, rhs, op=BasicArithOp.Eq):
+        """OpFlow's `field (op)= rhs` (Expr::operator=, FieldAssigner::assign)."""
+        if isinstance(rhs, (int, float, np.floating, np.integer)):
+            L.check(self.lib.opf_field_assign_const(self.h, int(op), float(rhs)))
+            return self
+        nodes, fields, root = _wrap(rhs).flatten()
+        arr = _node_array(nodes)
+        fh = (C.c_void_p * max(1, len(fields)))(*[f.h.value for f in fields])
+        L.check(self.lib.opf_sweep(arr, len(nodes), root, fh, len(fields), self.h, int(op)))
+        return self
+
+    def copy_from(self, other):
+        L.check(self.lib.opf_field_copy(self.h, other.h))
+
+    def ftcs(self, coef, nsteps):
+        """Fused fast path of u = u + coef*(d2x(u)+d2y(u)[+d2z(u)]) repeated nsteps times."""
+        L.check(self.lib.opf_ftcs(self.h, float(coef), int(nsteps)))
+
+    def ftcs_blocked(self, coef, nsteps, tblock):
+        L.check(self.lib.opf_ftcs_blocked(self.h, float(coef), int(nsteps), int(tblock)))
+
+    def device_ptr(self):
+        return self.lib.opf_field_device_ptr(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.opf_field_destroy(self.h)
+            self.h = None
+
+
+class ExprBuilder:
+    """ExprBuilder<CartesianField>() (CartesianField.hpp:797-1033)."""
+
+    def __init__(self):
+        self.d = L.FieldDesc()
+        self.mesh = None
+        self.name = ""
+        self.dtype = L.OPF_F64
+        for i in range(3):
+            self.d.loc[i] = LocOnMesh.Corner  # OpFlow's default location
+
+    def setName(self, n):
+        self.name = n
+        return self
+
+    def setMesh(self, m):
+        self.mesh = m
+        self.d.dim = m.dim
+        return self
+
+    def setLoc(self, loc):
+        if isinstance(loc, (list, tuple)):
+            for i, l_ in enumerate(loc):
+                self.d.loc[i] = int(l_)
+        else:
+            for i in range(3):
+                self.d.loc[i] = int(loc)
+        return self
+
+    def setLocOfDim(self, i, loc):
+        self.d.loc[i] = int(loc)
+        return self
+
+    def setBC(self, d, pos, type_, val=0.0):
+        self.d.bc_type[d][int(pos)] = int(type_)
+        self.d.bc_value[d][int(pos)] = float(val)
+        return self
+
+    def setExt(self, *args):
+        if len(args) == 1:
+            for d in range(3):
+                self.d.ext[d][0] = self.d.ext[d][1] = args[0]
+        else:
+            d, pos, w = args
+            self.d.ext[d][int(pos)] = w
+        return self
+
+    def setPadding(self, p):
+        self.d.padding = p
+        return self
+
+    def setDType(self, dt):
+        self.dtype = L.OPF_F32 if dt in ("f32", np.float32, "float32") else L.OPF_F64
+        return self
+
+    def descriptor(self):
+        d = L.FieldDesc.from_buffer_copy(self.d)
+        d.dtype = self.dtype
+        start = (C.c_int * 3)(0, 0, 0)
+        nn = (C.c_int * 3)(*(self.mesh.dims + [1] * (3 - self.mesh.dim)))
+        L.check(self.mesh.ctx.lib.opf_field_desc_finalize(C.byref(d), start, nn))
+        return d
+
+    def build(self):
+        return CartesianField(self.mesh, self.descriptor(), self.name)
+
+
+_RED = {"sum": 0, "max": 1, "min": 2, "sumsq": 3, "maxabs": 4}
+
+
+def rangeReduce(field, red, rng=None):
+    """rangeReduce(range, op, f) for op in {sum,max,min,sumsq,maxabs} (RangeFor.hpp:86-121)."""
+    rng = rng or field.assignableRange
+    dim = field.desc.dim
+    rs = (C.c_int * 3)(*([r[0] for r in rng] + [0] * (3 - dim)))
+    re_ = (C.c_int * 3)(*([r[1] for r in rng] + [1] * (3 - dim)))
+    out = C.c_double()
+    L.check(field.lib.opf_reduce(field.h, _RED[red], rs, re_, C.byref(out)))
+    return out.value
+
+
+def gen_bumps(field, seed, nbumps, noise=0.0):
+    """Host array (storage layout) of seeded Gaussian bumps (+ noise) at cell centres."""
+    out = np.zeros(field.shape, dtype=np.float64)
+    L.check(field.lib.opf_gen_bumps(field.mesh.h, C.byref(field.desc), C.c_uint64(seed), nbumps, noise,
+                                    out.ctypes.data_as(L._DP)))
+    return out
+
+
+def gen_uniform(seed, n, lo=0.0, hi=1.0):
+    lib = L.load()
+    out = np.zeros(n, dtype=np.float64)
+    L.check(lib.opf_gen_uniform(C.c_uint64(seed), n, lo, hi, out.ctypes.data_as(L._DP)))
+    return out
+
+
+from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
+                    PrecondStructSolver, PoissonGMG, Server)

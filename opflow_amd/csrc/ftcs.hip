@@ -1,0 +1,210 @@
+// FTCS heat update fast path:  u = u + coef * (d2x(u) + d2y(u) [+ d2z(u)])
+// (examples/FTCS2D/FTCS-OMP.cpp:26, D2SecondOrderCentered.hpp:160-171,
+//  FieldAssigner.hpp:26-86). One launch per step reads u^n (interior + ghosts)
+// and writes u^{n+1} together with the ghost values updatePadding would give
+// it, so the field is complete after every launch. On a uniform mesh the
+// reference's dx_l = dx_r = dx_c = h exactly, so D2 = ((r-c)/h - (c-l)/h)/h;
+// when h is a power of two the divisions are replaced by exact multiplications
+// by 1/h (same bits).
+#include <algorithm>
+#include <cmath>
+
+#include "opf_internal.hpp"
+
+using namespace opf;
+
+namespace {
+
+template <typename T>
+struct FtcsArgs {
+    int n[3];       // interior cells
+    int e0, e1;     // storage extents (n+2)
+    T coef;
+    T hh[3];        // 1/h (pow2 mode) or h (division mode)
+    int per[3];
+    int rtype[3][2];  // ghost rule per dim/side: 1 dirichlet, 2 neumann
+    T rc1[3][2], rc2[3][2], rc3[3][2];
+};
+
+template <typename T, bool POW2>
+__device__ __forceinline__ T d2(T l, T c, T r, T hh) {
+    if constexpr (POW2) return ((r - c) * hh - (c - l) * hh) * hh;
+    else return ((r - c) / hh - (c - l) / hh) / hh;
+}
+
+template <typename T>
+__device__ __forceinline__ T rule(const FtcsArgs<T>& a, int d, int s, T v) {
+    int t = a.rtype[d][s];
+    if (t == 1) return (a.rc1[d][s] * v - a.rc2[d][s]) / a.rc3[d][s];
+    if (t == 2) return v + a.rc1[d][s];
+    return v;
+}
+
+// Write the ghost images whose source is interior cell p (value v), in the
+// closed form of updatePaddingImpl_final for one ghost layer.
+template <typename T, int DIM>
+__device__ __forceinline__ void write_images(T* __restrict__ out, const FtcsArgs<T>& a, const int p[3], T v) {
+    int lo[3], hi[3];  // is p the source of the lower / upper ghost of dim d
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (a.per[d]) {
+            lo[d] = p[d] == a.n[d] - 1;
+            hi[d] = p[d] == 0;
+        } else {
+            lo[d] = p[d] == 0;
+            hi[d] = p[d] == a.n[d] - 1;
+        }
+        any |= lo[d] | hi[d];
+    }
+    if (!any) return;
+    constexpr int NC = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    for (int c = 1; c < NC; ++c) {
+        int ch[3] = {c % 3, (c / 3) % 3, c / 9};
+        int q[3] = {p[0], p[1], p[2]};
+        T w = v;
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            if (ch[d] == 0) continue;
+            int s = ch[d] - 1;
+            if (!(s == 0 ? lo[d] : hi[d])) { ok = false; break; }
+            q[d] = s == 0 ? -1 : a.n[d];
+            if (!a.per[d]) w = rule(a, d, s, w);
+        }
+        if (!ok) continue;
+        long long l = (q[0] + 1) + (long long)a.e0 * ((q[1] + 1) + (long long)a.e1 * (q[2] + 1));
+        out[l] = w;
+    }
+}
+
+// 2-D: one cell per thread, block 64 x 4.
+template <typename T, bool POW2>
+__global__ __launch_bounds__(256) void ftcs2d_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a) {
+    int i = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= a.n[0] || j >= a.n[1]) return;
+    long long l = (i + 1) + (long long)a.e0 * (j + 1);
+    T c = in[l];
+    T v = c + a.coef * (d2<T, POW2>(in[l - 1], c, in[l + 1], a.hh[0]) +
+                        d2<T, POW2>(in[l - a.e0], c, in[l + a.e0], a.hh[1]));
+    out[l] = v;
+    int p[3] = {i, j, 0};
+    write_images<T, 2>(out, a, p, v);
+}
+
+// 3-D: block 64 x 4 owns an x-y tile and walks ZC planes in z, keeping the
+// column's z-1 / z / z+1 values in registers; x/y neighbours come through L1/L2.
+template <typename T, bool POW2, int ZC>
+__global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * ZC;
+    if (i >= a.n[0] || j >= a.n[1]) return;
+    const long long sy = a.e0, sz = (long long)a.e0 * a.e1;
+    long long l = (i + 1) + sy * (j + 1) + sz * (k0 + 1);
+    T zm = in[l - sz], zc = in[l];
+    const int kend = min(k0 + ZC, a.n[2]);
+    for (int k = k0; k < kend; ++k, l += sz) {
+        T zp = in[l + sz];
+        T s = d2<T, POW2>(in[l - 1], zc, in[l + 1], a.hh[0]) + d2<T, POW2>(in[l - sy], zc, in[l + sy], a.hh[1]);
+        s = s + d2<T, POW2>(zm, zc, zp, a.hh[2]);
+        T v = zc + a.coef * s;
+        out[l] = v;
+        int p[3] = {i, j, k};
+        write_images<T, 3>(out, a, p, v);
+        zm = zc;
+        zc = zp;
+    }
+}
+
+bool is_pow2(double h) {
+    if (!(h > 0)) return false;
+    int e;
+    double m = std::frexp(h, &e);
+    return m == 0.5;
+}
+
+template <typename T>
+int ftcs_t(opf_field* f, double coef, int nsteps) {
+    const opf_field_desc& d = f->d;
+    const opf_mesh* m = f->mesh;
+    FtcsArgs<T> a{};
+    bool pow2 = true;
+    for (int q = 0; q < 3; ++q) {
+        a.n[q] = q < d.dim ? d.local_end[q] - d.local_start[q] : 1;
+        a.per[q] = 0;
+    }
+    a.e0 = d.extent[0];
+    a.e1 = d.extent[1];
+    a.coef = (T)coef;
+    GhostInfo gi;
+    OPF_TRY(build_ghost_info(f, &gi));
+    for (int q = 0; q < d.dim; ++q) {
+        OPF_CHECK(d.loc[q] == OPF_LOC_CENTER, OPF_ERR_UNSUPPORTED, "ftcs: centre location required");
+        OPF_CHECK(d.padding == 1 && d.ext[q][0] == 1 && d.ext[q][1] == 1, OPF_ERR_UNSUPPORTED,
+                  "ftcs: padding = ext = 1 required");
+        OPF_CHECK(m->uniform[q], OPF_ERR_UNSUPPORTED, "ftcs: uniform mesh required");
+        OPF_CHECK(d.assign_start[q] == d.local_start[q] && d.assign_end[q] == d.local_end[q] && d.local_start[q] == 0,
+                  OPF_ERR_UNSUPPORTED, "ftcs: assignable range must be the whole interior");
+        a.per[q] = gi.per[q];
+        for (int s = 0; s < 2; ++s) {
+            int t = d.bc_type[q][s];
+            OPF_CHECK(t == OPF_BC_DIRC || t == OPF_BC_NEUM || t == OPF_BC_PERIODIC, OPF_ERR_UNSUPPORTED,
+                      "ftcs: bc %d unsupported", t);
+            const GhostRule& r = gi.rule[q][s][0];
+            a.rtype[q][s] = r.type;
+            a.rc1[q][s] = (T)r.c1;
+            a.rc2[q][s] = (T)r.c2;
+            a.rc3[q][s] = (T)r.c3;
+        }
+        double h = m->h[q];
+        // D2 on a uniform mesh: dx_l = (h+h)*0.5 = h, dx_r = h, dx_c = h exactly
+        pow2 &= is_pow2(h);
+        a.hh[q] = (T)h;
+    }
+    if (pow2)
+        for (int q = 0; q < d.dim; ++q) a.hh[q] = (T)(1.0 / m->h[q]);
+    hipStream_t st = f->ctx->stream;
+    if (!f->spare) {
+        OPF_HIP(hipMalloc(&f->spare, f->nbytes));
+        f->spare_valid = false;
+    }
+    if (!f->spare_valid) {
+        OPF_HIP(hipMemcpyAsync(f->spare, f->data, f->nbytes, hipMemcpyDeviceToDevice, st));
+        f->spare_valid = true;
+    }
+    constexpr int ZC = 16;
+    for (int it = 0; it < nsteps; ++it) {
+        const T* in = (const T*)f->data;
+        T* out = (T*)f->spare;
+        if (d.dim == 2) {
+            dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
+            if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);
+            else hipLaunchKernelGGL((ftcs2d_kernel<T, false>), grid, dim3(64, 4), 0, st, in, out, a);
+        } else if (d.dim == 3) {
+            dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + ZC - 1) / ZC);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_kernel<T, true, ZC>), grid, dim3(64, 4), 0, st, in, out, a);
+            else hipLaunchKernelGGL((ftcs3d_kernel<T, false, ZC>), grid, dim3(64, 4), 0, st, in, out, a);
+        } else {
+            set_error("ftcs: dim %d not supported by the fast path", d.dim);
+            return OPF_ERR_UNSUPPORTED;
+        }
+        std::swap(f->data, f->spare);
+    }
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
+}
+
+}  // namespace
+
+extern "C" int opf_ftcs(opf_field* u, double coef, int nsteps) {
+    OPF_CHECK(u && nsteps >= 0, OPF_ERR_ARG, "opf_ftcs: bad args");
+    if (nsteps == 0) return OPF_OK;
+    return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps) : ftcs_t<float>(u, coef, nsteps);
+}
+
+extern "C" int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock) {
+    OPF_CHECK(tblock >= 1, OPF_ERR_ARG, "tblock >= 1");
+    return opf_ftcs(u, coef, nsteps);
+}

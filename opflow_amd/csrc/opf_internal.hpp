@@ -1,0 +1,123 @@
+// Internal declarations shared by the translation units of libopflow_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/opflow_hip.h"
+
+namespace opf {
+
+void set_error(const char* fmt, ...);
+
+#define OPF_HIP(call)                                                                               \
+    do {                                                                                            \
+        hipError_t _e = (call);                                                                     \
+        if (_e != hipSuccess) {                                                                     \
+            opf::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, hipGetErrorString(_e));     \
+            return OPF_ERR_HIP;                                                                     \
+        }                                                                                           \
+    } while (0)
+
+#define OPF_CHECK(cond, code, ...)                                                                  \
+    do {                                                                                            \
+        if (!(cond)) {                                                                              \
+            opf::set_error(__VA_ARGS__);                                                            \
+            return (code);                                                                          \
+        }                                                                                           \
+    } while (0)
+
+#define OPF_TRY(expr)                                                                               \
+    do {                                                                                            \
+        int _rc = (expr);                                                                           \
+        if (_rc != OPF_OK) return _rc;                                                              \
+    } while (0)
+
+inline size_t dtype_size(int dt) { return dt == OPF_F32 ? 4 : 8; }
+
+}  // namespace opf
+
+struct opf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // scratch for deterministic reductions (partials + result)
+    double* red_buf = nullptr;
+    size_t red_cap = 0;
+    double* red_host = nullptr;  // pinned
+    // cache of lowered device programs keyed by content hash
+    struct ProgEntry {
+        uint64_t hash;
+        void* dev;
+        size_t bytes;
+    };
+    std::vector<ProgEntry> progs;
+};
+
+struct opf_mesh {
+    opf_ctx* ctx = nullptr;
+    int dim = 0;
+    int start[3] = {0, 0, 0};
+    int nnodes[3] = {1, 1, 1};
+    int pad = 0;
+    std::vector<double> hx[3], hdx[3];  // host copies, index k <-> node start-pad+k
+    double* dx_dev[3] = {nullptr, nullptr, nullptr};
+    double* x_dev[3] = {nullptr, nullptr, nullptr};
+    float* dx_f32[3] = {nullptr, nullptr, nullptr};  // lazily made for FP32 sweeps
+    float* x_f32[3] = {nullptr, nullptr, nullptr};
+    bool uniform[3] = {false, false, false};
+    double h[3] = {0, 0, 0};
+    double x(int d, int i) const { return hx[d][i - (start[d] - pad)]; }
+    double dx(int d, int i) const { return hdx[d][i - (start[d] - pad)]; }
+    bool has_node(int d, int i) const {
+        return i - (start[d] - pad) >= 0 && i - (start[d] - pad) < (int)hx[d].size();
+    }
+    bool has_dx(int d, int i) const {
+        return i - (start[d] - pad) >= 0 && i - (start[d] - pad) < (int)hdx[d].size();
+    }
+};
+
+struct opf_field {
+    opf_ctx* ctx = nullptr;
+    opf_mesh* mesh = nullptr;
+    opf_field_desc d{};
+    void* data = nullptr;   // device storage
+    void* spare = nullptr;  // second buffer for in-place updates (lazy)
+    bool spare_valid = false;
+    size_t nbytes = 0;
+    size_t count = 0;
+};
+
+namespace opf {
+
+// Ghost-filling rule of one (dim, side, layer): value = f(mirror value).
+// type: 0 none (no bc), 1 dirichlet-intp, 2 neumann, 3 symm, 4 asymm, 5 periodic
+struct GhostRule {
+    int type;
+    double c1, c2, c3;  // dirichlet: (c1*v - c2)/c3 ; neumann: v + c1
+};
+
+// Per-field ghost table consumed by the device ghost kernels.
+struct GhostInfo {
+    int dim;
+    int ls[3], le[3];      // localRange
+    int gs[3], ge[3];      // logicalRange (ghost box)
+    int off[3];            // storage origin
+    int ext[3];            // storage extents
+    int per[3];            // periodic in dim
+    int corner[3];         // corner-located in dim (mirror about the boundary node)
+    int nobc[3][2];        // side has no fill (undefined bc) in dim
+    int ngl[3][2];         // ghost layers per side
+    GhostRule rule[3][2][4];  // up to 4 layers per side
+};
+
+int build_ghost_info(const opf_field* f, GhostInfo* gi);
+int launch_ghost_fill(const opf_field* f, void* data);
+
+// device program cache
+int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev);
+
+}  // namespace opf

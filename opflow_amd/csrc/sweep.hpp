@@ -1,0 +1,68 @@
+// Stack program format shared by the lowering (host) and the sweep kernels.
+#pragma once
+#include <array>
+#include <vector>
+
+#include "opf_internal.hpp"
+
+namespace opf {
+
+constexpr int SWEEP_MAX_FIELDS = 8;
+constexpr int SWEEP_MAX_DEPTH = 16;
+
+enum InsnOp {
+    I_LD = 0,     // push field[a] at linear offset b from the evaluation cell
+    I_LDU = 1,    // push the unknown (pad mode) at packed offset c
+    I_CONST = 2,  // push imm
+    I_DX = 3,     // push mesh dx of dim a at (cell index + b)
+    I_X = 4,      // push mesh x of dim a at (cell index + b)
+    I_ADD = 5,
+    I_SUB = 6,
+    I_MUL = 7,
+    I_DIV = 8,
+    I_NEG = 9,
+    I_WENO_UP = 10,
+    I_WENO_DOWN = 11
+};
+
+struct Insn {
+    int op, a, b, c;
+    double imm;
+};
+
+struct LoweredProgram {
+    std::vector<Insn> code;
+    int maxdepth = 0;
+    int omin[SWEEP_MAX_FIELDS][3], omax[SWEEP_MAX_FIELDS][3];
+    int tmin[3], tmax[3];
+    int loc[3];
+};
+
+struct FieldArg {
+    const void* ptr;
+    int off0, off1, off2;
+    int e0, e1;
+};
+
+struct SweepArgs {
+    FieldArg f[SWEEP_MAX_FIELDS];
+    const void* x[3];
+    const void* dx[3];
+    int morg[3];
+    int s[3], n[3];
+    void* out;
+    const void* old;
+    int oo[3];
+    int oe0, oe1;
+    int op;
+    const void* code;
+    int ncode;
+};
+
+int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
+               const opf_mesh* mesh, int unknown_slot, LoweredProgram* out);
+int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
+                 const int n[3]);
+int mesh_arrays(const opf_mesh* m, int dtype, const void* x[3], const void* dx[3]);
+
+}  // namespace opf

@@ -1,0 +1,203 @@
+"""Implicit-solve front-end: StructSolverParams, Solve, makeEqnSolveHandler
+(Equation/UnifiedSolve.hpp:36-67, Equation/HYPREEqnSolveHandler.hpp:44-48),
+plus direct multigrid and serving handles over the C ABI."""
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass
+class StructSolverParams:
+    """StructSolverParamsBase (Solvers/Struct/StructSolver.hpp:39-51) + engine knobs."""
+    type: int = 6  # StructSolverType::GMRES (UnifiedSolve.hpp:36 default)
+    tol: Optional[float] = None
+    maxIter: Optional[int] = None
+    staticMat: bool = False
+    pinValue: bool = False
+    precision: str = "fp64"  # "fp64" | "mixed"
+    numPreRelax: Optional[int] = None
+    numPostRelax: Optional[int] = None
+    verbose: bool = False
+
+
+@dataclass
+class EqnSolveState:  # Equation/EqnSolveHandler.hpp:17-25
+    niter: int = 0
+    relerr: float = 0.0
+    abserr: float = 0.0
+    nrefine: int = 0
+    converged: bool = False
+
+
+def _params(params, precParams):
+    p = L.SolveParams()
+    p.solver = int(params.type if params else 6)
+    p.precond = int(precParams.type if precParams else 0)
+    p.tol = float(params.tol) if params and params.tol is not None else 0.0
+    p.max_iter = int(params.maxIter) if params and params.maxIter is not None else 0
+    p.static_mat = int(bool(params and params.staticMat))
+    p.pin_value = int(bool(params and params.pinValue))
+    p.precision = 1 if params and params.precision == "mixed" else 0
+    src = precParams or params
+    p.pre_smooth = int(src.numPreRelax) if src and src.numPreRelax else 0
+    p.post_smooth = int(src.numPostRelax) if src and src.numPostRelax else 0
+    p.verbose = int(bool(params and params.verbose))
+    return p
+
+
+class PrecondStructSolver:
+    """PrecondStructSolver<type, pType>(params, precParams) (StructSolverPrecond.hpp:19-71)."""
+
+    def __init__(self, params=None, precParams=None):
+        self.params = params or StructSolverParams()
+        self.precParams = precParams
+
+
+class _Handler:
+    """EqnSolveHandler whose solve() runs on the device (EqnSolveHandler.hpp:28-33)."""
+
+    def __init__(self, getter, target, solver):
+        from . import Equation, _node_array  # noqa: F401
+        from . import Expr
+        eqn = getter(target)
+        if not hasattr(eqn, "lhs"):
+            raise TypeError("equation getter must return `lhs == rhs`")
+        # one combined post-order list holding both sides
+        both = Expr(3, eqn.lhs, eqn.rhs)  # lhs - rhs, root's children are the two sides
+        nodes, fields, root = both.flatten(unknown=target)
+        lhs_root, rhs_root = nodes[root][1], nodes[root][2]
+        nodes = nodes[:-1]
+        self.fields = [target] + [f for f in fields if f is not target]
+        slot = {id(f): i for i, f in enumerate(self.fields)}
+        remap = []
+        for (k, a, b, d, v) in nodes:
+            if k == 0:
+                a = slot[id(fields[a])]
+            remap.append((k, a, b, d, v))
+        from . import _node_array as na
+        self._nodes = na(remap)
+        fh = (C.c_void_p * len(self.fields))(*[f.h.value for f in self.fields])
+        self.params = _params(solver.params, solver.precParams)
+        h = C.c_void_p()
+        lib = target.lib
+        L.check(lib.opf_eqn_create(self._nodes, len(remap), lhs_root, rhs_root, fh, len(self.fields), target.h,
+                                   C.byref(self.params), C.byref(h)))
+        self.h, self.lib = h, lib
+
+    def solve(self):
+        st = L.SolveState()
+        L.check(self.lib.opf_eqn_solve(self.h, C.byref(st)))
+        return EqnSolveState(st.niter, st.relerr, st.abserr, st.nrefine, bool(st.converged))
+
+    def close(self):
+        if self.h:
+            self.lib.opf_eqn_destroy(self.h)
+            self.h = None
+
+
+def makeEqnSolveHandler(getter, target, solver):
+    return _Handler(getter, target, solver)
+
+
+def Solve(getter, target, params=None, precParams=None):
+    """Solve(func, target, params, precParams) (UnifiedSolve.hpp:36-43)."""
+    h = _Handler(getter, target, PrecondStructSolver(params, precParams))
+    try:
+        return h.solve()
+    finally:
+        h.close()
+
+
+def poisson_desc(dim, n, h, bc, sigma=0.0):
+    d = L.PoissonDesc()
+    d.dim = dim
+    for i in range(3):
+        d.n[i] = n[i] if i < dim else 1
+        d.h[i] = h[i] if i < dim else 1.0
+        for s in range(2):
+            d.bc[i][s] = int(bc[i][s]) if i < dim else 0
+    d.sigma = sigma
+    return d
+
+
+class PoissonGMG:
+    """Direct matrix-free GMG-PCG on device vectors (opf_gmg_*)."""
+
+    def __init__(self, ctx, desc, precision="fp64"):
+        self.lib, self.ctx, self.desc = ctx.lib, ctx, desc
+        h = C.c_void_p()
+        L.check(self.lib.opf_gmg_create(ctx.h, C.byref(desc), 1 if precision == "mixed" else 0, C.byref(h)))
+        self.h = h
+
+    def solve(self, b_ptr, x_ptr, tol=1e-10, max_iter=200, pre=0, post=0):
+        p = L.SolveParams()
+        p.solver, p.precond, p.tol, p.max_iter, p.pre_smooth, p.post_smooth = 5, 3, tol, max_iter, pre, post
+        st = L.SolveState()
+        L.check(self.lib.opf_gmg_solve(self.h, b_ptr, x_ptr, C.byref(p), C.byref(st)))
+        return EqnSolveState(st.niter, st.relerr, st.abserr, st.nrefine, bool(st.converged))
+
+    def apply(self, x_ptr, y_ptr):
+        L.check(self.lib.opf_gmg_apply(self.h, x_ptr, y_ptr))
+
+    def close(self):
+        if self.h:
+            self.lib.opf_gmg_destroy(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching Poisson/heat solve server (opf_srv_*)."""
+
+    def __init__(self, ctx, mem_budget, max_inflight=8, precision="mixed", batched=True, tol=1e-8, max_iter=100):
+        self.lib = ctx.lib
+        cfg = L.SrvConfig()
+        cfg.mem_budget, cfg.max_inflight = int(mem_budget), max_inflight
+        cfg.precision, cfg.batched, cfg.tol, cfg.max_iter = (1 if precision == "mixed" else 0), int(batched), tol, \
+            max_iter
+        h = C.c_void_p()
+        L.check(self.lib.opf_srv_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def submit(self, desc, seed, nbumps, noise=0.0, rhs=None):
+        r = L.SrvRequest()
+        r.desc, r.seed, r.nbumps, r.noise = desc, seed, nbumps, noise
+        self._keep = None
+        if rhs is not None:
+            self._keep = np.ascontiguousarray(rhs, dtype=np.float64)
+            r.rhs = self._keep.ctypes.data_as(L._DP)
+        i = C.c_uint64()
+        L.check(self.lib.opf_srv_submit(self.h, C.byref(r), C.byref(i)))
+        return i.value
+
+    def step(self):
+        n = C.c_int()
+        L.check(self.lib.opf_srv_step(self.h, C.byref(n)))
+        return n.value
+
+    def poll(self, maxn=64):
+        arr = (L.SrvResult * maxn)()
+        n = C.c_int()
+        L.check(self.lib.opf_srv_poll(self.h, arr, maxn, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def fetch(self, rid, n):
+        out = np.zeros(n)
+        L.check(self.lib.opf_srv_fetch(self.h, C.c_uint64(rid), out.ctypes.data_as(L._DP), n))
+        return out
+
+    def pending(self):
+        q, f = C.c_int(), C.c_int()
+        L.check(self.lib.opf_srv_pending(self.h, C.byref(q), C.byref(f)))
+        return q.value, f.value
+
+    def clock(self):
+        return self.lib.opf_srv_clock(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.opf_srv_destroy(self.h)
+            self.h = None

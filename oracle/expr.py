@@ -1,0 +1,175 @@
+"""numpy restatement of OpFlow's lazy expression evaluation -- TEST ONLY.
+
+evaluate() computes an expression tree (the post-order node list of the C
+ABI) over an index box the way Expression::evalAtImpl_final recurses
+(Expr/Expression.hpp:51-65): each operator evaluates its argument at shifted
+indices and combines them with the mesh spacings in the reference's operation
+order. numpy float64 elementwise ops round exactly like the reference's
+scalar C++ (no FMA contraction), so results are comparable bit for bit.
+"""
+import numpy as np
+
+FIELD, CONST, ADD, SUB, MUL, DIV, NEG = 0, 1, 2, 3, 4, 5, 6
+D2C, D1C, D1UP, D1DOWN, WENO_UP, WENO_DOWN, INTP_C2N, INTP_N2C, UNKNOWN = 7, 8, 9, 10, 11, 12, 13, 14, 15
+CORNER, CENTER = 0, 1
+
+
+def _locs(nodes, fields, dim):
+    locs = []
+    for (k, a, b, d, v) in nodes:
+        if k == FIELD:
+            desc = fields[a][1]
+            locs.append([desc.loc[q] for q in range(dim)])
+        elif k == CONST:
+            locs.append(None)
+        elif k in (ADD, SUB, MUL, DIV):
+            locs.append(list(locs[a]) if locs[a] is not None else (list(locs[b]) if locs[b] is not None else None))
+        else:
+            l_ = list(locs[a]) if locs[a] is not None else None
+            if l_ is not None:
+                if k == D1C:
+                    l_[d] = CORNER if l_[d] == CENTER else CENTER
+                if k == INTP_C2N:
+                    l_[d] = CORNER
+                if k == INTP_N2C:
+                    l_[d] = CENTER
+            locs.append(l_)
+    return locs
+
+
+def _weno(v, h, up):
+    """D1WENO53Upwind::kernel / D1WENO53Downwind::kernel (D1WENO53Upwind.hpp:133-151,
+    D1WENO53Downwind.hpp:136-154); v = six values in index order."""
+    if up:
+        pm2, pm1, p0, pp1, pp2, pp3 = v
+        d1, d2, d3, d4, d5 = (pp3 - pp2) / h, (pp2 - pp1) / h, (pp1 - p0) / h, (p0 - pm1) / h, (pm1 - pm2) / h
+    else:
+        pm3, pm2, pm1, p0, pp1, pp2 = v
+        d1, d2, d3, d4, d5 = (pm2 - pm3) / h, (pm1 - pm2) / h, (p0 - pm1) / h, (pp1 - p0) / h, (pp2 - pp1) / h
+    ddx1 = d1 / 3. - 7. * d2 / 6. + 11. * d3 / 6.
+    ddx2 = -d2 / 6. + 5 * d3 / 6 + d4 / 3
+    ddx3 = d3 / 3 + 5 * d4 / 6 - d5 / 6
+    def pow2(t):  # Math::pow2 (Math/Function/Numeric.hpp:104)
+        return t * t
+    s1 = 13. / 12. * pow2(d1 - 2 * d2 + d3) + pow2(d1 - 4 * d2 + 3 * d3) / 4
+    s2 = 13. / 12. * pow2(d2 - 2 * d3 + d4) + pow2(d2 - d4) / 4.
+    s3 = 13. / 12. * pow2(d3 - 2 * d4 + d5) + pow2(3 * d3 - 4 * d4 + d5) / 4.
+    m = np.maximum(np.maximum(np.maximum(np.maximum(d1 * d1, d2 * d2), d3 * d3), d4 * d4), d5 * d5)
+    eps = 1e-6 * m + 1e-99
+    a1 = .1 / pow2(s1 + eps)
+    a2 = .6 / pow2(s2 + eps)
+    a3 = .3 / pow2(s3 + eps)
+    w1 = a1 / (a1 + a2 + a3)
+    w2 = a2 / (a1 + a2 + a3)
+    w3 = a3 / (a1 + a2 + a3)
+    return w1 * ddx1 + w2 * ddx2 + w3 * ddx3
+
+
+def evaluate(nodes, root, fields, mesh, lo, hi):
+    """Value of node `root` over the index box [lo, hi) (lists per dim).
+
+    fields: list of (padded numpy array in (z,y,x) order, FieldDesc) per slot.
+    Returns a float64 array of shape (hi-lo) reversed (z,y,x)."""
+    dim = mesh.dim
+    locs = _locs(nodes, fields, dim)
+
+    def mesh_col(d, arr1d):
+        shape = [1] * dim
+        shape[dim - 1 - d] = arr1d.shape[0]
+        return arr1d.reshape(shape)
+
+    def DX(d, lo_, hi_, k):
+        return mesh_col(d, mesh.dx(d, np.arange(lo_[d], hi_[d]) + k))
+
+    def X(d, lo_, hi_, k):
+        return mesh_col(d, mesh.x(d, np.arange(lo_[d], hi_[d]) + k))
+
+    def sh(lo_, hi_, d, k):
+        l2, h2 = list(lo_), list(hi_)
+        l2[d] += k
+        h2[d] += k
+        return l2, h2
+
+    def ev(n, lo_, hi_):
+        k, a, b, d, v = nodes[n]
+        if k == FIELD:
+            arr, desc = fields[a]
+            sl = tuple(slice(lo_[q] - desc.offset[q], hi_[q] - desc.offset[q]) for q in reversed(range(dim)))
+            for q in range(dim):
+                if lo_[q] < desc.offset[q] or hi_[q] > desc.offset[q] + desc.extent[q]:
+                    raise IndexError(f"oracle read of slot {a} outside storage in dim {q}")
+            return np.asarray(arr, dtype=np.float64)[sl]
+        if k == CONST:
+            return np.float64(v)
+        if k == ADD:
+            return ev(a, lo_, hi_) + ev(b, lo_, hi_)
+        if k == SUB:
+            return ev(a, lo_, hi_) - ev(b, lo_, hi_)
+        if k == MUL:
+            return ev(a, lo_, hi_) * ev(b, lo_, hi_)
+        if k == DIV:
+            return ev(a, lo_, hi_) / ev(b, lo_, hi_)
+        if k == NEG:
+            return -ev(a, lo_, hi_)
+        corner = locs[a] is not None and locs[a][d] == CORNER
+        if k == D2C:  # D2SecondOrderCentered::eval (D2SecondOrderCentered.hpp:160-171)
+            l_ = ev(a, *sh(lo_, hi_, d, -1))
+            c = ev(a, lo_, hi_)
+            r = ev(a, *sh(lo_, hi_, d, +1))
+            dxl = DX(d, lo_, hi_, -1) if corner else (DX(d, lo_, hi_, -1) + DX(d, lo_, hi_, 0)) * 0.5
+            dxr = DX(d, lo_, hi_, 0) if corner else (DX(d, lo_, hi_, 0) + DX(d, lo_, hi_, 1)) * 0.5
+            dxc = (dxl + dxr) * 0.5
+            return ((r - c) / dxr - (c - l_) / dxl) / dxc
+        if k == D1C:  # D1FirstOrderCentered::eval (D1FirstOrderCentered.hpp:30-37)
+            if not corner:
+                return (ev(a, lo_, hi_) - ev(a, *sh(lo_, hi_, d, -1))) / (DX(d, lo_, hi_, -1) + DX(d, lo_, hi_, 0)) * 2
+            return (ev(a, *sh(lo_, hi_, d, 1)) - ev(a, lo_, hi_)) / DX(d, lo_, hi_, 0)
+        if k == D1UP:  # D1FirstOrderBiasedUpwind::eval (D1FirstOrderBiasedUpwind.hpp:53-58)
+            den = DX(d, lo_, hi_, 0) if corner else (DX(d, lo_, hi_, 0) + DX(d, lo_, hi_, 1)) * 0.5
+            return (ev(a, *sh(lo_, hi_, d, 1)) - ev(a, lo_, hi_)) / den
+        if k == D1DOWN:  # D1FirstOrderBiasedDownwind::eval (D1FirstOrderBiasedDownwind.hpp:52-57)
+            den = DX(d, lo_, hi_, -1) if corner else (DX(d, lo_, hi_, -1) + DX(d, lo_, hi_, 0)) * 0.5
+            return (ev(a, lo_, hi_) - ev(a, *sh(lo_, hi_, d, -1))) / den
+        if k in (WENO_UP, WENO_DOWN):
+            k0 = -2 if k == WENO_UP else -3
+            vals = [ev(a, *sh(lo_, hi_, d, k0 + t)) for t in range(6)]
+            return _weno(vals, DX(d, lo_, hi_, 0), k == WENO_UP)
+        if k == INTP_C2N:  # D1Linear<d,Cen2Cor>::eval + Interpolator1D::intp (D1Linear.hpp:34-42)
+            x1 = X(d, lo_, hi_, -1) + 0.5 * DX(d, lo_, hi_, -1)
+            x2 = X(d, lo_, hi_, 0) + 0.5 * DX(d, lo_, hi_, 0)
+            x = X(d, lo_, hi_, 0)
+            y1 = ev(a, *sh(lo_, hi_, d, -1))
+            y2 = ev(a, lo_, hi_)
+            A, B = x1 - x, x2 - x
+            return (A * y2 - B * y1) / (A - B)
+        if k == INTP_N2C:  # D1Linear<d,Cor2Cen>: Math::mid (Interpolator.hpp:83)
+            return (ev(a, lo_, hi_) + ev(a, *sh(lo_, hi_, d, 1))) * 0.5
+        raise ValueError(f"node kind {k}")
+
+    out = ev(root, list(lo), list(hi))
+    shape = tuple(hi[q] - lo[q] for q in reversed(range(dim)))
+    return np.broadcast_to(out, shape).astype(np.float64)
+
+
+def assign(nodes, root, fields, mesh, out_arr, out_desc, op=0, update_padding=None):
+    """OpFlow's `out (op)= expr`: evaluate over assignable∩local of out, write, then
+    updatePadding via the given callable (FieldAssigner.hpp:26-86)."""
+    dim = mesh.dim
+    lo = [max(out_desc.assign_start[q], out_desc.local_start[q]) for q in range(dim)]
+    hi = [min(out_desc.assign_end[q], out_desc.local_end[q]) for q in range(dim)]
+    val = evaluate(nodes, root, fields, mesh, lo, hi)
+    res = np.array(out_arr, dtype=np.float64, copy=True)
+    sl = tuple(slice(lo[q] - out_desc.offset[q], hi[q] - out_desc.offset[q]) for q in reversed(range(dim)))
+    if op == 0:
+        res[sl] = val
+    elif op == 1:
+        res[sl] = res[sl] + val
+    elif op == 2:
+        res[sl] = res[sl] - val
+    elif op == 3:
+        res[sl] = res[sl] * val
+    elif op == 4:
+        res[sl] = res[sl] / val
+    if update_padding is not None:
+        res = update_padding(res)
+    return res
