@@ -268,6 +268,10 @@ int build_ghost_info(const opf_field* f, GhostInfo* gi) {
         for (int s = 0; s < 2; ++s) {
             int nl = s == 0 ? d.local_start[i] - gi->gs[i] : gi->ge[i] - d.local_end[i];
             OPF_CHECK(nl <= 4, OPF_ERR_UNSUPPORTED, "more than 4 ghost layers");
+            // every mirror / wrap source must be an interior index
+            const int L = d.local_end[i] - d.local_start[i];
+            OPF_CHECK(nl <= (corner && !gi->per[i] ? L - 1 : L), OPF_ERR_UNSUPPORTED,
+                      "ghost width %d too large for %d interior points in dim %d", nl, L, i);
             gi->ngl[i][s] = nl;
             int t = d.bc_type[i][s];
             // step-1 fills only when localRange touches accessibleRange and bc is not periodic
