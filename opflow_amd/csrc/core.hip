@@ -6,6 +6,7 @@
 #include <cstdio>
 
 #include "opf_internal.hpp"
+#include "ghost_index.hpp"
 
 namespace opf {
 
@@ -359,58 +360,17 @@ __device__ __forceinline__ T apply_rule(const GhostRule& r, T v) {
 template <typename T>
 __global__ void ghost_kernel(T* __restrict__ data, GhostInfo gi, long long n0, long long n1, long long n2) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int k;
-    if (t < n0) k = 0;
-    else if (t < n0 + n1) { k = 1; t -= n0; }
-    else if (t < n0 + n1 + n2) { k = 2; t -= n0 + n1; }
-    else return;
-    int p[3];
-    // decompose t over (dims j<k: interior, dim k: ghost layers, dims j>k: ghost box)
-    for (int j = 0; j < 3; ++j) {
-        int len;
-        if (j < k) len = gi.le[j] - gi.ls[j];
-        else if (j == k) len = (gi.ls[j] - gi.gs[j]) + (gi.ge[j] - gi.le[j]);
-        else len = gi.ge[j] - gi.gs[j];
-        int r = (int)(t % len);
-        t /= len;
-        if (j < k) p[j] = gi.ls[j] + r;
-        else if (j == k) {
-            int nlo = gi.ls[j] - gi.gs[j];
-            p[j] = r < nlo ? gi.gs[j] + r : gi.le[j] + (r - nlo);
-        } else p[j] = gi.gs[j] + r;
+    int p[3], src[3], mir[3], lay[3], side[3];
+    int act = ghost_locate(gi, t, n0, n1, n2, p, src, mir, lay, side);
+    if (act == GHOST_SKIP) return;
+    if (act == GHOST_COPY) {
+        data[ghost_lin(gi, p)] = data[ghost_lin(gi, src)];
+        return;
     }
-    int src[3] = {p[0], p[1], p[2]};
-    bool wrapped = false;
-    for (int j = 0; j < 3; ++j) {
-        if (!gi.per[j]) continue;
-        int L = gi.le[j] - gi.ls[j];
-        if (src[j] < gi.ls[j]) { src[j] += L; wrapped = true; }
-        else if (src[j] >= gi.le[j]) { src[j] -= L; wrapped = true; }
-    }
-    int lay[3] = {-1, -1, -1}, side[3] = {0, 0, 0};
-    int mir[3] = {src[0], src[1], src[2]};
-    for (int j = 0; j < 3; ++j) {
-        if (src[j] < gi.ls[j]) { side[j] = 0; lay[j] = gi.ls[j] - 1 - src[j]; }
-        else if (src[j] >= gi.le[j]) { side[j] = 1; lay[j] = src[j] - gi.le[j]; }
-        else continue;
-        if (gi.nobc[j][side[j]]) {
-            // never filled by a BC pass; a periodic wrap copies whatever is stored there
-            if (wrapped) {
-                long long ls = (src[0] - gi.off[0]) + (long long)gi.ext[0] * ((src[1] - gi.off[1]) + (long long)gi.ext[1] * (src[2] - gi.off[2]));
-                long long lp = (p[0] - gi.off[0]) + (long long)gi.ext[0] * ((p[1] - gi.off[1]) + (long long)gi.ext[1] * (p[2] - gi.off[2]));
-                data[lp] = data[ls];
-            }
-            return;
-        }
-        if (side[j] == 0) mir[j] = gi.corner[j] ? 2 * gi.ls[j] - src[j] : 2 * gi.ls[j] - 1 - src[j];
-        else mir[j] = gi.corner[j] ? 2 * gi.le[j] - 2 - src[j] : 2 * gi.le[j] - 1 - src[j];
-    }
-    long long lm = (mir[0] - gi.off[0]) + (long long)gi.ext[0] * ((mir[1] - gi.off[1]) + (long long)gi.ext[1] * (mir[2] - gi.off[2]));
-    T v = data[lm];
+    T v = data[ghost_lin(gi, mir)];
     for (int j = 0; j < 3; ++j)
         if (lay[j] >= 0) v = apply_rule<T>(gi.rule[j][side[j]][lay[j]], v);
-    long long lp = (p[0] - gi.off[0]) + (long long)gi.ext[0] * ((p[1] - gi.off[1]) + (long long)gi.ext[1] * (p[2] - gi.off[2]));
-    data[lp] = v;
+    data[ghost_lin(gi, p)] = v;
 }
 
 // step 0 of updatePaddingImpl_final: Dirichlet boundary nodes of corner-located dims

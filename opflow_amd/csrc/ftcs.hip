@@ -15,6 +15,12 @@ using namespace opf;
 
 namespace {
 
+// storage index of cell q (interior origin 0, one ghost layer); z is padded only in 3-D
+template <int DIM>
+__host__ __device__ __forceinline__ long long ftcs_index(int e0, int e1, const int q[3]) {
+    return (q[0] + 1) + (long long)e0 * ((DIM >= 2 ? q[1] + 1 : 0) + (long long)e1 * (DIM == 3 ? q[2] + 1 : 0));
+}
+
 template <typename T>
 struct FtcsArgs {
     int n[3];       // interior cells
@@ -73,8 +79,7 @@ __device__ __forceinline__ void write_images(T* __restrict__ out, const FtcsArgs
             if (!a.per[d]) w = rule(a, d, s, w);
         }
         if (!ok) continue;
-        long long l = (q[0] + 1) + (long long)a.e0 * ((q[1] + 1) + (long long)a.e1 * (q[2] + 1));
-        out[l] = w;
+        out[ftcs_index<DIM>(a.e0, a.e1, q)] = w;
     }
 }
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/opflow_hip.h"
+#include "ghost_index.hpp"
 
 namespace opf {
 
@@ -92,27 +93,6 @@ struct opf_field {
 };
 
 namespace opf {
-
-// Ghost-filling rule of one (dim, side, layer): value = f(mirror value).
-// type: 0 none (no bc), 1 dirichlet-intp, 2 neumann, 3 symm, 4 asymm, 5 periodic
-struct GhostRule {
-    int type;
-    double c1, c2, c3;  // dirichlet: (c1*v - c2)/c3 ; neumann: v + c1
-};
-
-// Per-field ghost table consumed by the device ghost kernels.
-struct GhostInfo {
-    int dim;
-    int ls[3], le[3];      // localRange
-    int gs[3], ge[3];      // logicalRange (ghost box)
-    int off[3];            // storage origin
-    int ext[3];            // storage extents
-    int per[3];            // periodic in dim
-    int corner[3];         // corner-located in dim (mirror about the boundary node)
-    int nobc[3][2];        // side has no fill (undefined bc) in dim
-    int ngl[3][2];         // ghost layers per side
-    GhostRule rule[3][2][4];  // up to 4 layers per side
-};
 
 int build_ghost_info(const opf_field* f, GhostInfo* gi);
 int launch_ghost_fill(const opf_field* f, void* data);
