@@ -183,6 +183,12 @@ int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);
  * deterministic (fixed two-pass tree). range = [rs, re) in field indices.   */
 int opf_reduce(const opf_field* f, int red, const int rs[3], const int re[3], double* out);
 
+/* ---- raw device buffers (for the direct solver API) ---------------------- */
+int opf_buf_create(opf_ctx* ctx, size_t nbytes, void** dptr);
+int opf_buf_destroy(opf_ctx* ctx, void* dptr);
+int opf_buf_upload(opf_ctx* ctx, void* dptr, const void* host, size_t nbytes);
+int opf_buf_download(opf_ctx* ctx, const void* dptr, void* host, size_t nbytes);
+
 /* ---- synthetic inputs (host-only) ----------------------------------------- */
 /* SplitMix64 stream, u = (x >> 11) * 2^-53; lo + (hi-lo) u. */
 int opf_gen_uniform(uint64_t seed, size_t n, double lo, double hi, double* out);

@@ -356,6 +356,39 @@ class CartesianField(Expr):
     def copy_from(self, other):
         L.check(self.lib.opf_field_copy(self.h, other.h))
 
+    def clone(self, name=""):
+        """Copy construction `auto q = p;` (CartesianField.hpp:57-68): same descriptor, copied data."""
+        f = CartesianField(self.mesh, self.desc, name or self.name)
+        f.copy_from(self)
+        return f
+
+    def cell_coords(self):
+        """Coordinates initBy passes per dim over assignable∩local: x (corner) or x + .5 dx (centre)."""
+        d = self.desc
+        out = []
+        for q in range(d.dim):
+            s_, e_ = max(d.assign_start[q], d.local_start[q]), min(d.assign_end[q], d.local_end[q])
+            idx = np.arange(s_, e_)
+            x = self.mesh.xs[q][idx + self.mesh.pad]
+            if d.loc[q] == LocOnMesh.Center:
+                x = x + .5 * self.mesh.dxs[q][idx + self.mesh.pad]
+            out.append((s_, e_, x))
+        return out
+
+    def initBy(self, fn):
+        """initBy(f) (CartesianField.hpp:283-296): f(coords) over assignable∩local, then updatePadding.
+        fn receives a list of broadcastable coordinate arrays (x0, x1[, x2]) and returns values."""
+        d = self.desc
+        cc = self.cell_coords()
+        grids = np.meshgrid(*[c[2] for c in reversed(cc)], indexing="ij")[::-1]
+        vals = np.asarray(fn(grids), dtype=np.float64)
+        arr = self.numpy().astype(np.float64)
+        sl = tuple(slice(cc[q][0] - d.offset[q], cc[q][1] - d.offset[q]) for q in reversed(range(d.dim)))
+        arr[sl] = vals
+        self.upload(arr)
+        self.updatePadding()
+        return self
+
     def ftcs(self, coef, nsteps):
         """Fused fast path of u = u + coef*(d2x(u)+d2y(u)[+d2z(u)]) repeated nsteps times."""
         L.check(self.lib.opf_ftcs(self.h, float(coef), int(nsteps)))
@@ -469,4 +502,4 @@ def gen_uniform(seed, n, lo=0.0, hi=1.0):
 
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
-                    PrecondStructSolver, PoissonGMG, Server)
+                    PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc)

@@ -117,6 +117,10 @@ SIGNATURES = {
     "opf_ftcs": (_I, [_P, _D, _I]),
     "opf_ftcs_blocked": (_I, [_P, _D, _I, _I]),
     "opf_reduce": (_I, [_P, _I, _IP, _IP, _DP]),
+    "opf_buf_create": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
+    "opf_buf_destroy": (_I, [_P, _P]),
+    "opf_buf_upload": (_I, [_P, _P, _P, C.c_size_t]),
+    "opf_buf_download": (_I, [_P, _P, _P, C.c_size_t]),
     "opf_gen_uniform": (_I, [C.c_uint64, C.c_size_t, _D, _D, _DP]),
     "opf_gen_bumps": (_I, [_P, C.POINTER(FieldDesc), C.c_uint64, _I, _D, _DP]),
     "opf_eqn_create": (_I, [C.POINTER(ExprNode), _I, _I, _I, C.POINTER(_P), _I, _P, C.POINTER(SolveParams),

@@ -96,6 +96,33 @@ int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev) {
 }
 }  // namespace opf
 
+// ---------------------------------------------------------------- raw buffers
+extern "C" int opf_buf_create(opf_ctx* c, size_t nbytes, void** dptr) {
+    OPF_CHECK(c && dptr, OPF_ERR_ARG, "opf_buf_create: null arg");
+    hipError_t e = hipMalloc(dptr, nbytes ? nbytes : 1);
+    OPF_CHECK(e == hipSuccess, OPF_ERR_NOMEM, "opf_buf_create: %s", hipGetErrorString(e));
+    OPF_HIP(hipMemsetAsync(*dptr, 0, nbytes, c->stream));
+    return OPF_OK;
+}
+extern "C" int opf_buf_destroy(opf_ctx* c, void* dptr) {
+    OPF_CHECK(c, OPF_ERR_ARG, "null ctx");
+    OPF_HIP(hipStreamSynchronize(c->stream));
+    if (dptr) OPF_HIP(hipFree(dptr));
+    return OPF_OK;
+}
+extern "C" int opf_buf_upload(opf_ctx* c, void* dptr, const void* host, size_t nbytes) {
+    OPF_CHECK(c && dptr && host, OPF_ERR_ARG, "opf_buf_upload: null arg");
+    OPF_HIP(hipMemcpyAsync(dptr, host, nbytes, hipMemcpyHostToDevice, c->stream));
+    OPF_HIP(hipStreamSynchronize(c->stream));
+    return OPF_OK;
+}
+extern "C" int opf_buf_download(opf_ctx* c, const void* dptr, void* host, size_t nbytes) {
+    OPF_CHECK(c && dptr && host, OPF_ERR_ARG, "opf_buf_download: null arg");
+    OPF_HIP(hipMemcpyAsync(host, dptr, nbytes, hipMemcpyDeviceToHost, c->stream));
+    OPF_HIP(hipStreamSynchronize(c->stream));
+    return OPF_OK;
+}
+
 // ---------------------------------------------------------------- mesh
 extern "C" int opf_mesh_uniform_arrays(int nnodes, double lo, double hi, int pad, double* x, double* dx) {
     // MeshBuilder::set1DMesh(min, max, k) + setExtMesh(Symm) with range start 0

@@ -1,0 +1,943 @@
+// Matrix-free geometric multigrid preconditioned flexible CG on gfx950.
+//
+// Levels are cell-centred with coarsening by 2 in every dimension;
+// restriction averages the 2^d children's residuals, prolongation is
+// (bi/tri)linear with BC-consistent coarse ghosts, the smoother is weighted
+// Jacobi, the coarsest level is solved with a dense FP64 inverse. The Krylov
+// method is flexible CG (Polak-Ribiere beta), so the slightly non-symmetric
+// V-cycle (average restriction, linear prolongation) is harmless. All
+// reductions are two-pass and fixed-order (deterministic); the CG scalars
+// live in device memory so an iteration needs no host round trip.
+//
+// Mixed precision: the V-cycle and the inner CG run in FP32 with FP32
+// storage, FP64 is kept for the outer residual b - A x, the solution update
+// and the coarsest solve (iterative refinement).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "gmg.hpp"
+
+namespace opf {
+
+// ------------------------------------------------------------------ helpers
+__host__ __device__ __forceinline__ long long face_index(int d, int i, int j, int k, int n0, int n1) {
+    // face arrays of dim d have extents n + e_d
+    int f0 = n0 + (d == 0), f1 = n1 + (d == 1);
+    return i + (long long)f0 * (j + (long long)f1 * k);
+}
+
+template <typename T>
+__device__ __forceinline__ T face_coef(const GOp<T>& o, bool var, int d, int i, int j, int k) {
+    return var ? o.af[d][face_index(d, i, j, k, o.n0, o.n1)] : o.a[d];
+}
+
+// (A x)_c and the diagonal D_c; xv(ii,jj,kk) returns x at a cell.
+template <typename T, bool VAR, int DIM, typename XV>
+__device__ __forceinline__ T apply_op(const GOp<T>& o, int i, int j, int k, T xc, XV xv, T& D) {
+    T acc = 0;
+    D = VAR ? o.sg[i + (long long)o.n0 * (j + (long long)o.n1 * k)] : o.sigma;
+    int c[3] = {i, j, k};
+    int n[3] = {o.n0, o.n1, o.n2};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+        hi[d] += 1;
+        T alo = face_coef<T>(o, VAR, d, lo[0], lo[1], lo[2]);
+        T ahi = face_coef<T>(o, VAR, d, hi[0], hi[1], hi[2]);
+        if (c[d] > 0) {
+            int q[3] = {i, j, k};
+            q[d] -= 1;
+            D += alo;
+            acc += alo * xv(q[0], q[1], q[2]);
+        } else if (o.per[d]) {
+            int q[3] = {i, j, k};
+            q[d] = n[d] - 1;
+            D += alo;
+            acc += alo * xv(q[0], q[1], q[2]);
+        } else {
+            D += alo * o.bfac[d][0];
+        }
+        if (c[d] < n[d] - 1) {
+            int q[3] = {i, j, k};
+            q[d] += 1;
+            D += ahi;
+            acc += ahi * xv(q[0], q[1], q[2]);
+        } else if (o.per[d]) {
+            int q[3] = {i, j, k};
+            q[d] = 0;
+            D += ahi;
+            acc += ahi * xv(q[0], q[1], q[2]);
+        } else {
+            D += ahi * o.bfac[d][1];
+        }
+    }
+    return D * xc - acc;
+}
+
+__host__ __device__ __forceinline__ void cell_of(long long t, int n0, int n1, int& i, int& j, int& k) {
+    i = (int)(t % n0);
+    long long r = t / n0;
+    j = (int)(r % n1);
+    k = (int)(r / n1);
+}
+
+__device__ double gmg_block_sum(double v) {
+    __shared__ double sh[8];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < nw; ++q) r += sh[q];
+        sh[0] = r;
+    }
+    __syncthreads();
+    r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ kernels
+// r = b - A x  (x may be null: zero), optional partial sum of r^2 (grid-stride,
+// fixed grid => deterministic). TO: output precision.
+template <typename T, typename TO, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
+                                               TO* __restrict__ r, double* __restrict__ part) {
+    long long N = (long long)o.n0 * o.n1 * o.n2;
+    double acc = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_of(t, o.n0, o.n1, i, j, k);
+        T v = b[t];
+        if (x) {
+            auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+            T D;
+            v = v - apply_op<T, VAR, DIM>(o, i, j, k, x[t], xv, D);
+        }
+        if (r) r[t] = (TO)v;
+        acc += (double)v * (double)v;
+    }
+    if (part) {
+        acc = gmg_block_sum(acc);
+        if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    }
+}
+
+// y = A x
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_apply(GOp<T> o, const T* __restrict__ x, T* __restrict__ y) {
+    long long N = (long long)o.n0 * o.n1 * o.n2;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    int i, j, k;
+    cell_of(t, o.n0, o.n1, i, j, k);
+    auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+    T D;
+    y[t] = apply_op<T, VAR, DIM>(o, i, j, k, x[t], xv, D);
+}
+
+// weighted Jacobi: xo = xi + w (b - A xi)/D ; ZERO: xo = w b / D
+template <typename T, bool VAR, int DIM, bool ZERO>
+__global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
+                                                T* __restrict__ xo, T w) {
+    long long N = (long long)o.n0 * o.n1 * o.n2;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    int i, j, k;
+    cell_of(t, o.n0, o.n1, i, j, k);
+    T D;
+    if (ZERO) {
+        auto zv = [&](int, int, int) { return (T)0; };
+        apply_op<T, VAR, DIM>(o, i, j, k, (T)0, zv, D);
+        xo[t] = w * b[t] / D;
+    } else {
+        auto xv = [&](int a, int bb, int c) { return xi[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+        T xc = xi[t];
+        T ax = apply_op<T, VAR, DIM>(o, i, j, k, xc, xv, D);
+        xo[t] = xc + w * (b[t] - ax) / D;
+    }
+}
+
+// coarse rhs = average over the 2^d children of the fine residual b - A x
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
+                                                  T* __restrict__ bc, int c0, int c1, int c2) {
+    long long NC = (long long)c0 * c1 * c2;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NC) return;
+    int I, J, K;
+    cell_of(t, c0, c1, I, J, K);
+    auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+    T s = 0;
+    const int nz = DIM == 3 ? 2 : 1;
+    for (int dk = 0; dk < nz; ++dk)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di) {
+                int i = 2 * I + di, j = 2 * J + dj, k = DIM == 3 ? 2 * K + dk : 0;
+                long long f = i + (long long)o.n0 * (j + (long long)o.n1 * k);
+                T D;
+                s += b[f] - apply_op<T, VAR, DIM>(o, i, j, k, x[f], xv, D);
+            }
+    bc[t] = s * (T)(DIM == 3 ? 0.125 : 0.25);
+}
+
+// x_f += P x_c (linear interpolation, coarse ghosts: periodic wrap or alpha * boundary value)
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, int n0, int n1, int n2, const T* __restrict__ xc,
+                                                 int c0, int c1, int c2, GOp<T> o) {
+    long long N = (long long)n0 * n1 * n2;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    int i, j, k;
+    cell_of(t, n0, n1, i, j, k);
+    int f[3] = {i, j, k}, cn[3] = {c0, c1, c2};
+    int par[3], oth[3];
+    for (int d = 0; d < 3; ++d) {
+        par[d] = f[d] >> 1;
+        oth[d] = (f[d] & 1) ? par[d] + 1 : par[d] - 1;
+    }
+    T s = 0;
+    const int ncomb = 1 << DIM;
+    for (int m = 0; m < ncomb; ++m) {
+        T w = 1;
+        T fac = 1;
+        int q[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; ++d) {
+            bool use_oth = (m >> d) & 1;
+            w *= use_oth ? (T)0.25 : (T)0.75;
+            int c = use_oth ? oth[d] : par[d];
+            if (c < 0 || c >= cn[d]) {
+                if (o.per[d]) c = c < 0 ? c + cn[d] : c - cn[d];
+                else {
+                    // ghost = alpha * boundary value; alpha = 1 - bfac
+                    fac *= (T)1 - o.bfac[d][c < 0 ? 0 : 1];
+                    c = par[d];
+                }
+            }
+            q[d] = c;
+        }
+        s += w * fac * xc[q[0] + (long long)c0 * (q[1] + (long long)c1 * q[2])];
+    }
+    xf[t] += s;
+}
+
+// coarsest level: x = Cinv b (FP64 inverse), one block
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, int n, const T* __restrict__ b,
+                                               T* __restrict__ x) {
+    __shared__ double sb[1024];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sb[q] = (double)b[q];
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        double s = 0;
+        const double* a = cinv + (long long)row * n;
+        for (int c = 0; c < n; ++c) s += a[c] * sb[c];
+        x[row] = (T)s;
+    }
+}
+
+// scalar slots
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_RR, S_BETA, S_ZQ, S_ZMEAN, S_BB, S_RR0, S_N };
+enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0 };
+
+__global__ void k_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ s, double invN,
+                         int singular) {
+    double v[3];
+    for (int q = 0; q < 3; ++q) {
+        double acc = 0;
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) acc += part[q * GMG_MAXB + t];
+        v[q] = gmg_block_sum(acc);
+        if (mode != FIN_DOT3 && mode != FIN_DOT3_FIRST) break;
+    }
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+        case FIN_PQ:
+            s[S_PQ] = v[0];
+            s[S_ALPHA] = v[0] != 0.0 ? s[S_RZ] / v[0] : 0.0;
+            break;
+        case FIN_RR: s[S_RR] = v[0]; break;
+        case FIN_BB: s[S_BB] = v[0]; break;
+        case FIN_RR0: s[S_RR0] = v[0]; break;
+        case FIN_DOT3_FIRST:
+            s[S_RZ] = v[0];
+            s[S_ZMEAN] = singular ? v[2] * invN : 0.0;
+            s[S_BETA] = 0.0;
+            break;
+        case FIN_DOT3: {
+            double zq = v[1];
+            s[S_ZQ] = zq;
+            s[S_BETA] = s[S_RZ] != 0.0 ? -s[S_ALPHA] * zq / s[S_RZ] : 0.0;
+            s[S_RZ] = v[0];
+            s[S_ZMEAN] = singular ? v[2] * invN : 0.0;
+        } break;
+    }
+}
+
+// p_new = (z - zmean) + beta p_old ; q = A p_new ; partial (p_new, q)
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ z, const T* __restrict__ pold,
+                                                T* __restrict__ pnew, T* __restrict__ q,
+                                                const double* __restrict__ s, double* __restrict__ part) {
+    long long N = (long long)o.n0 * o.n1 * o.n2;
+    const T beta = (T)s[S_BETA];
+    const T zm = (T)s[S_ZMEAN];
+    double acc = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_of(t, o.n0, o.n1, i, j, k);
+        auto pv = [&](int a, int bb, int c) {
+            long long l = a + (long long)o.n0 * (bb + (long long)o.n1 * c);
+            return (z[l] - zm) + beta * pold[l];
+        };
+        T pc = pv(i, j, k);
+        T D;
+        T qc = apply_op<T, VAR, DIM>(o, i, j, k, pc, pv, D);
+        pnew[t] = pc;
+        q[t] = qc;
+        acc += (double)pc * (double)qc;
+    }
+    acc = gmg_block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// x += alpha p ; r -= alpha q ; partial (r, r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpy2(long long N, T* __restrict__ x, const T* __restrict__ p,
+                                               T* __restrict__ r, const T* __restrict__ q,
+                                               const double* __restrict__ s, double* __restrict__ part) {
+    const T alpha = (T)s[S_ALPHA];
+    double acc = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        x[t] += alpha * p[t];
+        T rv = r[t] - alpha * q[t];
+        r[t] = rv;
+        acc += (double)rv * (double)rv;
+    }
+    acc = gmg_block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// partials of (r, z), (z, q), sum(z)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot3(long long N, const T* __restrict__ r, const T* __restrict__ z,
+                                              const T* __restrict__ q, double* __restrict__ part) {
+    double a = 0, b = 0, c = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        double zv = (double)z[t];
+        a += (double)r[t] * zv;
+        b += zv * (double)q[t];
+        c += zv;
+    }
+    a = gmg_block_sum(a);
+    b = gmg_block_sum(b);
+    c = gmg_block_sum(c);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[GMG_MAXB + blockIdx.x] = b;
+        part[2 * GMG_MAXB + blockIdx.x] = c;
+    }
+}
+
+// x (FP64) += e (FP32 or FP64)
+template <typename TE>
+__global__ void k_add_to64(long long N, double* __restrict__ x, const TE* __restrict__ e) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] += (double)e[t];
+}
+
+template <typename T>
+__global__ void k_fill(long long N, T* __restrict__ x, T v) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] = v;
+}
+
+// subtract a device scalar mean (singular problems): x -= s[slot] * scale
+template <typename T>
+__global__ void k_sub_mean(long long N, T* __restrict__ x, const double* __restrict__ s, int slot, double scale) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] -= (T)(s[slot] * scale);
+}
+
+template <typename T>
+__global__ void k_sum(long long N, const T* __restrict__ x, double* __restrict__ part) {
+    double a = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x)
+        a += (double)x[t];
+    a = gmg_block_sum(a);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// ------------------------------------------------------------------ host side
+static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+static inline int rblk(long long n) { return (int)std::min<long long>(GMG_MAXB, (n + 255) / 256); }
+
+template <typename T>
+static GOp<T> make_op(const Hierarchy& H, int l) {
+    const GLevelHost& L = H.host[l];
+    const GLevelDev& D = H.dev[l];
+    GOp<T> o{};
+    o.n0 = L.n[0];
+    o.n1 = L.n[1];
+    o.n2 = L.n[2];
+    o.dim = H.dim;
+    for (int d = 0; d < 3; ++d) {
+        o.per[d] = H.per[d];
+        o.a[d] = (T)L.a[d];
+        for (int s = 0; s < 2; ++s) o.bfac[d][s] = (T)(1.0 - H.alpha[d][s]);
+        if (std::is_same<T, double>::value) o.af[d] = (const T*)D.af64[d];
+        else o.af[d] = (const T*)D.af32[d];
+    }
+    o.sigma = (T)L.sigma;
+    o.sg = std::is_same<T, double>::value ? (const T*)D.sg64 : (const T*)D.sg32;
+    return o;
+}
+
+static long long face_count(const int n[3], int d) {
+    return (long long)(n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2));
+}
+
+// dense operator of a level (host, FP64), for the coarsest solve
+static void dense_matrix(const Hierarchy& H, const GLevelHost& L, std::vector<double>& A) {
+    int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    long long N = L.N;
+    A.assign(N * N, 0.0);
+    for (long long c = 0; c < N; ++c) {
+        int i = (int)(c % n0), j = (int)((c / n0) % n1), k = (int)(c / ((long long)n0 * n1));
+        int cc[3] = {i, j, k};
+        double D = L.var ? L.sg[c] : L.sigma;
+        for (int d = 0; d < H.dim; ++d) {
+            for (int s = 0; s < 2; ++s) {
+                int fc[3] = {i, j, k};
+                if (s == 1) fc[d] += 1;
+                double a = L.var ? L.af[d][face_index(d, fc[0], fc[1], fc[2], n0, n1)] : L.a[d];
+                int q[3] = {i, j, k};
+                q[d] += s == 0 ? -1 : 1;
+                bool inside = q[d] >= 0 && q[d] < L.n[d];
+                if (!inside && H.per[d]) {
+                    q[d] = q[d] < 0 ? L.n[d] - 1 : 0;
+                    inside = true;
+                }
+                if (inside) {
+                    D += a;
+                    long long nb = q[0] + (long long)n0 * (q[1] + (long long)n1 * q[2]);
+                    A[c * N + nb] -= a;
+                } else {
+                    D += a * (1.0 - H.alpha[d][s]);
+                }
+            }
+        }
+        (void)cc;
+        A[c * N + c] += D;
+    }
+}
+
+// Gauss-Jordan inverse with partial pivoting (small dense matrices)
+static bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)p * n + c])) p = r;
+        if (A[(size_t)p * n + c] == 0.0) return false;
+        if (p != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)p * n + k], A[(size_t)c * n + k]);
+                std::swap(inv[(size_t)p * n + k], inv[(size_t)c * n + k]);
+            }
+        double d = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) {
+            A[(size_t)c * n + k] *= d;
+            inv[(size_t)c * n + k] *= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+            }
+        }
+    }
+    return true;
+}
+
+static void coarsen(const Hierarchy& H, const GLevelHost& F, GLevelHost& C) {
+    for (int d = 0; d < 3; ++d) C.n[d] = d < H.dim ? F.n[d] / 2 : 1;
+    C.N = (long long)C.n[0] * C.n[1] * C.n[2];
+    C.var = F.var;
+    for (int d = 0; d < 3; ++d) C.a[d] = F.a[d] * 0.25;
+    C.sigma = F.sigma;
+    if (!F.var) return;
+    const int tw = H.dim == 3 ? 4 : 2;  // fine faces per coarse face
+    for (int d = 0; d < H.dim; ++d) {
+        C.af[d].assign(face_count(C.n, d), 0.0);
+        int e0 = C.n[0] + (d == 0), e1 = C.n[1] + (d == 1), e2 = C.n[2] + (d == 2);
+        for (int K = 0; K < e2; ++K)
+            for (int J = 0; J < e1; ++J)
+                for (int I = 0; I < e0; ++I) {
+                    double s = 0;
+                    int nz = (H.dim == 3 && d != 2) ? 2 : 1;
+                    int ny = d != 1 ? 2 : 1;
+                    int nx = d != 0 ? 2 : 1;
+                    for (int dk = 0; dk < nz; ++dk)
+                        for (int dj = 0; dj < ny; ++dj)
+                            for (int di = 0; di < nx; ++di) {
+                                int fi = 2 * I + di, fj = 2 * J + dj, fk = H.dim == 3 ? 2 * K + dk : 0;
+                                s += F.af[d][face_index(d, fi, fj, fk, F.n[0], F.n[1])];
+                            }
+                    C.af[d][face_index(d, I, J, K, C.n[0], C.n[1])] = s / tw * 0.25;
+                }
+    }
+    C.sg.assign(C.N, 0.0);
+    for (long long c = 0; c < C.N; ++c) {
+        int I = (int)(c % C.n[0]), J = (int)((c / C.n[0]) % C.n[1]), K = (int)(c / ((long long)C.n[0] * C.n[1]));
+        double s = 0;
+        int nz = H.dim == 3 ? 2 : 1;
+        for (int dk = 0; dk < nz; ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    long long f = (2 * I + di) + (long long)F.n[0] * ((2 * J + dj) + (long long)F.n[1] * (H.dim == 3 ? 2 * K + dk : 0));
+                    s += F.sg[f];
+                }
+        C.sg[c] = s / (1 << H.dim);
+    }
+}
+
+template <typename T>
+static int alloc(void** p, long long n, size_t& bytes) {
+    OPF_HIP(hipMalloc(p, sizeof(T) * std::max<long long>(n, 1)));
+    bytes += sizeof(T) * std::max<long long>(n, 1);
+    return OPF_OK;
+}
+
+int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
+              int precision) {
+    H.precision = precision;
+    H.var = fine.var;
+    for (int d = 0; d < 3; ++d) {
+        H.per[d] = per[d];
+        for (int s = 0; s < 2; ++s) H.alpha[d][s] = alpha[d][s];
+    }
+    H.dim = fine.n[2] > 1 ? 3 : 2;
+    if (fine.n[1] == 1) return (set_error("gmg: 1-D grids are not supported"), OPF_ERR_UNSUPPORTED);
+    H.omega = H.dim == 3 ? 6.0 / 7.0 : 0.8;
+    // singular: no Dirichlet-like face and no diagonal shift
+    bool shift = fine.var ? false : fine.sigma != 0.0;
+    if (fine.var)
+        for (double v : fine.sg) shift |= v != 0.0;
+    bool anchored = false;
+    for (int d = 0; d < H.dim; ++d)
+        for (int s = 0; s < 2; ++s) anchored |= !per[d] && alpha[d][s] != 1.0;
+    H.singular = !shift && !anchored;
+    H.host.clear();
+    H.host.push_back(fine);
+    for (;;) {
+        const GLevelHost& F = H.host.back();
+        bool can = F.N > 64;
+        for (int d = 0; d < H.dim; ++d) can &= (F.n[d] % 2 == 0) && F.n[d] >= 4;
+        if (!can) break;
+        GLevelHost C;
+        coarsen(H, F, C);
+        H.host.push_back(C);
+    }
+    const GLevelHost& Lc = H.host.back();
+    OPF_CHECK(Lc.N <= 1024, OPF_ERR_UNSUPPORTED,
+              "gmg: coarsest grid has %lld cells (> 1024); use sizes divisible by a power of two", Lc.N);
+    // coarsest dense inverse
+    std::vector<double> A, inv;
+    dense_matrix(H, Lc, A);
+    if (H.singular) {
+        double dm = 0;
+        for (long long c = 0; c < Lc.N; ++c) dm += A[c * Lc.N + c];
+        dm /= (double)Lc.N * Lc.N;
+        for (auto& v : A) v += dm;
+    }
+    OPF_CHECK(invert(A, (int)Lc.N, inv), OPF_ERR_ARG, "gmg: singular coarsest operator");
+    H.cN = (int)Lc.N;
+    size_t& bytes = H.bytes;
+    bytes = 0;
+    OPF_TRY(alloc<double>((void**)&H.cinv, (long long)H.cN * H.cN, bytes));
+    OPF_HIP(hipMemcpy(H.cinv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    // device levels
+    H.dev.assign(H.host.size(), GLevelDev{});
+    const bool mixed = precision == OPF_PREC_MIXED;
+    const size_t ts = mixed ? sizeof(float) : sizeof(double);
+    for (size_t l = 0; l < H.host.size(); ++l) {
+        GLevelHost& L = H.host[l];
+        GLevelDev& D = H.dev[l];
+        for (int d = 0; d < 3; ++d) D.n[d] = L.n[d];
+        D.N = L.N;
+        if (L.var) {
+            for (int d = 0; d < H.dim; ++d) {
+                long long nf = face_count(L.n, d);
+                OPF_TRY(alloc<double>((void**)&D.af64[d], nf, bytes));
+                OPF_HIP(hipMemcpy(D.af64[d], L.af[d].data(), nf * sizeof(double), hipMemcpyHostToDevice));
+                if (mixed) {
+                    std::vector<float> f(L.af[d].begin(), L.af[d].end());
+                    OPF_TRY(alloc<float>((void**)&D.af32[d], nf, bytes));
+                    OPF_HIP(hipMemcpy(D.af32[d], f.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+                }
+            }
+            OPF_TRY(alloc<double>((void**)&D.sg64, L.N, bytes));
+            OPF_HIP(hipMemcpy(D.sg64, L.sg.data(), L.N * sizeof(double), hipMemcpyHostToDevice));
+            if (mixed) {
+                std::vector<float> f(L.sg.begin(), L.sg.end());
+                OPF_TRY(alloc<float>((void**)&D.sg32, L.N, bytes));
+                OPF_HIP(hipMemcpy(D.sg32, f.data(), L.N * sizeof(float), hipMemcpyHostToDevice));
+            }
+            if (l > 0) {  // the fine arrays are kept only where needed
+                L.af[0].clear(); L.af[1].clear(); L.af[2].clear(); L.sg.clear();
+            }
+        }
+        if (l > 0) {
+            OPF_HIP(hipMalloc(&D.x, ts * L.N));
+            OPF_HIP(hipMalloc(&D.b, ts * L.N));
+            bytes += 2 * ts * L.N;
+        }
+        OPF_HIP(hipMalloc(&D.t, ts * L.N));
+        bytes += ts * L.N;
+    }
+    long long N = fine.N;
+    OPF_HIP(hipMalloc(&H.r, ts * N));
+    OPF_HIP(hipMalloc(&H.z, ts * N));
+    OPF_HIP(hipMalloc(&H.p[0], ts * N));
+    OPF_HIP(hipMalloc(&H.p[1], ts * N));
+    OPF_HIP(hipMalloc(&H.q, ts * N));
+    bytes += 5 * ts * N;
+    if (mixed) {
+        OPF_HIP(hipMalloc(&H.e, ts * N));
+        bytes += ts * N;
+    }
+    OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * N));
+    bytes += sizeof(double) * N;
+    OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * 4 * GMG_MAXB));
+    OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * 16));
+    OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * 16));
+    OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * 16, hipHostMallocDefault));
+    (void)ctx;
+    return OPF_OK;
+}
+
+void gmg_free(Hierarchy& H) {
+    for (auto& D : H.dev) {
+        for (int d = 0; d < 3; ++d) {
+            if (D.af64[d]) hipFree(D.af64[d]);
+            if (D.af32[d]) hipFree(D.af32[d]);
+        }
+        if (D.sg64) hipFree(D.sg64);
+        if (D.sg32) hipFree(D.sg32);
+        if (D.x) hipFree(D.x);
+        if (D.b) hipFree(D.b);
+        if (D.t) hipFree(D.t);
+    }
+    H.dev.clear();
+    void* ptrs[] = {H.cinv, H.r, H.z, H.p[0], H.p[1], H.q, H.e, H.r64, H.part, H.scal};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (H.hscal) hipHostFree(H.hscal);
+    H.cinv = nullptr;
+    H.r = H.z = H.p[0] = H.p[1] = H.q = H.e = nullptr;
+    H.r64 = H.part = H.scal = H.hscal = nullptr;
+}
+
+// ---- launch wrappers dispatching on (VAR, DIM)
+#define GMG_DISPATCH(VAR, DIM, ...)                                                 \
+    do {                                                                            \
+        if (VAR) {                                                                  \
+            if (DIM == 3) { constexpr bool V_ = true; constexpr int D_ = 3; __VA_ARGS__; }  \
+            else { constexpr bool V_ = true; constexpr int D_ = 2; __VA_ARGS__; }           \
+        } else {                                                                    \
+            if (DIM == 3) { constexpr bool V_ = false; constexpr int D_ = 3; __VA_ARGS__; } \
+            else { constexpr bool V_ = false; constexpr int D_ = 2; __VA_ARGS__; }          \
+        }                                                                           \
+    } while (0)
+
+template <typename T>
+static void vcycle(const Hierarchy& H, hipStream_t st, int l, const T* rhs, T* out) {
+    const GLevelDev& D = H.dev[l];
+    const GLevelHost& L = H.host[l];
+    const long long N = L.N;
+    if (l == (int)H.dev.size() - 1) {
+        hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(256), 0, st, H.cinv, H.cN, rhs, out);
+        return;
+    }
+    GOp<T> o = make_op<T>(H, l);
+    const T w = (T)H.omega;
+    // Jacobi sweeps ping-pong between `out` and the level temp; start on the
+    // buffer that makes the last sweep land in `out`
+    T* cur = out;
+    T* tmp = (T*)D.t;
+    if (((H.pre - 1) + H.post) & 1) std::swap(cur, tmp);
+    // pre-smoothing from a zero guess
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, true>), dim3(nblk(N)), dim3(256), 0, st, o,
+                                                   (const T*)nullptr, rhs, cur, w));
+    for (int s = 1; s < H.pre; ++s) {
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N)), dim3(256), 0, st,
+                                                       o, (const T*)cur, rhs, tmp, w));
+        std::swap(cur, tmp);
+    }
+    const GLevelDev& C = H.dev[l + 1];
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N)), dim3(256), 0, st, o,
+                                                   (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2]));
+    vcycle<T>(H, st, l + 1, (const T*)C.b, (T*)C.x);
+    if (H.dim == 3)
+        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N)), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
+                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o);
+    else
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N)), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
+                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o);
+    for (int s = 0; s < H.post; ++s) {
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N)), dim3(256), 0, st,
+                                                       o, (const T*)cur, rhs, tmp, w));
+        std::swap(cur, tmp);
+    }
+}
+
+static int read_scal(const Hierarchy& H, hipStream_t st) {
+    OPF_HIP(hipMemcpyAsync(H.hscal, H.scal, sizeof(double) * S_N, hipMemcpyDeviceToHost, st));
+    OPF_HIP(hipStreamSynchronize(st));
+    return OPF_OK;
+}
+
+// Flexible PCG in precision T on A e = r (r is modified; e accumulates from its current value).
+// Stops when ||r|| <= rtol * ||r0|| or after maxit iterations. Returns iterations.
+template <typename T>
+static int fcg(const Hierarchy& H, hipStream_t st, T* e, T* r, double rr0, double rtol, int maxit, int* iters,
+               double* rr_out) {
+    const GLevelHost& L = H.host[0];
+    const long long N = L.N;
+    GOp<T> o = make_op<T>(H, 0);
+    const int G = rblk(N);
+    T* z = (T*)H.z;
+    T* q = (T*)H.q;
+    int pc = 0;
+    // beta = 0 on the first iteration multiplies p_old: it must hold finite values
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N)), dim3(256), 0, st, N, (T*)H.p[0], (T)0);
+    vcycle<T>(H, st, 0, r, z);
+    hipLaunchKernelGGL(k_dot3<T>, dim3(G), dim3(256), 0, st, N, r, z, z, H.part);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_DOT3_FIRST, H.scal, 1.0 / N,
+                       (int)H.singular);
+    int it = 0;
+    double rr = rr0;
+    const int check_every = N > (1 << 20) ? 1 : 2;
+    for (it = 1; it <= maxit; ++it) {
+        T* pold = (T*)H.p[pc];
+        T* pnew = (T*)H.p[1 - pc];
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G), dim3(256), 0, st, o,
+                                                       (const T*)z, (const T*)pold, pnew, q, H.scal, H.part));
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_PQ, H.scal, 1.0 / N,
+                           (int)H.singular);
+        hipLaunchKernelGGL(k_axpy2<T>, dim3(G), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_RR, H.scal, 1.0 / N,
+                           (int)H.singular);
+        pc = 1 - pc;
+        if (it % check_every == 0 || it == maxit) {
+            OPF_TRY(read_scal(H, st));
+            rr = H.hscal[S_RR];
+            if (!(rr > rtol * rtol * rr0) || !std::isfinite(rr)) break;
+        }
+        vcycle<T>(H, st, 0, r, z);
+        hipLaunchKernelGGL(k_dot3<T>, dim3(G), dim3(256), 0, st, N, r, z, q, H.part);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_DOT3, H.scal, 1.0 / N,
+                           (int)H.singular);
+    }
+    OPF_HIP(hipGetLastError());
+    *iters = std::min(it, maxit);
+    *rr_out = rr;
+    return OPF_OK;
+}
+
+template <typename T>
+static void launch_resid64(const Hierarchy& H, hipStream_t st, const double* x, const double* b, T* r) {
+    GOp<double> o = make_op<double>(H, 0);
+    const long long N = H.host[0].N;
+    const int G = rblk(N);
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, T, V_, D_>), dim3(G), dim3(256), 0, st, o, x, b,
+                                                   r, H.part));
+}
+
+int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,
+              int verbose) {
+    hipStream_t s = ctx->stream;
+    const long long N = H.host[0].N;
+    const int G = rblk(N);
+    // ||b||
+    launch_resid64<double>(H, s, nullptr, b, (double*)nullptr);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_BB, H.scal, 1.0 / N, (int)H.singular);
+    int total = 0, nref = 0;
+    double rr = 0, bb = 0;
+    if (H.precision == OPF_PREC_FP64) {
+        double* r = (double*)H.r;
+        launch_resid64<double>(H, s, x, b, r);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N,
+                           (int)H.singular);
+        OPF_TRY(read_scal(H, s));
+        bb = H.hscal[S_BB];
+        double rr0 = H.hscal[S_RR0];
+        if (H.singular) {  // project the residual onto the range of A
+            hipLaunchKernelGGL(k_sum<double>, dim3(G), dim3(256), 0, s, N, r, H.part);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N, 1);
+            hipLaunchKernelGGL(k_sub_mean<double>, dim3(nblk(N)), dim3(256), 0, s, N, r, H.scal, (int)S_RR0, 1.0 / N);
+        }
+        double target = tol * tol * bb;
+        if (rr0 > target && bb > 0) {
+            int its = 0;
+            double rrf = 0;
+            OPF_TRY(fcg<double>(H, s, x, r, rr0, std::sqrt(target / rr0), maxit, &its, &rrf));
+            total = its;
+        }
+        nref = 1;
+    } else {
+        // iterative refinement: FP64 residual / update, FP32 inner FCG
+        float* r = (float*)H.r;
+        float* e = (float*)H.e;
+        bool first = true;
+        double target = 0;
+        for (nref = 0; nref < 40; ++nref) {
+            launch_resid64<float>(H, s, x, b, r);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N,
+                               (int)H.singular);
+            OPF_TRY(read_scal(H, s));
+            if (first) {
+                bb = H.hscal[S_BB];
+                target = tol * tol * bb;
+                first = false;
+            }
+            double rr0 = H.hscal[S_RR0];
+            if (verbose) fprintf(stderr, "[gmg mixed] refine %d relres %.3e\n", nref, std::sqrt(rr0 / bb));
+            if (!(rr0 > target) || bb == 0 || total >= maxit) break;
+            if (H.singular) {
+                hipLaunchKernelGGL(k_sum<float>, dim3(G), dim3(256), 0, s, N, r, H.part);
+                hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N, 1);
+                hipLaunchKernelGGL(k_sub_mean<float>, dim3(nblk(N)), dim3(256), 0, s, N, r, H.scal, (int)S_RR0,
+                                   1.0 / N);
+            }
+            hipLaunchKernelGGL(k_fill<float>, dim3(nblk(N)), dim3(256), 0, s, N, e, 0.0f);
+            int its = 0;
+            double rrf = 0;
+            // reduce the residual by ~1e-4 per refinement (FP32 resolves ~1e-6)
+            double inner = std::max(1e-4, std::sqrt(target / rr0) * 0.5);
+            OPF_TRY(fcg<float>(H, s, e, r, rr0, std::min(inner, 0.5), std::max(1, maxit - total), &its, &rrf));
+            total += its;
+            hipLaunchKernelGGL(k_add_to64<float>, dim3(nblk(N)), dim3(256), 0, s, N, x, (const float*)e);
+        }
+    }
+    // final true residual in FP64
+    launch_resid64<double>(H, s, x, b, H.r64);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR, H.scal, 1.0 / N, (int)H.singular);
+    OPF_TRY(read_scal(H, s));
+    bb = H.hscal[S_BB];
+    rr = H.hscal[S_RR];
+    OPF_HIP(hipGetLastError());
+    st->niter = total;
+    st->nrefine = nref;
+    st->abserr = std::sqrt(rr);
+    st->relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    st->converged = st->relerr <= tol * (1 + 1e-12) || bb == 0;
+    if (verbose) fprintf(stderr, "[gmg] iters %d refine %d relres %.3e\n", total, nref, st->relerr);
+    return OPF_OK;
+}
+
+int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y) {
+    GOp<double> o = make_op<double>(H, 0);
+    const long long N = H.host[0].N;
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_apply<double, V_, D_>), dim3(nblk(N)), dim3(256), 0,
+                                                   ctx->stream, o, x, y));
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
+}
+
+}  // namespace opf
+
+// ------------------------------------------------------------------ C ABI
+using namespace opf;
+
+struct opf_gmg {
+    opf_ctx* ctx;
+    Hierarchy H;
+    opf_poisson_desc d;
+};
+
+static int poisson_level(const opf_poisson_desc* d, GLevelHost& L, int per[3], double alpha[3][2]) {
+    OPF_CHECK(d && (d->dim == 2 || d->dim == 3), OPF_ERR_ARG, "gmg: dim must be 2 or 3");
+    L = GLevelHost{};
+    for (int q = 0; q < 3; ++q) L.n[q] = q < d->dim ? d->n[q] : 1;
+    L.N = (long long)L.n[0] * L.n[1] * L.n[2];
+    L.var = false;
+    L.sigma = d->sigma;
+    for (int q = 0; q < 3; ++q) {
+        per[q] = 0;
+        alpha[q][0] = alpha[q][1] = 1.0;
+        L.a[q] = 0.0;
+        if (q >= d->dim) continue;
+        OPF_CHECK(d->n[q] >= 2 && d->h[q] > 0, OPF_ERR_ARG, "gmg: bad n/h in dim %d", q);
+        L.a[q] = 1.0 / (d->h[q] * d->h[q]);
+        for (int s = 0; s < 2; ++s) {
+            int t = d->bc[q][s];
+            OPF_CHECK(t == OPF_BC_DIRC || t == OPF_BC_NEUM || t == OPF_BC_PERIODIC, OPF_ERR_UNSUPPORTED,
+                      "gmg: bc %d unsupported", t);
+            alpha[q][s] = t == OPF_BC_DIRC ? -1.0 : 1.0;
+        }
+        per[q] = d->bc[q][0] == OPF_BC_PERIODIC;
+        OPF_CHECK(per[q] == (d->bc[q][1] == OPF_BC_PERIODIC), OPF_ERR_ARG, "gmg: periodic on one side only");
+    }
+    return OPF_OK;
+}
+
+extern "C" size_t opf_gmg_footprint(const opf_poisson_desc* d, int precision) {
+    if (!d) return 0;
+    long long N = 1;
+    for (int q = 0; q < d->dim; ++q) N *= d->n[q];
+    size_t ts = precision == OPF_PREC_MIXED ? 4 : 8;
+    double lv = d->dim == 3 ? 8.0 / 7.0 : 4.0 / 3.0;
+    // 3 vectors per level (x, b, t) + 5/6 Krylov vectors + FP64 residual
+    return (size_t)(N * ts * (3 * lv + 6) + N * 8) + (1 << 20);
+}
+
+extern "C" int opf_gmg_create(opf_ctx* ctx, const opf_poisson_desc* d, int precision, opf_gmg** out) {
+    OPF_CHECK(ctx && d && out, OPF_ERR_ARG, "opf_gmg_create: null arg");
+    OPF_CHECK(precision == OPF_PREC_FP64 || precision == OPF_PREC_MIXED, OPF_ERR_ARG, "bad precision");
+    GLevelHost L;
+    int per[3];
+    double alpha[3][2];
+    OPF_TRY(poisson_level(d, L, per, alpha));
+    auto* g = new opf_gmg();
+    g->ctx = ctx;
+    g->d = *d;
+    int rc = gmg_build(ctx, g->H, L, per, alpha, precision);
+    if (rc != OPF_OK) {
+        gmg_free(g->H);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return OPF_OK;
+}
+
+extern "C" int opf_gmg_solve(opf_gmg* g, const double* b, double* x, const opf_solve_params* p,
+                             opf_solve_state* st) {
+    OPF_CHECK(g && b && x && st, OPF_ERR_ARG, "opf_gmg_solve: null arg");
+    double tol = p && p->tol > 0 ? p->tol : 1e-6;
+    int maxit = p && p->max_iter > 0 ? p->max_iter : 200;
+    if (p && p->pre_smooth > 0) g->H.pre = p->pre_smooth;
+    if (p && p->post_smooth > 0) g->H.post = p->post_smooth;
+    return gmg_solve(g->ctx, g->H, b, x, tol, maxit, st, p ? p->verbose : 0);
+}
+
+extern "C" int opf_gmg_apply(opf_gmg* g, const double* x, double* y) {
+    OPF_CHECK(g && x && y, OPF_ERR_ARG, "opf_gmg_apply: null arg");
+    return gmg_apply64(g->ctx, g->H, x, y);
+}
+
+extern "C" int opf_gmg_destroy(opf_gmg* g) {
+    if (!g) return OPF_OK;
+    hipStreamSynchronize(g->ctx->stream);
+    gmg_free(g->H);
+    delete g;
+    return OPF_OK;
+}

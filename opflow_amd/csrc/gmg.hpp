@@ -1,0 +1,92 @@
+// Matrix-free geometric multigrid + flexible CG for cell-centred 5/7-point
+// operators in divergence form (the SPD form of the StencilPad systems OpFlow
+// assembles for Laplacian-type equations, HYPREEqnSolveHandler.hpp:125-166):
+//
+//   (A x)_i = sigma_i x_i + sum_faces a_f (x_i - x_nb)
+//
+// where a boundary face's neighbour is the ghost value alpha * x_i (alpha = -1
+// for a centre Dirichlet face on a uniform mesh, +1 for Neumann) and a
+// periodic face wraps. Coefficients are either constant per dimension or
+// per-face arrays (variable coefficients).
+#pragma once
+
+#include <vector>
+
+#include "opf_internal.hpp"
+
+namespace opf {
+
+constexpr int GMG_MAXB = 2048;  // max blocks of a reduction kernel (partials per value)
+
+// Operator of one level as seen by the kernels (T = float or double).
+template <typename T>
+struct GOp {
+    int n0, n1, n2;
+    int dim;
+    int per[3];
+    T a[3];          // constant face coefficient per dim (const mode)
+    T sigma;         // constant diagonal shift (const mode)
+    T bfac[3][2];    // boundary face factor (1 - alpha) per dim/side (non-periodic)
+    const T* af[3];  // per-face coefficients, face array dims n + e_d (variable mode)
+    const T* sg;     // per-cell sigma (variable mode)
+};
+
+// Host description of one level (double master copy of the coefficients).
+struct GLevelHost {
+    int n[3];
+    long long N;
+    bool var;
+    double a[3], sigma;
+    std::vector<double> af[3], sg;  // variable mode
+};
+
+struct GLevelDev {
+    int n[3];
+    long long N;
+    // coefficients in both precisions (variable mode arrays)
+    double* af64[3] = {nullptr, nullptr, nullptr};
+    float* af32[3] = {nullptr, nullptr, nullptr};
+    double* sg64 = nullptr;
+    float* sg32 = nullptr;
+    // work vectors of the V-cycle precision: x (solution), b (rhs), t (temp)
+    void* x = nullptr;
+    void* b = nullptr;
+    void* t = nullptr;
+};
+
+struct Hierarchy {
+    int dim = 2;
+    int per[3] = {0, 0, 0};
+    double alpha[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+    bool var = false;
+    bool singular = false;
+    int precision = OPF_PREC_FP64;  // V-cycle / inner-Krylov precision
+    double omega = 0.8;
+    int pre = 2, post = 2;
+    std::vector<GLevelHost> host;
+    std::vector<GLevelDev> dev;
+    // coarsest dense inverse (FP64)
+    double* cinv = nullptr;
+    int cN = 0;
+    // Krylov vectors at level 0 (precision of the inner solve) and FP64 outer vectors
+    void* r = nullptr;
+    void* z = nullptr;
+    void* p[2] = {nullptr, nullptr};
+    void* q = nullptr;
+    void* e = nullptr;      // inner solution (mixed)
+    double* r64 = nullptr;  // outer residual (mixed) / work
+    double* part = nullptr; // reduction partials [8][GMG_MAXB]
+    double* scal = nullptr; // device scalars
+    double* hscal = nullptr;  // pinned host mirror
+    size_t bytes = 0;
+};
+
+int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
+              int precision);
+void gmg_free(Hierarchy& H);
+// solve A x = b on the stream: b, x FP64 device arrays (N each). Returns iterations etc.
+int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,
+              int verbose);
+int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y);
+
+}  // namespace opf

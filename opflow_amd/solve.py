@@ -124,6 +124,31 @@ def poisson_desc(dim, n, h, bc, sigma=0.0):
     return d
 
 
+class DeviceBuffer:
+    """Raw device array of float64 owned by the library (opf_buf_*)."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n, self.lib = ctx, n, ctx.lib
+        p = C.c_void_p()
+        L.check(self.lib.opf_buf_create(ctx.h, 8 * n, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self.n
+        L.check(self.lib.opf_buf_upload(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.n)
+        L.check(self.lib.opf_buf_download(self.ctx.h, self.ptr, out.ctypes.data, out.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.lib.opf_buf_destroy(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
 class PoissonGMG:
     """Direct matrix-free GMG-PCG on device vectors (opf_gmg_*)."""
 

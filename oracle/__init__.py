@@ -31,6 +31,13 @@ def lib():
         _lib.orc_desc_finalize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.orc_reduce.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         _lib.orc_mesh_uniform.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.orc_csr_poisson.restype = C.c_long
+        _lib.orc_csr_poisson.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+        _lib.orc_poisson_apply.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                           C.c_void_p]
+        _lib.orc_pcg_mg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                    C.c_double, C.c_int, C.c_void_p]
     return _lib
 
 
@@ -99,6 +106,69 @@ def reduce(desc, arr, red, rs, re_):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def csr_poisson(desc, mesh, c=1.0, pin_last=False):
+    """CSR (ptr, col, val, rhs) of `c == sum_d d2<D2SecondOrderCentered>(e)` (CSRMatrixGenerator restated)."""
+    ncell = 1
+    for d in range(desc.dim):
+        ncell *= desc.access_end[d] - desc.access_start[d]
+    ptr = np.zeros(ncell + 1, dtype=np.int64)
+    col = np.zeros(ncell * 32, dtype=np.int64)
+    val = np.zeros(ncell * 32)
+    rhs = np.zeros(ncell)
+    nnz = lib().orc_csr_poisson(C.byref(desc), C.byref(mesh.s), c, int(pin_last), ptr.ctypes.data, col.ctypes.data,
+                                val.ctypes.data, rhs.ctypes.data)
+    return ptr, col[:nnz], val[:nnz], rhs
+
+
+def _pois_args(n, h, bc):
+    dim = len(n)
+    n3 = (C.c_int * 3)(*(list(n) + [1] * (3 - dim)))
+    h3 = (C.c_double * 3)(*(list(h) + [1.0] * (3 - dim)))
+    bc3 = ((C.c_int * 2) * 3)()
+    for d in range(dim):
+        bc3[d][0], bc3[d][1] = bc[d]
+    return dim, n3, h3, bc3
+
+
+def poisson_apply(n, h, bc, x, sigma=0.0):
+    dim, n3, h3, bc3 = _pois_args(n, h, bc)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    lib().orc_poisson_apply(dim, n3, h3, bc3, sigma, x.ctypes.data, y.ctypes.data)
+    return y
+
+
+def pcg_mg(n, h, bc, b, x0=None, sigma=0.0, tol=1e-10, maxit=200):
+    dim, n3, h3, bc3 = _pois_args(n, h, bc)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    rel = C.c_double()
+    it = lib().orc_pcg_mg(dim, n3, h3, bc3, sigma, b.ctypes.data, x.ctypes.data, tol, maxit, C.byref(rel))
+    return x, it, rel.value
+
+
+REF_PATH = os.path.join(_HERE, "_ref", "libamgcl_ref.so")
+
+
+def amgcl_solve(ptr, col, val, rhs, tol=1e-12, maxiter=500, kind=0):
+    """Reference solve with AMGCL (oracle/_ref, built from the reference tree's headers)."""
+    if not os.path.exists(REF_PATH):
+        return None
+    ref = C.CDLL(REF_PATH)
+    ref.amgcl_ref_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    n = len(rhs)
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros(n)
+    it, res = C.c_int(), C.c_double()
+    ref.amgcl_ref_solve(n, ptr.ctypes.data, col.ctypes.data, val.ctypes.data, rhs.ctypes.data, x.ctypes.data, tol,
+                        maxiter, kind, C.byref(it), C.byref(res))
+    return x, it.value, res.value
 
 
 from .expr import evaluate, assign  # noqa: E402,F401
