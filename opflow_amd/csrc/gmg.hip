@@ -82,6 +82,21 @@ __host__ __device__ __forceinline__ void cell_of(long long t, int n0, int n1, in
     k = (int)(r / n1);
 }
 
+// Batched execution: blockIdx.y selects a request slot; slot vectors are
+// stored back to back (stride vs), scalars and reduction partials likewise.
+// mask (optional, device) switches slots off without changing the launch.
+struct Bat {
+    const int* mask;
+    long long vs;
+    int ss;
+    int ps;
+};
+#define BAT_SLOT(b)                                  \
+    const int slot = blockIdx.y;                     \
+    if ((b).mask && !(b).mask[slot]) return;         \
+    const long long vo = (long long)slot * (b).vs;   \
+    (void)vo;
+
 __device__ double gmg_block_sum(double v) {
     __shared__ double sh[8];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -105,8 +120,13 @@ __device__ double gmg_block_sum(double v) {
 // fixed grid => deterministic). TO: output precision.
 template <typename T, typename TO, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
-                                               TO* __restrict__ r, double* __restrict__ part) {
+                                               TO* __restrict__ r, double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
     long long N = (long long)o.n0 * o.n1 * o.n2;
+    if (x) x += vo;
+    b += vo;
+    if (r) r += vo;
+    if (part) part += (long long)slot * bt.ps;
     double acc = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
         int i, j, k;
@@ -142,8 +162,12 @@ __global__ __launch_bounds__(256) void k_apply(GOp<T> o, const T* __restrict__ x
 // weighted Jacobi: xo = xi + w (b - A xi)/D ; ZERO: xo = w b / D
 template <typename T, bool VAR, int DIM, bool ZERO>
 __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
-                                                T* __restrict__ xo, T w) {
+                                                T* __restrict__ xo, T w, Bat bt) {
+    BAT_SLOT(bt);
     long long N = (long long)o.n0 * o.n1 * o.n2;
+    if (xi) xi += vo;
+    b += vo;
+    xo += vo;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     int i, j, k;
@@ -164,8 +188,12 @@ __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ 
 // coarse rhs = average over the 2^d children of the fine residual b - A x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
-                                                  T* __restrict__ bc, int c0, int c1, int c2) {
+                                                  T* __restrict__ bc, int c0, int c1, int c2, Bat bt) {
+    BAT_SLOT(bt);
     long long NC = (long long)c0 * c1 * c2;
+    x += vo;
+    b += vo;
+    bc += (long long)slot * NC;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NC) return;
     int I, J, K;
@@ -187,8 +215,11 @@ __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict_
 // x_f += P x_c (linear interpolation, coarse ghosts: periodic wrap or alpha * boundary value)
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, int n0, int n1, int n2, const T* __restrict__ xc,
-                                                 int c0, int c1, int c2, GOp<T> o) {
+                                                 int c0, int c1, int c2, GOp<T> o, Bat bt) {
+    BAT_SLOT(bt);
     long long N = (long long)n0 * n1 * n2;
+    xf += vo;
+    xc += (long long)slot * ((long long)c0 * c1 * c2);
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     int i, j, k;
@@ -227,8 +258,11 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, int n0, int
 // coarsest level: x = Cinv b (FP64 inverse), one block
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, int n, const T* __restrict__ b,
-                                               T* __restrict__ x) {
+                                               T* __restrict__ x, Bat bt) {
     __shared__ double sb[1024];
+    BAT_SLOT(bt);
+    b += (long long)slot * n;
+    x += (long long)slot * n;
     for (int q = threadIdx.x; q < n; q += blockDim.x) sb[q] = (double)b[q];
     __syncthreads();
     for (int row = threadIdx.x; row < n; row += blockDim.x) {
@@ -239,12 +273,12 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
     }
 }
 
-// scalar slots
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_RR, S_BETA, S_ZQ, S_ZMEAN, S_BB, S_RR0, S_N };
-enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0 };
 
 __global__ void k_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ s, double invN,
-                         int singular) {
+                         int singular, Bat bt) {
+    BAT_SLOT(bt);
+    part += (long long)slot * bt.ps;
+    s += (long long)slot * bt.ss;
     double v[3];
     for (int q = 0; q < 3; ++q) {
         double acc = 0;
@@ -280,8 +314,15 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int mode, doub
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ z, const T* __restrict__ pold,
                                                 T* __restrict__ pnew, T* __restrict__ q,
-                                                const double* __restrict__ s, double* __restrict__ part) {
+                                                const double* __restrict__ s, double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
     long long N = (long long)o.n0 * o.n1 * o.n2;
+    z += vo;
+    pold += vo;
+    pnew += vo;
+    q += vo;
+    s += (long long)slot * bt.ss;
+    part += (long long)slot * bt.ps;
     const T beta = (T)s[S_BETA];
     const T zm = (T)s[S_ZMEAN];
     double acc = 0;
@@ -307,7 +348,14 @@ __global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(256) void k_axpy2(long long N, T* __restrict__ x, const T* __restrict__ p,
                                                T* __restrict__ r, const T* __restrict__ q,
-                                               const double* __restrict__ s, double* __restrict__ part) {
+                                               const double* __restrict__ s, double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
+    p += vo;
+    r += vo;
+    q += vo;
+    s += (long long)slot * bt.ss;
+    part += (long long)slot * bt.ps;
     const T alpha = (T)s[S_ALPHA];
     double acc = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
@@ -323,7 +371,12 @@ __global__ __launch_bounds__(256) void k_axpy2(long long N, T* __restrict__ x, c
 // partials of (r, z), (z, q), sum(z)
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot3(long long N, const T* __restrict__ r, const T* __restrict__ z,
-                                              const T* __restrict__ q, double* __restrict__ part) {
+                                              const T* __restrict__ q, double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
+    r += vo;
+    z += vo;
+    q += vo;
+    part += (long long)slot * bt.ps;
     double a = 0, b = 0, c = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
         double zv = (double)z[t];
@@ -343,26 +396,38 @@ __global__ __launch_bounds__(256) void k_dot3(long long N, const T* __restrict__
 
 // x (FP64) += e (FP32 or FP64)
 template <typename TE>
-__global__ void k_add_to64(long long N, double* __restrict__ x, const TE* __restrict__ e) {
+__global__ void k_add_to64(long long N, double* __restrict__ x, const TE* __restrict__ e, Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
+    e += vo;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) x[t] += (double)e[t];
 }
 
 template <typename T>
-__global__ void k_fill(long long N, T* __restrict__ x, T v) {
+__global__ void k_fill(long long N, T* __restrict__ x, T v, Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) x[t] = v;
 }
 
 // subtract a device scalar mean (singular problems): x -= s[slot] * scale
 template <typename T>
-__global__ void k_sub_mean(long long N, T* __restrict__ x, const double* __restrict__ s, int slot, double scale) {
+__global__ void k_sub_mean(long long N, T* __restrict__ x, const double* __restrict__ s, int which, double scale,
+                           Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
+    s += (long long)slot * bt.ss;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) x[t] -= (T)(s[slot] * scale);
+    if (t < N) x[t] -= (T)(s[which] * scale);
 }
 
 template <typename T>
-__global__ void k_sum(long long N, const T* __restrict__ x, double* __restrict__ part) {
+__global__ void k_sum(long long N, const T* __restrict__ x, double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
+    part += (long long)slot * bt.ps;
     double a = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x)
         a += (double)x[t];
@@ -516,8 +581,9 @@ static int alloc(void** p, long long n, size_t& bytes) {
 }
 
 int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
-              int precision) {
+              int precision, int slots) {
     H.precision = precision;
+    H.B = std::max(1, slots);
     H.var = fine.var;
     for (int d = 0; d < 3; ++d) {
         H.per[d] = per[d];
@@ -594,31 +660,32 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
                 L.af[0].clear(); L.af[1].clear(); L.af[2].clear(); L.sg.clear();
             }
         }
+        const size_t BN = (size_t)H.B * L.N;
         if (l > 0) {
-            OPF_HIP(hipMalloc(&D.x, ts * L.N));
-            OPF_HIP(hipMalloc(&D.b, ts * L.N));
-            bytes += 2 * ts * L.N;
+            OPF_HIP(hipMalloc(&D.x, ts * BN));
+            OPF_HIP(hipMalloc(&D.b, ts * BN));
+            bytes += 2 * ts * BN;
         }
-        OPF_HIP(hipMalloc(&D.t, ts * L.N));
-        bytes += ts * L.N;
+        OPF_HIP(hipMalloc(&D.t, ts * BN));
+        bytes += ts * BN;
     }
-    long long N = fine.N;
-    OPF_HIP(hipMalloc(&H.r, ts * N));
-    OPF_HIP(hipMalloc(&H.z, ts * N));
-    OPF_HIP(hipMalloc(&H.p[0], ts * N));
-    OPF_HIP(hipMalloc(&H.p[1], ts * N));
-    OPF_HIP(hipMalloc(&H.q, ts * N));
-    bytes += 5 * ts * N;
+    const size_t BN = (size_t)H.B * fine.N;
+    OPF_HIP(hipMalloc(&H.r, ts * BN));
+    OPF_HIP(hipMalloc(&H.z, ts * BN));
+    OPF_HIP(hipMalloc(&H.p[0], ts * BN));
+    OPF_HIP(hipMalloc(&H.p[1], ts * BN));
+    OPF_HIP(hipMalloc(&H.q, ts * BN));
+    bytes += 5 * ts * BN;
     if (mixed) {
-        OPF_HIP(hipMalloc(&H.e, ts * N));
-        bytes += ts * N;
+        OPF_HIP(hipMalloc(&H.e, ts * BN));
+        bytes += ts * BN;
     }
-    OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * N));
-    bytes += sizeof(double) * N;
-    OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * 4 * GMG_MAXB));
-    OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * 16));
-    OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * 16));
-    OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * 16, hipHostMallocDefault));
+    OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * BN));
+    bytes += sizeof(double) * BN;
+    OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * GMG_PSTRIDE * H.B));
+    OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * GMG_SSTRIDE * H.B));
+    OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * GMG_SSTRIDE * H.B));
+    OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * GMG_SSTRIDE * H.B, hipHostMallocDefault));
     (void)ctx;
     return OPF_OK;
 }
@@ -657,13 +724,18 @@ void gmg_free(Hierarchy& H) {
         }                                                                           \
     } while (0)
 
+static inline Bat bat(const Hierarchy& H, const int* mask, long long vs) {
+    return Bat{mask, vs, GMG_SSTRIDE, GMG_PSTRIDE};
+}
+
 template <typename T>
-static void vcycle(const Hierarchy& H, hipStream_t st, int l, const T* rhs, T* out) {
+void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out) {
     const GLevelDev& D = H.dev[l];
     const GLevelHost& L = H.host[l];
     const long long N = L.N;
+    const unsigned B = (unsigned)H.B;
     if (l == (int)H.dev.size() - 1) {
-        hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(256), 0, st, H.cinv, H.cN, rhs, out);
+        hipLaunchKernelGGL(k_dense<T>, dim3(1, B), dim3(256), 0, st, H.cinv, H.cN, rhs, out, bat(H, mask, N));
         return;
     }
     GOp<T> o = make_op<T>(H, l);
@@ -673,78 +745,131 @@ static void vcycle(const Hierarchy& H, hipStream_t st, int l, const T* rhs, T* o
     T* cur = out;
     T* tmp = (T*)D.t;
     if (((H.pre - 1) + H.post) & 1) std::swap(cur, tmp);
-    // pre-smoothing from a zero guess
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, true>), dim3(nblk(N)), dim3(256), 0, st, o,
-                                                   (const T*)nullptr, rhs, cur, w));
+    const Bat bt = bat(H, mask, N);
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, true>), dim3(nblk(N), B), dim3(256), 0, st, o,
+                                                  (const T*)nullptr, rhs, cur, w, bt));
     for (int s = 1; s < H.pre; ++s) {
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N)), dim3(256), 0, st,
-                                                       o, (const T*)cur, rhs, tmp, w));
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256), 0,
+                                                      st, o, (const T*)cur, rhs, tmp, w, bt));
         std::swap(cur, tmp);
     }
     const GLevelDev& C = H.dev[l + 1];
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N)), dim3(256), 0, st, o,
-                                                   (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2]));
-    vcycle<T>(H, st, l + 1, (const T*)C.b, (T*)C.x);
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N), B), dim3(256), 0, st, o,
+                                                  (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], bt));
+    ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
     if (H.dim == 3)
-        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N)), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
-                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o);
+        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
+                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o, bt);
     else
-        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N)), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
-                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o);
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
+                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o, bt);
     for (int s = 0; s < H.post; ++s) {
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N)), dim3(256), 0, st,
-                                                       o, (const T*)cur, rhs, tmp, w));
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256), 0,
+                                                      st, o, (const T*)cur, rhs, tmp, w, bt));
         std::swap(cur, tmp);
     }
 }
 
-static int read_scal(const Hierarchy& H, hipStream_t st) {
-    OPF_HIP(hipMemcpyAsync(H.hscal, H.scal, sizeof(double) * S_N, hipMemcpyDeviceToHost, st));
+static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular) {
+    const long long N = H.host[0].N;
+    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(256), 0, st, H.part, rblk(N), mode, H.scal, 1.0 / N,
+                       singular, bat(H, mask, N));
+}
+
+// start of an inner FCG solve on r: p_old = 0, z = M r, rz = (r, z)
+template <typename T>
+void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask) {
+    const long long N = H.host[0].N;
+    const unsigned B = (unsigned)H.B;
+    const Bat bt = bat(H, mask, N);
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N), B), dim3(256), 0, st, N, (T*)H.p[0], (T)0, bt);
+    ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z);
+    hipLaunchKernelGGL(k_dot3<T>, dim3(rblk(N), B), dim3(256), 0, st, N, (const T*)H.r, (const T*)H.z,
+                       (const T*)H.z, H.part, bt);
+    ph_finish(H, st, mask, FIN_DOT3_FIRST, H.singular);
+}
+
+// one FCG iteration on (e, r) with p ping-pong index pc
+template <typename T>
+void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int pc) {
+    const long long N = H.host[0].N;
+    const unsigned B = (unsigned)H.B;
+    const int G = rblk(N);
+    const Bat bt = bat(H, mask, N);
+    GOp<T> o = make_op<T>(H, 0);
+    T* z = (T*)H.z;
+    T* q = (T*)H.q;
+    T* r = (T*)H.r;
+    T* pold = (T*)H.p[pc];
+    T* pnew = (T*)H.p[1 - pc];
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o, (const T*)z,
+                                                  (const T*)pold, pnew, q, H.scal, H.part, bt));
+    ph_finish(H, st, mask, FIN_PQ, H.singular);
+    hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
+                       bt);
+    ph_finish(H, st, mask, FIN_RR, H.singular);
+    ph_vcycle<T>(H, st, mask, 0, (const T*)r, z);
+    hipLaunchKernelGGL(k_dot3<T>, dim3(G, B), dim3(256), 0, st, N, (const T*)r, (const T*)z, (const T*)q, H.part, bt);
+    ph_finish(H, st, mask, FIN_DOT3, H.singular);
+}
+
+// r_T = b - A x in FP64 (x may be null), partial sums of r^2 -> finish `mode`
+template <typename TO>
+void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const double* x, const double* b, TO* r,
+                int mode) {
+    GOp<double> o = make_op<double>(H, 0);
+    const long long N = H.host[0].N;
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(rblk(N), (unsigned)H.B),
+                                                  dim3(256), 0, st, o, x, b, r, H.part, bat(H, mask, N)));
+    ph_finish(H, st, mask, mode, H.singular);
+}
+
+template <typename T>
+void ph_project(const Hierarchy& H, hipStream_t st, const int* mask, T* r) {
+    const long long N = H.host[0].N;
+    const Bat bt = bat(H, mask, N);
+    hipLaunchKernelGGL(k_sum<T>, dim3(rblk(N), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
+    ph_finish(H, st, mask, FIN_RR0, 1);
+    hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, r, H.scal, (int)S_RR0,
+                       1.0 / N, bt);
+}
+
+template <typename TE>
+void ph_add64(const Hierarchy& H, hipStream_t st, const int* mask, double* x, const TE* e) {
+    const long long N = H.host[0].N;
+    hipLaunchKernelGGL(k_add_to64<TE>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, x, e, bat(H, mask, N));
+}
+
+template <typename T>
+void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v) {
+    const long long N = H.host[0].N;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, v, (T)0, bat(H, mask, N));
+}
+
+int read_scal(const Hierarchy& H, hipStream_t st) {
+    OPF_HIP(hipMemcpyAsync(H.hscal, H.scal, sizeof(double) * GMG_SSTRIDE * H.B, hipMemcpyDeviceToHost, st));
     OPF_HIP(hipStreamSynchronize(st));
     return OPF_OK;
 }
 
-// Flexible PCG in precision T on A e = r (r is modified; e accumulates from its current value).
-// Stops when ||r|| <= rtol * ||r0|| or after maxit iterations. Returns iterations.
+// Flexible PCG in precision T on A e = r (slot 0; r is modified, e accumulates).
 template <typename T>
-static int fcg(const Hierarchy& H, hipStream_t st, T* e, T* r, double rr0, double rtol, int maxit, int* iters,
+static int fcg(const Hierarchy& H, hipStream_t st, T* e, double rr0, double rtol, int maxit, int* iters,
                double* rr_out) {
-    const GLevelHost& L = H.host[0];
-    const long long N = L.N;
-    GOp<T> o = make_op<T>(H, 0);
-    const int G = rblk(N);
-    T* z = (T*)H.z;
-    T* q = (T*)H.q;
-    int pc = 0;
-    // beta = 0 on the first iteration multiplies p_old: it must hold finite values
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N)), dim3(256), 0, st, N, (T*)H.p[0], (T)0);
-    vcycle<T>(H, st, 0, r, z);
-    hipLaunchKernelGGL(k_dot3<T>, dim3(G), dim3(256), 0, st, N, r, z, z, H.part);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_DOT3_FIRST, H.scal, 1.0 / N,
-                       (int)H.singular);
+    const long long N = H.host[0].N;
+    ph_fcg_start<T>(H, st, nullptr);
     int it = 0;
     double rr = rr0;
     const int check_every = N > (1 << 20) ? 1 : 2;
+    int pc = 0;
     for (it = 1; it <= maxit; ++it) {
-        T* pold = (T*)H.p[pc];
-        T* pnew = (T*)H.p[1 - pc];
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G), dim3(256), 0, st, o,
-                                                       (const T*)z, (const T*)pold, pnew, q, H.scal, H.part));
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_PQ, H.scal, 1.0 / N,
-                           (int)H.singular);
-        hipLaunchKernelGGL(k_axpy2<T>, dim3(G), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_RR, H.scal, 1.0 / N,
-                           (int)H.singular);
+        ph_fcg_iter<T>(H, st, nullptr, e, pc);
         pc = 1 - pc;
         if (it % check_every == 0 || it == maxit) {
             OPF_TRY(read_scal(H, st));
             rr = H.hscal[S_RR];
             if (!(rr > rtol * rtol * rr0) || !std::isfinite(rr)) break;
         }
-        vcycle<T>(H, st, 0, r, z);
-        hipLaunchKernelGGL(k_dot3<T>, dim3(G), dim3(256), 0, st, N, r, z, q, H.part);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, H.part, G, (int)FIN_DOT3, H.scal, 1.0 / N,
-                           (int)H.singular);
     }
     OPF_HIP(hipGetLastError());
     *iters = std::min(it, maxit);
@@ -752,56 +877,35 @@ static int fcg(const Hierarchy& H, hipStream_t st, T* e, T* r, double rr0, doubl
     return OPF_OK;
 }
 
-template <typename T>
-static void launch_resid64(const Hierarchy& H, hipStream_t st, const double* x, const double* b, T* r) {
-    GOp<double> o = make_op<double>(H, 0);
-    const long long N = H.host[0].N;
-    const int G = rblk(N);
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, T, V_, D_>), dim3(G), dim3(256), 0, st, o, x, b,
-                                                   r, H.part));
-}
-
 int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,
               int verbose) {
+    OPF_CHECK(H.B == 1, OPF_ERR_ARG, "gmg_solve drives single-slot hierarchies");
     hipStream_t s = ctx->stream;
-    const long long N = H.host[0].N;
-    const int G = rblk(N);
-    // ||b||
-    launch_resid64<double>(H, s, nullptr, b, (double*)nullptr);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_BB, H.scal, 1.0 / N, (int)H.singular);
+    ph_resid64<double>(H, s, nullptr, nullptr, b, (double*)nullptr, FIN_BB);
     int total = 0, nref = 0;
     double rr = 0, bb = 0;
     if (H.precision == OPF_PREC_FP64) {
         double* r = (double*)H.r;
-        launch_resid64<double>(H, s, x, b, r);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N,
-                           (int)H.singular);
+        ph_resid64<double>(H, s, nullptr, x, b, r, FIN_RR0);
         OPF_TRY(read_scal(H, s));
         bb = H.hscal[S_BB];
         double rr0 = H.hscal[S_RR0];
-        if (H.singular) {  // project the residual onto the range of A
-            hipLaunchKernelGGL(k_sum<double>, dim3(G), dim3(256), 0, s, N, r, H.part);
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N, 1);
-            hipLaunchKernelGGL(k_sub_mean<double>, dim3(nblk(N)), dim3(256), 0, s, N, r, H.scal, (int)S_RR0, 1.0 / N);
-        }
+        if (H.singular) ph_project<double>(H, s, nullptr, r);
         double target = tol * tol * bb;
         if (rr0 > target && bb > 0) {
             int its = 0;
             double rrf = 0;
-            OPF_TRY(fcg<double>(H, s, x, r, rr0, std::sqrt(target / rr0), maxit, &its, &rrf));
+            OPF_TRY(fcg<double>(H, s, x, rr0, std::sqrt(target / rr0), maxit, &its, &rrf));
             total = its;
         }
         nref = 1;
     } else {
         // iterative refinement: FP64 residual / update, FP32 inner FCG
-        float* r = (float*)H.r;
         float* e = (float*)H.e;
         bool first = true;
         double target = 0;
         for (nref = 0; nref < 40; ++nref) {
-            launch_resid64<float>(H, s, x, b, r);
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N,
-                               (int)H.singular);
+            ph_resid64<float>(H, s, nullptr, x, b, (float*)H.r, FIN_RR0);
             OPF_TRY(read_scal(H, s));
             if (first) {
                 bb = H.hscal[S_BB];
@@ -811,25 +915,19 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
             double rr0 = H.hscal[S_RR0];
             if (verbose) fprintf(stderr, "[gmg mixed] refine %d relres %.3e\n", nref, std::sqrt(rr0 / bb));
             if (!(rr0 > target) || bb == 0 || total >= maxit) break;
-            if (H.singular) {
-                hipLaunchKernelGGL(k_sum<float>, dim3(G), dim3(256), 0, s, N, r, H.part);
-                hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR0, H.scal, 1.0 / N, 1);
-                hipLaunchKernelGGL(k_sub_mean<float>, dim3(nblk(N)), dim3(256), 0, s, N, r, H.scal, (int)S_RR0,
-                                   1.0 / N);
-            }
-            hipLaunchKernelGGL(k_fill<float>, dim3(nblk(N)), dim3(256), 0, s, N, e, 0.0f);
+            if (H.singular) ph_project<float>(H, s, nullptr, (float*)H.r);
+            ph_zero<float>(H, s, nullptr, e);
             int its = 0;
             double rrf = 0;
             // reduce the residual by ~1e-4 per refinement (FP32 resolves ~1e-6)
             double inner = std::max(1e-4, std::sqrt(target / rr0) * 0.5);
-            OPF_TRY(fcg<float>(H, s, e, r, rr0, std::min(inner, 0.5), std::max(1, maxit - total), &its, &rrf));
+            OPF_TRY(fcg<float>(H, s, e, rr0, std::min(inner, 0.5), std::max(1, maxit - total), &its, &rrf));
             total += its;
-            hipLaunchKernelGGL(k_add_to64<float>, dim3(nblk(N)), dim3(256), 0, s, N, x, (const float*)e);
+            ph_add64<float>(H, s, nullptr, x, (const float*)e);
         }
     }
     // final true residual in FP64
-    launch_resid64<double>(H, s, x, b, H.r64);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, H.part, G, (int)FIN_RR, H.scal, 1.0 / N, (int)H.singular);
+    ph_resid64<double>(H, s, nullptr, x, b, H.r64, FIN_RR);
     OPF_TRY(read_scal(H, s));
     bb = H.hscal[S_BB];
     rr = H.hscal[S_RR];
@@ -847,10 +945,23 @@ int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y) {
     GOp<double> o = make_op<double>(H, 0);
     const long long N = H.host[0].N;
     GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_apply<double, V_, D_>), dim3(nblk(N)), dim3(256), 0,
-                                                   ctx->stream, o, x, y));
+                                                  ctx->stream, o, x, y));
     OPF_HIP(hipGetLastError());
     return OPF_OK;
 }
+
+// explicit instantiations used by the serving loop
+template void ph_fcg_start<float>(const Hierarchy&, hipStream_t, const int*);
+template void ph_fcg_start<double>(const Hierarchy&, hipStream_t, const int*);
+template void ph_fcg_iter<float>(const Hierarchy&, hipStream_t, const int*, float*, int);
+template void ph_fcg_iter<double>(const Hierarchy&, hipStream_t, const int*, double*, int);
+template void ph_resid64<float>(const Hierarchy&, hipStream_t, const int*, const double*, const double*, float*, int);
+template void ph_resid64<double>(const Hierarchy&, hipStream_t, const int*, const double*, const double*, double*,
+                                 int);
+template void ph_add64<float>(const Hierarchy&, hipStream_t, const int*, double*, const float*);
+template void ph_add64<double>(const Hierarchy&, hipStream_t, const int*, double*, const double*);
+template void ph_zero<float>(const Hierarchy&, hipStream_t, const int*, float*);
+template void ph_zero<double>(const Hierarchy&, hipStream_t, const int*, double*);
 
 }  // namespace opf
 

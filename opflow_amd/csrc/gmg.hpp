@@ -17,6 +17,8 @@
 namespace opf {
 
 constexpr int GMG_MAXB = 2048;  // max blocks of a reduction kernel (partials per value)
+constexpr int GMG_SSTRIDE = 16;            // scalars per slot
+constexpr int GMG_PSTRIDE = 4 * GMG_MAXB;  // reduction partials per slot
 
 // Operator of one level as seen by the kernels (T = float or double).
 template <typename T>
@@ -60,6 +62,7 @@ struct Hierarchy {
     double alpha[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
     bool var = false;
     bool singular = false;
+    int B = 1;  // request slots sharing this hierarchy (vectors are slot-major)
     int precision = OPF_PREC_FP64;  // V-cycle / inner-Krylov precision
     double omega = 0.8;
     int pre = 2, post = 2;
@@ -82,7 +85,22 @@ struct Hierarchy {
 };
 
 int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
-              int precision);
+              int precision, int slots = 1);
+int read_scal(const Hierarchy& H, hipStream_t st);
+// batched phases over all slots selected by a device mask (null: all slots)
+template <typename T>
+void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask);
+template <typename T>
+void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int pc);
+template <typename TO>
+void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const double* x, const double* b, TO* r,
+                int mode);
+template <typename TE>
+void ph_add64(const Hierarchy& H, hipStream_t st, const int* mask, double* x, const TE* e);
+template <typename T>
+void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v);
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_RR, S_BETA, S_ZQ, S_ZMEAN, S_BB, S_RR0, S_N };
+enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0 };
 void gmg_free(Hierarchy& H);
 // solve A x = b on the stream: b, x FP64 device arrays (N each). Returns iterations etc.
 int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,

@@ -1,0 +1,127 @@
+// C++ restatement of the reference's equation tests (test/Core/Equation/
+// DircEqnTest.cpp, NeumEqnTest.cpp, PeriodicEqnTest.cpp) compiled against the
+// engine's OpFlow front-end. Prints one PASS/FAIL line per case; exit code 0
+// when all pass.
+#include <OpFlow>
+
+#include <cmath>
+#include <cstdio>
+
+using namespace OpFlow;
+using Mesh = CartesianMesh<Meta::int_<2>>;
+using Field = CartesianField<Real, Mesh>;
+
+static int failures = 0;
+
+struct Case {
+    Mesh m;
+    Field p, r, b, p_true;
+    Case(BCType bc, int n, double hi, double pv, double rv) {
+        m = MeshBuilder<Mesh>().newMesh(n, n).setMeshOfDim(0, 0., hi).setMeshOfDim(1, 0., hi).build();
+        auto mk = [&](const char* name, double v) {
+            auto bld = ExprBuilder<Field>().setMesh(m).setName(name);
+            if (bc == BCType::Periodic)
+                for (int d = 0; d < 2; ++d) bld.setBC(d, DimPos::start, bc).setBC(d, DimPos::end, bc);
+            else
+                for (int d = 0; d < 2; ++d) bld.setBC(d, DimPos::start, bc, v).setBC(d, DimPos::end, bc, v);
+            return bld.setExt(1).setLoc({LocOnMesh::Center, LocOnMesh::Center}).build();
+        };
+        p = mk("p", pv);
+        r = mk("r", rv);
+        p_true = p;
+        p_true.name = "ptrue";
+        b = p;
+        b.name = "b";
+    }
+    void reset_case(double xc, double yc, double rout) {
+        r.initBy([&](auto&& x) {
+            auto dist = Math::norm2(x[0] - xc, x[1] - yc);
+            auto hevi = Math::smoothHeviside(r.getMesh().dx(0, 0) * 8, dist - 0.2);
+            return 1. * hevi + (1. - hevi) * rout;
+        });
+        b = dx<D1FirstOrderCentered>(dx<D1FirstOrderCentered>(p_true) / d1IntpCenterToCorner<0>((r))) +
+            dy<D1FirstOrderCentered>(dy<D1FirstOrderCentered>(p_true) / d1IntpCenterToCorner<1>(r));
+        p = 0.;
+    }
+    auto poisson_eqn() {
+        return [&](auto&& e) {
+            return b == dx<D1FirstOrderCentered>(dx<D1FirstOrderCentered>(e) / d1IntpCenterToCorner<0>((r))) +
+                                dy<D1FirstOrderCentered>(dy<D1FirstOrderCentered>(e) / d1IntpCenterToCorner<1>(r));
+        };
+    }
+    double max_rel_err() {
+        double e = 0;
+        for (int j = p.assignableRange.start[1]; j < p.assignableRange.end[1]; ++j)
+            for (int i = p.assignableRange.start[0]; i < p.assignableRange.end[0]; ++i) {
+                DS::MDIndex<2> k{i, j};
+                double res = p.evalAt(k) - p_true.evalAt(k);
+                e = std::max(e, std::abs(res) / std::abs(p_true.evalAt(k)));
+            }
+        return e;
+    }
+};
+
+static void report(const char* name, double err, double tol, const EqnSolveState& st) {
+    bool ok = err <= tol && std::isfinite(err);
+    failures += !ok;
+    std::printf("%s %s: max rel err %.3e (tol %.1e), iters %d, relres %.3e\n", ok ? "PASS" : "FAIL", name, err, tol,
+                st.niter, st.relerr);
+}
+
+int main() {
+    {  // DircEqnTest::ManualUnifiedSolve
+        Case c(BCType::Dirc, 65, 1.0, 0.0, 1.0);
+        c.p_true.initBy([](auto&& x) { return x[0] * (1. - x[0]) * x[1] * (1. - x[1]); });
+        c.reset_case(0.5, 0.5, 1000);
+        StructSolverParams<StructSolverType::GMRES> params;
+        params.tol = 1e-10;
+        StructSolverParams<StructSolverType::PFMG> p_params;
+        auto st = Solve(c.poisson_eqn(), c.p, params, p_params);
+        report("DircEqnTest.ManualUnifiedSolve", c.max_rel_err(), 1e-8, st);
+        // DefaultHandlerSolve pattern, solved twice with a static matrix
+        params.staticMat = true;
+        auto solver = PrecondStructSolver<StructSolverType::GMRES, StructSolverType::PFMG>(params, p_params);
+        auto handler = makeEqnSolveHandler(c.poisson_eqn(), c.p, solver);
+        c.p = 0.;
+        handler->solve();
+        c.p = 0.;
+        auto st2 = handler->solve();
+        report("DircEqnTest.HandlerSolveTwice", c.max_rel_err(), 1e-8, st2);
+    }
+    {  // NeumEqnTest::ManualUnifiedSolve (pinValue, mean removed)
+        Case c(BCType::Neum, 33, 2 * PI, 0.0, 0.0);
+        c.p_true.initBy([](auto&& x) { return std::cos(x[0]) * std::cos(x[1]); });
+        auto ave = rangeReduce(c.p.assignableRange, SumOp{}, c.p_true);
+        c.p_true -= ave / c.p.assignableRange.count();
+        c.reset_case(0.5, 0.5, 1);
+        StructSolverParams<StructSolverType::GMRES> params;
+        params.tol = 1e-10;
+        params.pinValue = true;
+        StructSolverParams<StructSolverType::PFMG> p_params;
+        auto st = Solve(c.poisson_eqn(), c.p, params, p_params);
+        ave = rangeReduce(c.p.assignableRange, SumOp{}, c.p);
+        c.p -= ave / c.p.assignableRange.count();
+        report("NeumEqnTest.ManualUnifiedSolve", c.max_rel_err(), 5e-8, st);
+    }
+    {  // PeriodicEqnTest::ManualUnifiedSolve
+        Case c(BCType::Periodic, 33, 2 * PI, 0.0, 0.0);
+        c.p_true.initBy([](auto&& x) { return std::sin(x[0]) * std::cos(x[1]); });
+        c.reset_case(0.5, 0.5, 1);
+        StructSolverParams<StructSolverType::GMRES> params;
+        params.tol = 1e-10;
+        StructSolverParams<StructSolverType::PFMG> p_params;
+        auto st = Solve(c.poisson_eqn(), c.p, params, p_params);
+        report("PeriodicEqnTest.ManualUnifiedSolve", c.max_rel_err(), 1e-9, st);
+    }
+    {  // mixed-precision solve through the same API
+        Case c(BCType::Dirc, 65, 1.0, 0.0, 1.0);
+        c.p_true.initBy([](auto&& x) { return x[0] * (1. - x[0]) * x[1] * (1. - x[1]); });
+        c.reset_case(0.5, 0.5, 1000);
+        StructSolverParams<StructSolverType::PCG> params;
+        params.tol = 1e-10;
+        params.mixedPrecision = true;
+        auto st = Solve(c.poisson_eqn(), c.p, params);
+        report("DircEqnTest.MixedPrecision", c.max_rel_err(), 1e-8, st);
+    }
+    return failures ? 1 : 0;
+}
