@@ -289,6 +289,8 @@ typedef struct {
     int nbumps;
     double noise;
     const double* rhs;   /* optional host RHS (prod(n) doubles); NULL -> generated  */
+    double* out_x;       /* optional host buffer: the solution is copied here when done */
+    double t_submit;     /* arrival time on the server clock (< 0: now)             */
 } opf_srv_request;
 typedef struct {
     uint64_t id;
@@ -297,13 +299,13 @@ typedef struct {
     double t_submit, t_admit, t_done; /* seconds on the server clock             */
     double checksum;     /* sum of the solution                                     */
 } opf_srv_result;
+/* bumps (+ noise) on the cell centres (h*i + h/2) of the unit-cube grid of d, x-fastest */
+int opf_gen_bumps_grid(const opf_poisson_desc* d, uint64_t seed, int nbumps, double noise, double* out);
 int opf_srv_create(opf_ctx* ctx, const opf_srv_config* cfg, opf_srv** out);
 int opf_srv_submit(opf_srv* s, const opf_srv_request* req, uint64_t* id);
 /* advance the device work by one scheduling round; returns number of finished */
 int opf_srv_step(opf_srv* s, int* nfinished);
 int opf_srv_poll(opf_srv* s, opf_srv_result* out, int max, int* n);
-/* copy a finished request's solution (prod(n) doubles) to host; call before poll releases it */
-int opf_srv_fetch(opf_srv* s, uint64_t id, double* host_x, size_t n);
 int opf_srv_pending(opf_srv* s, int* queued, int* inflight);
 double opf_srv_clock(opf_srv* s);
 int opf_srv_destroy(opf_srv* s);

@@ -502,4 +502,4 @@ def gen_uniform(seed, n, lo=0.0, hi=1.0):
 
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
-                    PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc)
+                    PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc, gen_bumps_grid)

@@ -75,7 +75,7 @@ class SrvConfig(C.Structure):
 
 class SrvRequest(C.Structure):
     _fields_ = [("desc", PoissonDesc), ("seed", C.c_uint64), ("nbumps", C.c_int), ("noise", C.c_double),
-                ("rhs", C.POINTER(C.c_double))]
+                ("rhs", C.POINTER(C.c_double)), ("out_x", C.POINTER(C.c_double)), ("t_submit", C.c_double)]
 
 
 class SrvResult(C.Structure):
@@ -136,7 +136,7 @@ SIGNATURES = {
     "opf_srv_submit": (_I, [_P, C.POINTER(SrvRequest), C.POINTER(C.c_uint64)]),
     "opf_srv_step": (_I, [_P, _IP]),
     "opf_srv_poll": (_I, [_P, C.POINTER(SrvResult), _I, _IP]),
-    "opf_srv_fetch": (_I, [_P, C.c_uint64, _DP, C.c_size_t]),
+    "opf_gen_bumps_grid": (_I, [C.POINTER(PoissonDesc), C.c_uint64, _I, _D, _DP]),
     "opf_srv_pending": (_I, [_P, _IP, _IP]),
     "opf_srv_clock": (_D, [_P]),
     "opf_srv_destroy": (_I, [_P]),

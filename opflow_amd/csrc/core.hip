@@ -723,6 +723,40 @@ extern "C" int opf_gen_uniform(uint64_t seed, size_t n, double lo, double hi, do
     return OPF_OK;
 }
 
+namespace opf {
+// bumps on a unit-cube cell-centred grid (x-fastest, no ghosts); the same
+// generator and coordinates (h*i + 0.5*h) as opf_gen_bumps on a uniform mesh
+void gen_bumps_grid(int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out) {
+    SplitMix64 g{seed};
+    double c[64][3], w[64], A[64];
+    nbumps = std::min(std::max(nbumps, 0), 64);
+    for (int b = 0; b < nbumps; ++b) {
+        for (int k = 0; k < 3; ++k) c[b][k] = k < dim ? 0.2 + 0.6 * g.uniform() : 0.0;
+        w[b] = 0.03 + 0.07 * g.uniform();
+        A[b] = 0.5 + 0.5 * g.uniform();
+    }
+    int nn[3] = {n[0], dim > 1 ? n[1] : 1, dim > 2 ? n[2] : 1};
+    double h[3];
+    for (int q = 0; q < 3; ++q) h[q] = 1.0 / nn[q];
+    size_t l = 0;
+    for (int k = 0; k < nn[2]; ++k)
+        for (int j = 0; j < nn[1]; ++j)
+            for (int i = 0; i < nn[0]; ++i, ++l) {
+                int idx[3] = {i, j, k};
+                double xc[3];
+                for (int q = 0; q < 3; ++q) xc[q] = h[q] * idx[q] + .5 * h[q];
+                double v = 0.0;
+                for (int b = 0; b < nbumps; ++b) {
+                    double r2 = 0.0;
+                    for (int q = 0; q < dim; ++q) r2 += (xc[q] - c[b][q]) * (xc[q] - c[b][q]);
+                    v += A[b] * std::exp(-r2 / (2.0 * w[b] * w[b]));
+                }
+                if (noise != 0.0) v += noise * g.normal();
+                out[l] = v;
+            }
+}
+}  // namespace opf
+
 extern "C" int opf_gen_bumps(const opf_mesh* m, const opf_field_desc* d, uint64_t seed, int nbumps, double noise,
                              double* out) {
     OPF_CHECK(m && d && out && nbumps >= 0 && nbumps <= 64, OPF_ERR_ARG, "opf_gen_bumps: bad args");

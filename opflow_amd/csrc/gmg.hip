@@ -676,10 +676,8 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
     OPF_HIP(hipMalloc(&H.p[1], ts * BN));
     OPF_HIP(hipMalloc(&H.q, ts * BN));
     bytes += 5 * ts * BN;
-    if (mixed) {
-        OPF_HIP(hipMalloc(&H.e, ts * BN));
-        bytes += ts * BN;
-    }
+    OPF_HIP(hipMalloc(&H.e, ts * BN));  // inner correction (mixed) / serving correction
+    bytes += ts * BN;
     OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * BN));
     bytes += sizeof(double) * BN;
     OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * GMG_PSTRIDE * H.B));

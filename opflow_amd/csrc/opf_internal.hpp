@@ -97,6 +97,9 @@ namespace opf {
 int build_ghost_info(const opf_field* f, GhostInfo* gi);
 int launch_ghost_fill(const opf_field* f, void* data);
 
+// seeded Gaussian bumps on a unit-cube cell-centred grid (host)
+void gen_bumps_grid(int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out);
+
 // device program cache
 int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev);
 

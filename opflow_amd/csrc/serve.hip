@@ -1,0 +1,470 @@
+// Continuous-batching server for Poisson / backward-Euler heat solves.
+//
+// Requests are grouped by operator (grid shape, spacing, BCs, diagonal
+// shift). Each group owns one resident multigrid hierarchy with B request
+// slots whose vectors are stored slot after slot, so every smoothing,
+// restriction, prolongation and Krylov kernel of a V-cycle runs all of the
+// group's slots in one launch (blockIdx.y = slot, a device mask selects the
+// slots taking part). Groups stay resident in HBM after their slots drain
+// and are reused by later requests of the same shape; an empty group is
+// evicted only when a new shape needs its memory.
+//
+// Each slot runs FP32-inner / FP64-refinement: an FP64 residual r = b - A x,
+// an FP32 flexible-CG inner solve on A e = r, x += e. One scheduling round
+//   1. admits queued requests into free slots while the memory budget allows,
+//   2. runs the FP64 outer step for the slots that need it (batched),
+//   3. replays the group's captured hipGraph of two batched inner iterations,
+//   4. reads every slot's scalars with one copy and advances the state machines.
+// Requests that arrive while others run are admitted in the next round, so
+// their work overlaps on the device with requests already in flight.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "gmg.hpp"
+
+using namespace opf;
+
+namespace {
+
+enum Phase { FREE = 0, NEWREQ, OUTER, ITER };
+enum { M_NEW = 0, M_ADD, M_OUTER, M_START, M_ITER, M_COUNT };
+
+struct Pending {
+    uint64_t id;
+    opf_srv_request req;
+    std::vector<double> rhs;
+    double t_submit;
+};
+
+struct Slot {
+    int phase = FREE;
+    uint64_t id = 0;
+    double t_submit = 0, t_admit = 0;
+    double bb = 0, target = 0, rr0 = 0;
+    int its = 0, nref = 0, inner_its = 0;
+    bool pending_add = false;
+    double* out_x = nullptr;
+};
+
+struct Group {
+    opf_poisson_desc d{};
+    Hierarchy H;
+    int B = 1;
+    long long N = 0;
+    double* x64 = nullptr;
+    double* b64 = nullptr;
+    int* dmask = nullptr;   // [M_COUNT][B]
+    int* hmask = nullptr;   // pinned mirror
+    std::vector<Slot> slots;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t bytes = 0;
+    double last_used = 0;
+    int busy() const {
+        int c = 0;
+        for (auto& s : slots) c += s.phase != FREE;
+        return c;
+    }
+};
+
+bool same_op(const opf_poisson_desc& a, const opf_poisson_desc& b) {
+    if (a.dim != b.dim || a.sigma != b.sigma) return false;
+    for (int q = 0; q < 3; ++q) {
+        if (a.n[q] != b.n[q] || a.h[q] != b.h[q]) return false;
+        if (a.bc[q][0] != b.bc[q][0] || a.bc[q][1] != b.bc[q][1]) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+struct opf_srv {
+    opf_ctx* ctx = nullptr;
+    opf_srv_config cfg{};
+    std::deque<Pending> queue;
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<opf_srv_result> done;
+    uint64_t next_id = 1;
+    size_t used = 0;
+    int inflight = 0;
+    std::chrono::steady_clock::time_point t0;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+int slots_for(const opf_srv& s, long long N) {
+    int B = N <= (1 << 18) ? 8 : N <= (1 << 21) ? 4 : N <= (1 << 23) ? 2 : 1;
+    return std::max(1, std::min(B, s.cfg.max_inflight));
+}
+
+size_t group_bytes(const opf_poisson_desc& d, int precision, int B, long long N) {
+    return opf_gmg_footprint(&d, precision) * (size_t)B + (size_t)B * N * 16 + 4096;
+}
+
+void free_group(Group* g) {
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    gmg_free(g->H);
+    if (g->x64) hipFree(g->x64);
+    if (g->b64) hipFree(g->b64);
+    if (g->dmask) hipFree(g->dmask);
+    if (g->hmask) hipHostFree(g->hmask);
+}
+
+int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
+    int per[3] = {0, 0, 0};
+    double alpha[3][2];
+    GLevelHost L{};
+    for (int q = 0; q < 3; ++q) L.n[q] = q < d.dim ? d.n[q] : 1;
+    L.N = (long long)L.n[0] * L.n[1] * L.n[2];
+    L.var = false;
+    L.sigma = d.sigma;
+    for (int q = 0; q < 3; ++q) {
+        alpha[q][0] = alpha[q][1] = 1.0;
+        L.a[q] = q < d.dim ? 1.0 / (d.h[q] * d.h[q]) : 0.0;
+        if (q >= d.dim) continue;
+        for (int side = 0; side < 2; ++side) {
+            OPF_CHECK(d.bc[q][side] == OPF_BC_DIRC || d.bc[q][side] == OPF_BC_NEUM || d.bc[q][side] == OPF_BC_PERIODIC,
+                      OPF_ERR_UNSUPPORTED, "serve: bc unsupported");
+            alpha[q][side] = d.bc[q][side] == OPF_BC_DIRC ? -1.0 : 1.0;
+        }
+        per[q] = d.bc[q][0] == OPF_BC_PERIODIC;
+    }
+    auto g = std::make_unique<Group>();
+    g->d = d;
+    g->N = L.N;
+    g->B = slots_for(*s, L.N);
+    int prec = s->cfg.precision == OPF_PREC_MIXED ? OPF_PREC_MIXED : OPF_PREC_FP64;
+    int rc = gmg_build(s->ctx, g->H, L, per, alpha, prec, g->B);
+    if (rc != OPF_OK) {
+        free_group(g.get());
+        return rc;
+    }
+    if (g->H.singular) {
+        free_group(g.get());
+        set_error("serve: singular operators (no Dirichlet face, sigma = 0) are not served");
+        return OPF_ERR_UNSUPPORTED;
+    }
+    const size_t BN = (size_t)g->B * g->N;
+    if (hipMalloc(&g->x64, 8 * BN) != hipSuccess || hipMalloc(&g->b64, 8 * BN) != hipSuccess ||
+        hipMalloc(&g->dmask, sizeof(int) * M_COUNT * g->B) != hipSuccess ||
+        hipHostMalloc(&g->hmask, sizeof(int) * M_COUNT * g->B, hipHostMallocDefault) != hipSuccess) {
+        free_group(g.get());
+        set_error("serve: out of device memory");
+        return OPF_ERR_NOMEM;
+    }
+    g->slots.assign(g->B, Slot{});
+    g->bytes = g->H.bytes + 16 * BN;
+    *out = g.get();
+    s->used += g->bytes;
+    s->groups.push_back(std::move(g));
+    return OPF_OK;
+}
+
+// capture two batched inner iterations (the p ping-pong returns to its start)
+int capture(opf_srv* s, Group* g) {
+    if (!s->cfg.batched || g->exec) return OPF_OK;
+    hipStream_t st = s->ctx->stream;
+    const int* m = g->dmask + M_ITER * g->B;
+    OPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    if (g->H.precision == OPF_PREC_MIXED) {
+        ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, 0);
+        ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, 1);
+    } else {
+        ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, 0);
+        ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, 1);
+    }
+    OPF_HIP(hipStreamEndCapture(st, &g->graph));
+    OPF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    return OPF_OK;
+}
+
+int try_admit(opf_srv* s) {
+    hipStream_t st = s->ctx->stream;
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        if (s->inflight >= s->cfg.max_inflight) break;
+        Pending& p = *it;
+        Group* g = nullptr;
+        for (auto& gp : s->groups)
+            if (same_op(gp->d, p.req.desc) && gp->busy() < gp->B) {
+                g = gp.get();
+                break;
+            }
+        if (!g) {
+            long long N = 1;
+            for (int q = 0; q < p.req.desc.dim; ++q) N *= p.req.desc.n[q];
+            int B = slots_for(*s, N);
+            size_t need = group_bytes(p.req.desc, s->cfg.precision, B, N);
+            // evict idle groups (least recently used first) until the new one fits
+            while (s->used + need > s->cfg.mem_budget) {
+                int victim = -1;
+                double oldest = 1e300;
+                for (size_t q = 0; q < s->groups.size(); ++q)
+                    if (s->groups[q]->busy() == 0 && s->groups[q]->last_used < oldest) {
+                        oldest = s->groups[q]->last_used;
+                        victim = (int)q;
+                    }
+                if (victim < 0) break;
+                OPF_HIP(hipStreamSynchronize(st));
+                s->used -= s->groups[victim]->bytes;
+                free_group(s->groups[victim].get());
+                s->groups.erase(s->groups.begin() + victim);
+            }
+            if (s->used + need > s->cfg.mem_budget) {
+                OPF_CHECK(!s->groups.empty() || s->inflight > 0, OPF_ERR_NOMEM,
+                          "serve: request needs %zu bytes, budget %zu", need, s->cfg.mem_budget);
+                ++it;  // does not fit now: later requests may
+                continue;
+            }
+            OPF_TRY(make_group(s, p.req.desc, &g));
+        }
+        int k = 0;
+        while (g->slots[k].phase != FREE) ++k;
+        Slot& sl = g->slots[k];
+        sl = Slot{};
+        sl.phase = NEWREQ;
+        sl.id = p.id;
+        sl.t_submit = p.t_submit;
+        sl.t_admit = s->now();
+        sl.out_x = p.req.out_x;
+        OPF_HIP(hipMemcpyAsync(g->b64 + (size_t)k * g->N, p.rhs.data(), 8 * g->N, hipMemcpyHostToDevice, st));
+        OPF_HIP(hipMemsetAsync(g->x64 + (size_t)k * g->N, 0, 8 * g->N, st));
+        OPF_HIP(hipStreamSynchronize(st));  // the host RHS buffer is released below
+        ++s->inflight;
+        it = s->queue.erase(it);
+    }
+    return OPF_OK;
+}
+
+void set_masks(Group* g) {
+    for (int m = 0; m < M_COUNT; ++m)
+        for (int k = 0; k < g->B; ++k) g->hmask[m * g->B + k] = 0;
+    for (int k = 0; k < g->B; ++k) {
+        const Slot& sl = g->slots[k];
+        if (sl.phase == NEWREQ) {
+            g->hmask[M_NEW * g->B + k] = 1;
+            g->hmask[M_OUTER * g->B + k] = 1;
+        }
+        if (sl.phase == OUTER) {
+            g->hmask[M_OUTER * g->B + k] = 1;
+            if (sl.pending_add) g->hmask[M_ADD * g->B + k] = 1;
+        }
+        if (sl.phase == ITER) g->hmask[M_ITER * g->B + k] = 1;
+    }
+}
+
+int round(opf_srv* s, int* nfinished) {
+    hipStream_t st = s->ctx->stream;
+    *nfinished = 0;
+    OPF_TRY(try_admit(s));
+    const double tol = s->cfg.tol > 0 ? s->cfg.tol : 1e-8;
+    const int maxit = s->cfg.max_iter > 0 ? s->cfg.max_iter : 200;
+    // ---- outer FP64 step for NEW / OUTER slots
+    bool any_outer = false;
+    for (auto& gp : s->groups) {
+        Group* g = gp.get();
+        bool need = false;
+        for (auto& sl : g->slots) need |= sl.phase == NEWREQ || sl.phase == OUTER;
+        if (!need) continue;
+        any_outer = true;
+        set_masks(g);
+        OPF_HIP(hipMemcpyAsync(g->dmask, g->hmask, sizeof(int) * M_COUNT * g->B, hipMemcpyHostToDevice, st));
+        const int* mk = g->dmask;
+        ph_resid64<double>(g->H, st, mk + M_NEW * g->B, nullptr, g->b64, (double*)nullptr, FIN_BB);
+        if (g->H.precision == OPF_PREC_MIXED) {
+            ph_add64<float>(g->H, st, mk + M_ADD * g->B, g->x64, (const float*)g->H.e);
+            ph_resid64<float>(g->H, st, mk + M_OUTER * g->B, g->x64, g->b64, (float*)g->H.r, FIN_RR0);
+        } else {
+            ph_add64<double>(g->H, st, mk + M_ADD * g->B, g->x64, (const double*)g->H.e);
+            ph_resid64<double>(g->H, st, mk + M_OUTER * g->B, g->x64, g->b64, (double*)g->H.r, FIN_RR0);
+        }
+    }
+    if (any_outer) {
+        for (auto& gp : s->groups) {
+            Group* g = gp.get();
+            bool need = false;
+            for (auto& sl : g->slots) need |= sl.phase == NEWREQ || sl.phase == OUTER;
+            if (need)
+                OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B,
+                                       hipMemcpyDeviceToHost, st));
+        }
+        OPF_HIP(hipStreamSynchronize(st));
+        for (auto& gp : s->groups) {
+            Group* g = gp.get();
+            bool start = false;
+            for (int k = 0; k < g->B; ++k) {
+                Slot& sl = g->slots[k];
+                if (sl.phase != NEWREQ && sl.phase != OUTER) continue;
+                const double* sc = g->H.hscal + (size_t)k * GMG_SSTRIDE;
+                if (sl.phase == NEWREQ) {
+                    sl.bb = sc[S_BB];
+                    sl.target = tol * tol * sl.bb;
+                }
+                sl.rr0 = sc[S_RR0];
+                sl.pending_add = false;
+                if (!(sl.rr0 > sl.target) || sl.bb == 0 || sl.its >= maxit || sl.nref >= 40) {
+                    // finished: report, copy out, free the slot
+                    opf_srv_result r{};
+                    r.id = sl.id;
+                    r.status = OPF_OK;
+                    r.state.niter = sl.its;
+                    r.state.nrefine = sl.nref;
+                    r.state.abserr = std::sqrt(sl.rr0);
+                    r.state.relerr = sl.bb > 0 ? std::sqrt(sl.rr0 / sl.bb) : 0.0;
+                    r.state.converged = r.state.relerr <= tol * (1 + 1e-12) || sl.bb == 0;
+                    r.t_submit = sl.t_submit;
+                    r.t_admit = sl.t_admit;
+                    if (sl.out_x)
+                        OPF_HIP(hipMemcpyAsync(sl.out_x, g->x64 + (size_t)k * g->N, 8 * g->N, hipMemcpyDeviceToHost,
+                                               st));
+                    r.t_done = s->now();
+                    s->done.push_back(r);
+                    sl.phase = FREE;
+                    --s->inflight;
+                    ++*nfinished;
+                    g->last_used = s->now();
+                    continue;
+                }
+                sl.phase = ITER;
+                sl.nref++;
+                sl.inner_its = 0;
+                g->hmask[M_START * g->B + k] = 1;
+                start = true;
+            }
+            if (start) {
+                OPF_HIP(hipMemcpyAsync(g->dmask + M_START * g->B, g->hmask + M_START * g->B, sizeof(int) * g->B,
+                                       hipMemcpyHostToDevice, st));
+                const int* m = g->dmask + M_START * g->B;
+                if (g->H.precision == OPF_PREC_MIXED) {
+                    ph_zero<float>(g->H, st, m, (float*)g->H.e);
+                    ph_fcg_start<float>(g->H, st, m);
+                } else {
+                    ph_zero<double>(g->H, st, m, (double*)g->H.e);
+                    ph_fcg_start<double>(g->H, st, m);
+                }
+            }
+        }
+        OPF_HIP(hipStreamSynchronize(st));  // out_x copies complete before results are polled
+    }
+    // ---- batched inner iterations (graph replay)
+    bool any_iter = false;
+    for (auto& gp : s->groups) {
+        Group* g = gp.get();
+        bool need = false;
+        for (auto& sl : g->slots) need |= sl.phase == ITER;
+        if (!need) continue;
+        any_iter = true;
+        set_masks(g);
+        OPF_HIP(hipMemcpyAsync(g->dmask + M_ITER * g->B, g->hmask + M_ITER * g->B, sizeof(int) * g->B,
+                               hipMemcpyHostToDevice, st));
+        if (s->cfg.batched) {
+            OPF_TRY(capture(s, g));
+            OPF_HIP(hipGraphLaunch(g->exec, st));
+        } else {
+            const int* m = g->dmask + M_ITER * g->B;
+            for (int pc = 0; pc < 2; ++pc) {
+                if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
+                else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
+            }
+        }
+        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost,
+                               st));
+    }
+    if (any_iter) {
+        OPF_HIP(hipStreamSynchronize(st));
+        for (auto& gp : s->groups) {
+            Group* g = gp.get();
+            for (int k = 0; k < g->B; ++k) {
+                Slot& sl = g->slots[k];
+                if (sl.phase != ITER) continue;
+                sl.its += 2;
+                sl.inner_its += 2;
+                double rr = g->H.hscal[(size_t)k * GMG_SSTRIDE + S_RR];
+                // FP32 inner solve: reduce the residual by ~1e-4 per refinement (or reach the target)
+                double inner = std::max(1e-4, std::sqrt(sl.target / sl.rr0) * 0.5);
+                if (!(rr > inner * inner * sl.rr0) || !std::isfinite(rr) || sl.inner_its >= 40 || sl.its >= maxit) {
+                    sl.phase = OUTER;
+                    sl.pending_add = true;
+                }
+            }
+        }
+    }
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
+}
+
+}  // namespace
+
+extern "C" int opf_gen_bumps_grid(const opf_poisson_desc* d, uint64_t seed, int nbumps, double noise, double* out) {
+    OPF_CHECK(d && out && d->dim >= 1 && d->dim <= 3, OPF_ERR_ARG, "opf_gen_bumps_grid: bad args");
+    gen_bumps_grid(d->dim, d->n, seed, nbumps, noise, out);
+    return OPF_OK;
+}
+
+extern "C" int opf_srv_create(opf_ctx* ctx, const opf_srv_config* cfg, opf_srv** out) {
+    OPF_CHECK(ctx && cfg && out && cfg->max_inflight >= 1 && cfg->mem_budget > 0, OPF_ERR_ARG,
+              "opf_srv_create: bad args");
+    auto* s = new opf_srv();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return OPF_OK;
+}
+
+extern "C" int opf_srv_submit(opf_srv* s, const opf_srv_request* req, uint64_t* id) {
+    OPF_CHECK(s && req && id, OPF_ERR_ARG, "opf_srv_submit: null arg");
+    OPF_CHECK(req->desc.dim == 2 || req->desc.dim == 3, OPF_ERR_ARG, "opf_srv_submit: dim 2 or 3");
+    Pending p;
+    p.id = s->next_id++;
+    p.req = *req;
+    long long N = 1;
+    for (int q = 0; q < req->desc.dim; ++q) {
+        OPF_CHECK(req->desc.n[q] >= 2, OPF_ERR_ARG, "opf_srv_submit: n >= 2");
+        N *= req->desc.n[q];
+    }
+    p.rhs.resize(N);
+    if (req->rhs) std::copy(req->rhs, req->rhs + N, p.rhs.begin());
+    else gen_bumps_grid(req->desc.dim, req->desc.n, req->seed, req->nbumps, req->noise, p.rhs.data());
+    p.t_submit = req->t_submit >= 0 ? req->t_submit : s->now();
+    s->queue.push_back(std::move(p));
+    *id = s->queue.back().id;
+    return OPF_OK;
+}
+
+extern "C" int opf_srv_step(opf_srv* s, int* nfinished) {
+    OPF_CHECK(s && nfinished, OPF_ERR_ARG, "opf_srv_step: null arg");
+    return round(s, nfinished);
+}
+
+extern "C" int opf_srv_poll(opf_srv* s, opf_srv_result* out, int max, int* n) {
+    OPF_CHECK(s && n && (out || max == 0), OPF_ERR_ARG, "opf_srv_poll: null arg");
+    int k = std::min<int>(max, (int)s->done.size());
+    for (int q = 0; q < k; ++q) out[q] = s->done[q];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return OPF_OK;
+}
+
+extern "C" int opf_srv_pending(opf_srv* s, int* queued, int* inflight) {
+    OPF_CHECK(s && queued && inflight, OPF_ERR_ARG, "opf_srv_pending: null arg");
+    *queued = (int)s->queue.size();
+    *inflight = s->inflight;
+    return OPF_OK;
+}
+
+extern "C" double opf_srv_clock(opf_srv* s) { return s ? s->now() : 0.0; }
+
+extern "C" int opf_srv_destroy(opf_srv* s) {
+    if (!s) return OPF_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    for (auto& g : s->groups) free_group(g.get());
+    delete s;
+    return OPF_OK;
+}

@@ -112,6 +112,16 @@ def Solve(getter, target, params=None, precParams=None):
         h.close()
 
 
+def gen_bumps_grid(desc, seed, nbumps, noise=0.0):
+    """RHS of seeded bumps (+ noise) on the unit-cube cell centres of a Poisson descriptor."""
+    n = 1
+    for q in range(desc.dim):
+        n *= desc.n[q]
+    out = np.zeros(n)
+    L.check(L.load().opf_gen_bumps_grid(C.byref(desc), C.c_uint64(seed), nbumps, noise, out.ctypes.data_as(L._DP)))
+    return out
+
+
 def poisson_desc(dim, n, h, bc, sigma=0.0):
     d = L.PoissonDesc()
     d.dim = dim
@@ -186,14 +196,21 @@ class Server:
         h = C.c_void_p()
         L.check(self.lib.opf_srv_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h
+        self._outs = []  # host buffers the server writes into
 
-    def submit(self, desc, seed, nbumps, noise=0.0, rhs=None):
+    def submit(self, desc, seed, nbumps, noise=0.0, rhs=None, out=None, t_submit=-1.0):
+        """Queue a solve; `out` (float64 array of prod(n)) receives the solution when it finishes."""
         r = L.SrvRequest()
-        r.desc, r.seed, r.nbumps, r.noise = desc, seed, nbumps, noise
-        self._keep = None
+        r.desc, r.seed, r.nbumps, r.noise, r.t_submit = desc, seed, nbumps, noise, t_submit
+        keep = []
         if rhs is not None:
-            self._keep = np.ascontiguousarray(rhs, dtype=np.float64)
-            r.rhs = self._keep.ctypes.data_as(L._DP)
+            a = np.ascontiguousarray(rhs, dtype=np.float64)
+            keep.append(a)
+            r.rhs = a.ctypes.data_as(L._DP)
+        if out is not None:
+            assert out.dtype == np.float64 and out.flags["C_CONTIGUOUS"]
+            self._outs.append(out)
+            r.out_x = out.ctypes.data_as(L._DP)
         i = C.c_uint64()
         L.check(self.lib.opf_srv_submit(self.h, C.byref(r), C.byref(i)))
         return i.value
@@ -203,16 +220,11 @@ class Server:
         L.check(self.lib.opf_srv_step(self.h, C.byref(n)))
         return n.value
 
-    def poll(self, maxn=64):
+    def poll(self, maxn=256):
         arr = (L.SrvResult * maxn)()
         n = C.c_int()
         L.check(self.lib.opf_srv_poll(self.h, arr, maxn, C.byref(n)))
         return [arr[i] for i in range(n.value)]
-
-    def fetch(self, rid, n):
-        out = np.zeros(n)
-        L.check(self.lib.opf_srv_fetch(self.h, C.c_uint64(rid), out.ctypes.data_as(L._DP), n))
-        return out
 
     def pending(self):
         q, f = C.c_int(), C.c_int()

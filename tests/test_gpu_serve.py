@@ -1,0 +1,98 @@
+"""Continuous-batching server: every served result meets the tolerance, a
+request's result is the same alone and in a mix (bitwise: slots never
+interact and reductions are fixed-order), graph replays equal direct
+launches, and memory-based admission queues requests that do not fit."""
+import numpy as np
+import pytest
+
+import opflow_amd as of
+import oracle
+
+pytestmark = pytest.mark.gpu
+D = 1
+
+
+def _desc(n):
+    dim = len(n)
+    return of.poisson_desc(dim, n, [1.0 / n[0]] * dim, [(D, D)] * dim)
+
+
+def _serve(ctx, reqs, budget=8 << 30, batched=True, precision="mixed", late=(), max_inflight=16):
+    srv = of.Server(ctx, budget, max_inflight=max_inflight, precision=precision, batched=batched, tol=1e-8)
+    outs, ids = {}, {}
+    for k, (n, seed, nb) in enumerate(reqs):
+        outs[k] = np.zeros(int(np.prod(n)))
+        ids[srv.submit(_desc(n), seed, nb, out=outs[k])] = k
+    results = {}
+    rounds = 0
+    late = list(late)
+    while len(results) < len(reqs) + len(late) and rounds < 10000:
+        srv.step()
+        rounds += 1
+        if rounds == 3:  # requests arriving while others are on the GPU
+            for n, seed, nb in late:
+                k = len(outs)
+                outs[k] = np.zeros(int(np.prod(n)))
+                ids[srv.submit(_desc(n), seed, nb, out=outs[k])] = k
+            reqs = list(reqs) + late
+            late = []
+        for r in srv.poll():
+            results[ids[r.id]] = r
+        q, f = srv.pending()
+        assert f <= max_inflight
+    srv.close()
+    return outs, results
+
+
+MIX = [([64, 64], 1, 3), ([128, 128], 2, 5), ([64, 64], 3, 1), ([32, 32, 32], 4, 8), ([256, 256], 5, 2),
+       ([64, 64, 64], 6, 4), ([128, 128], 7, 6), ([32, 32, 32], 8, 2)]
+
+
+def test_served_results_meet_tolerance(ctx):
+    outs, res = _serve(ctx, MIX)
+    assert len(res) == len(MIX)
+    for k, (n, seed, nb) in enumerate(MIX):
+        r = res[k]
+        assert r.status == 0 and r.state.converged, (n, r.state.relerr)
+        b = of.gen_bumps_grid(_desc(n), seed, nb)
+        res_vec = b - oracle.poisson_apply(n, [1.0 / n[0]] * len(n), [(D, D)] * len(n), outs[k])
+        assert np.linalg.norm(res_vec) <= 1.01e-8 * np.linalg.norm(b)
+        assert r.t_done >= r.t_admit >= r.t_submit - 1e-9
+
+
+def test_alone_equals_in_mix(ctx):
+    target = ([128, 128], 2, 5)
+    alone, _ = _serve(ctx, [target])
+    mixed, _ = _serve(ctx, MIX[:3], late=[target] + MIX[3:])
+    k = 3  # first late request
+    np.testing.assert_array_equal(alone[0], mixed[k])
+    # and within tolerance of a direct (non-served) mixed-precision solve
+    desc = _desc(target[0])
+    b = of.gen_bumps_grid(desc, target[1], target[2])
+    g = of.PoissonGMG(ctx, desc, "mixed")
+    bb, xx = of.DeviceBuffer(ctx, b.size), of.DeviceBuffer(ctx, b.size)
+    bb.upload(b)
+    st = g.solve(bb.ptr, xx.ptr, tol=1e-8)
+    x = xx.numpy()
+    assert np.max(np.abs(x - alone[0])) <= 1e-6 * np.max(np.abs(x))
+    g.close(); bb.close(); xx.close()
+
+
+def test_graph_replays_match_direct_launches(ctx):
+    a, _ = _serve(ctx, MIX, batched=True)
+    b, _ = _serve(ctx, MIX, batched=False)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_memory_admission_queues(ctx):
+    # budget for roughly one 256^2 group: requests wait for memory and still finish
+    desc = _desc([256, 256])
+    one = of.Server(ctx, 1 << 30)  # just to get the lib handle
+    one.close()
+    fp = ctx.lib.opf_gmg_footprint(desc, 1)
+    budget = int(fp * 4 * 1.6 + 256 * 256 * 16 * 4)
+    reqs = [([256, 256], s, 2) for s in range(3)] + [([128, 128], 10 + s, 3) for s in range(3)]
+    outs, res = _serve(ctx, reqs, budget=budget)
+    assert len(res) == len(reqs)
+    assert all(r.state.converged for r in res.values())
