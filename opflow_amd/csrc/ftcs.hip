@@ -98,6 +98,95 @@ __global__ __launch_bounds__(256) void ftcs2d_kernel(const T* __restrict__ in, T
     write_images<T, 2>(out, a, p, v);
 }
 
+
+// 2-D temporal blocking: one workgroup advances a TS x TS output tile by `nt`
+// (<= TB) steps inside LDS, from a region padded by TB cells per side (clamped
+// to the ghost layer on non-periodic sides, wrapped on periodic ones). Every
+// step recomputes all region cells whose stencil lies inside the region; the
+// values at distance < s from an open region edge go stale, which never reaches
+// the tile. Ghost cells inside the region are refilled from the fresh interior
+// with the BC rules, x first then y (the corner composition of updatePadding).
+// Same arithmetic per cell and step as ftcs2d_kernel, so the bits are equal.
+template <int TS, int TB>
+struct Ftcs2dTile {
+    static constexpr int W = TS + 2 * TB;
+};
+
+template <typename T, bool POW2, int TS, int TB>
+__global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
+                                                        int nt) {
+    constexpr int W = Ftcs2dTile<TS, TB>::W;
+    __shared__ T buf[2][W][W];
+    const int tilesx = (a.n[0] + TS - 1) / TS;
+    const int x0 = (blockIdx.x % tilesx) * TS, y0 = (blockIdx.x / tilesx) * TS;
+    // region origin in global cell coordinates; non-periodic sides clamp to the ghost layer
+    int lo[2], hi[2];
+    const int org[2] = {x0, y0};
+    for (int d = 0; d < 2; ++d) {
+        lo[d] = org[d] - TB;
+        hi[d] = org[d] + TS + TB;
+        if (!a.per[d]) {
+            lo[d] = max(lo[d], -1);
+            hi[d] = min(hi[d], a.n[d] + 1);
+        }
+    }
+    const int wx = hi[0] - lo[0], wy = hi[1] - lo[1];
+    auto gidx = [&](int gx, int gy) -> long long {
+        if (a.per[0]) gx = ((gx % a.n[0]) + a.n[0]) % a.n[0];
+        if (a.per[1]) gy = ((gy % a.n[1]) + a.n[1]) % a.n[1];
+        return (gx + 1) + (long long)a.e0 * (gy + 1);
+    };
+    for (int t = threadIdx.x; t < wx * wy; t += blockDim.x) {
+        int lx = t % wx, ly = t / wx;
+        buf[0][ly][lx] = in[gidx(lo[0] + lx, lo[1] + ly)];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int s = 0; s < nt; ++s) {
+        const int nxt = 1 - cur;
+        // interior cells with all four neighbours inside the region
+        for (int t = threadIdx.x; t < wx * wy; t += blockDim.x) {
+            int lx = t % wx, ly = t / wx;
+            int gx = lo[0] + lx, gy = lo[1] + ly;
+            bool inx = a.per[0] || (gx >= 0 && gx < a.n[0]);
+            bool iny = a.per[1] || (gy >= 0 && gy < a.n[1]);
+            if (!inx || !iny || lx == 0 || ly == 0 || lx == wx - 1 || ly == wy - 1) continue;
+            T c = buf[cur][ly][lx];
+            T v = c + a.coef * (d2<T, POW2>(buf[cur][ly][lx - 1], c, buf[cur][ly][lx + 1], a.hh[0]) +
+                                d2<T, POW2>(buf[cur][ly - 1][lx], c, buf[cur][ly + 1][lx], a.hh[1]));
+            buf[nxt][ly][lx] = v;
+        }
+        __syncthreads();
+        // ghosts of non-periodic x sides (rows inside the y interior), then y sides (all columns)
+        if (!a.per[0]) {
+            for (int ly = threadIdx.x; ly < wy; ly += blockDim.x) {
+                int gy = lo[1] + ly;
+                if (!a.per[1] && (gy < 0 || gy >= a.n[1])) continue;
+                if (lo[0] == -1) buf[nxt][ly][0] = rule(a, 0, 0, buf[nxt][ly][1]);
+                if (hi[0] == a.n[0] + 1) buf[nxt][ly][wx - 1] = rule(a, 0, 1, buf[nxt][ly][wx - 2]);
+            }
+            __syncthreads();
+        }
+        if (!a.per[1]) {
+            for (int lx = threadIdx.x; lx < wx; lx += blockDim.x) {
+                if (lo[1] == -1) buf[nxt][0][lx] = rule(a, 1, 0, buf[nxt][1][lx]);
+                if (hi[1] == a.n[1] + 1) buf[nxt][wy - 1][lx] = rule(a, 1, 1, buf[nxt][wy - 2][lx]);
+            }
+            __syncthreads();
+        }
+        cur = nxt;
+    }
+    // write the tile and the ghost images it is the source of
+    for (int t = threadIdx.x; t < TS * TS; t += blockDim.x) {
+        int i = x0 + t % TS, j = y0 + t / TS;
+        if (i >= a.n[0] || j >= a.n[1]) continue;
+        T v = buf[cur][j - lo[1]][i - lo[0]];
+        int p[3] = {i, j, 0};
+        out[ftcs_index<2>(a.e0, a.e1, p)] = v;
+        write_images<T, 2>(out, a, p, v);
+    }
+}
+
 // 3-D: block 64 x 4 owns an x-y tile and walks ZC planes in z, keeping the
 // column's z-1 / z / z+1 values in registers; x/y neighbours come through L1/L2.
 template <typename T, bool POW2, int ZC>
@@ -180,10 +269,17 @@ int ftcs_t(opf_field* f, double coef, int nsteps) {
         f->spare_valid = true;
     }
     constexpr int ZC = 16;
+    constexpr int TS2 = 32, TB2 = 8;
     for (int it = 0; it < nsteps; ++it) {
         const T* in = (const T*)f->data;
         T* out = (T*)f->spare;
-        if (d.dim == 2) {
+        if (d.dim == 2 && nsteps - it >= TB2 && a.n[0] >= TB2 && a.n[1] >= TB2) {
+            // temporally blocked: TB2 steps per launch
+            int tiles = ((a.n[0] + TS2 - 1) / TS2) * ((a.n[1] + TS2 - 1) / TS2);
+            if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, TS2, TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, TB2);
+            else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, TS2, TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, TB2);
+            it += TB2 - 1;
+        } else if (d.dim == 2) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
             if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);
             else hipLaunchKernelGGL((ftcs2d_kernel<T, false>), grid, dim3(64, 4), 0, st, in, out, a);
