@@ -174,8 +174,11 @@ int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_field* const
  * Requires a uniform mesh, centre location, padding = ext = 1 and Dirc/Neum/
  * Periodic BCs; returns OPF_ERR_UNSUPPORTED otherwise (use opf_sweep).      */
 int opf_ftcs(opf_field* u, double coef, int nsteps);
-/* Same update, steps fused `tblock` at a time inside one launch (temporal
- * blocking, 3-D periodic/Dirichlet fields). tblock = 1 equals opf_ftcs.    */
+/* Same update with the temporal-blocking choice made by the caller:
+ * tblock = 1 runs one launch per step; tblock = 0 lets the library fuse steps
+ * inside one launch where it can (2-D: 8 steps per launch, any supported BC;
+ * 3-D: 2 steps per launch when periodic in every dimension), which is what
+ * opf_ftcs does. Results are bit-identical either way.                      */
 int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);
 
 /* ---- reductions ----------------------------------------------------------- */

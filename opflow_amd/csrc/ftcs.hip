@@ -212,6 +212,102 @@ __global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T
     }
 }
 
+// 3-D temporal blocking for fields periodic in every dimension. A workgroup
+// owns a TX x TY column of the x-y plane and the z chunk [z0, z0 + zc). It
+// streams planes of u^n, each with a K-cell wrapped halo in x and y, through
+// LDS and advances them K steps in a z pipeline: once plane zz of level 0 has
+// landed, level L (1..K) computes its plane zz - L from level L-1's planes
+// zz-L-1 .. zz-L+1 (a ring of 3 planes per level) over an x-y window that
+// shrinks by one cell per level; level K is exactly the tile and goes straight
+// to global memory. The next level-0 plane is prefetched into registers while
+// the levels compute. Per cell and step the arithmetic is that of
+// ftcs3d_kernel, so the result has the same bits as K single steps.
+template <int TX, int TY, int K>
+struct Ftcs3dTile {
+    static constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
+};
+
+template <typename T, bool POW2, int TX, int TY, int K, int NT>
+__global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
+                                                       int zc) {
+    static_assert(K >= 2, "level K must not share a barrier interval with the level-0 store");
+    using G = Ftcs3dTile<TX, TY, K>;
+    constexpr int RX = G::RX, RY = G::RY, RP = G::RP;
+    constexpr int NL = (RP + NT - 1) / NT;
+    __shared__ T ring[K][3][RP];
+    const int tilesx = (a.n[0] + TX - 1) / TX, tilesy = (a.n[1] + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int x0 = (b % tilesx) * TX;
+    b /= tilesx;
+    const int y0 = (b % tilesy) * TY;
+    const int z0 = (b / tilesy) * zc;
+    const int z1 = min(z0 + zc, a.n[2]);
+    const long long sz = (long long)a.e0 * a.e1;
+    // wrapped in-plane storage offsets of this thread's level-0 elements
+    int off[NL];
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        int e = threadIdx.x + m * NT;
+        int lx = e % RX, ly = e / RX;
+        int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
+        int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
+        off[m] = e < RP ? (gx + 1) + a.e0 * (gy + 1) : -1;
+    }
+    auto plane = [&](int z) -> const T* {
+        z = (z % a.n[2] + a.n[2]) % a.n[2];
+        return in + sz * (z + 1);
+    };
+    auto slot = [](int z) { return (z + 3 * K) % 3; };  // z >= -K
+    T pre[NL];
+    {
+        const T* p = plane(z0 - K);
+#pragma unroll
+        for (int m = 0; m < NL; ++m)
+            if (off[m] >= 0) pre[m] = p[off[m]];
+    }
+    for (int zz = z0 - K; zz < z1 + K; ++zz) {
+        T* r0 = ring[0][slot(zz)];
+#pragma unroll
+        for (int m = 0; m < NL; ++m)
+            if (off[m] >= 0) r0[threadIdx.x + m * NT] = pre[m];
+        if (zz + 1 < z1 + K) {
+            const T* p = plane(zz + 1);
+#pragma unroll
+            for (int m = 0; m < NL; ++m)
+                if (off[m] >= 0) pre[m] = p[off[m]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int L = 1; L <= K; ++L) {
+            const int zl = zz - L;
+            if (zl < z0 - K + L) break;  // pipeline still filling (uniform over the block)
+            const T* pm = ring[L - 1][slot(zl - 1)];
+            const T* pc = ring[L - 1][slot(zl)];
+            const T* pp = ring[L - 1][slot(zl + 1)];
+            const int w = RX - 2 * L, h = RY - 2 * L;
+            for (int t = threadIdx.x; t < w * h; t += NT) {
+                const int lx = L + t % w, ly = L + t / w;
+                const int e = lx + RX * ly;
+                const T c = pc[e];
+                T s = d2<T, POW2>(pc[e - 1], c, pc[e + 1], a.hh[0]) + d2<T, POW2>(pc[e - RX], c, pc[e + RX], a.hh[1]);
+                s = s + d2<T, POW2>(pm[e], c, pp[e], a.hh[2]);
+                const T v = c + a.coef * s;
+                if (L < K) {
+                    ring[L][slot(zl)][e] = v;
+                } else {
+                    const int i = x0 + lx - K, j = y0 + ly - K;
+                    if (i < a.n[0] && j < a.n[1]) {
+                        int p[3] = {i, j, zl};
+                        out[ftcs_index<3>(a.e0, a.e1, p)] = v;
+                        write_images<T, 3>(out, a, p, v);
+                    }
+                }
+            }
+            if (L < K) __syncthreads();
+        }
+    }
+}
+
 bool is_pow2(double h) {
     if (!(h > 0)) return false;
     int e;
@@ -220,7 +316,7 @@ bool is_pow2(double h) {
 }
 
 template <typename T>
-int ftcs_t(opf_field* f, double coef, int nsteps) {
+int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
     const opf_field_desc& d = f->d;
     const opf_mesh* m = f->mesh;
     FtcsArgs<T> a{};
@@ -270,10 +366,14 @@ int ftcs_t(opf_field* f, double coef, int nsteps) {
     }
     constexpr int ZC = 16;
     constexpr int TS2 = 32, TB2 = 8;
+    constexpr int TX3 = 32, TY3 = 32, K3 = 2, ZC3 = 64;
+    const bool tb3 = tblock != 1 && d.dim == 3 && a.per[0] && a.per[1] && a.per[2] &&
+                     (long long)a.e0 * a.e1 < (1LL << 31);
+    const bool tb2 = tblock != 1;
     for (int it = 0; it < nsteps; ++it) {
         const T* in = (const T*)f->data;
         T* out = (T*)f->spare;
-        if (d.dim == 2 && nsteps - it >= TB2 && a.n[0] >= TB2 && a.n[1] >= TB2) {
+        if (d.dim == 2 && tb2 && nsteps - it >= TB2 && a.n[0] >= TB2 && a.n[1] >= TB2) {
             // temporally blocked: TB2 steps per launch
             int tiles = ((a.n[0] + TS2 - 1) / TS2) * ((a.n[1] + TS2 - 1) / TS2);
             if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, TS2, TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, TB2);
@@ -283,6 +383,12 @@ int ftcs_t(opf_field* f, double coef, int nsteps) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
             if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);
             else hipLaunchKernelGGL((ftcs2d_kernel<T, false>), grid, dim3(64, 4), 0, st, in, out, a);
+        } else if (d.dim == 3 && tb3 && nsteps - it >= K3) {
+            // temporally blocked: K3 steps per launch
+            int nb = ((a.n[0] + TX3 - 1) / TX3) * ((a.n[1] + TY3 - 1) / TY3) * ((a.n[2] + ZC3 - 1) / ZC3);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, TX3, TY3, K3, 256>), dim3(nb), dim3(256), 0, st, in, out, a, ZC3);
+            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, TX3, TY3, K3, 256>), dim3(nb), dim3(256), 0, st, in, out, a, ZC3);
+            it += K3 - 1;
         } else if (d.dim == 3) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + ZC - 1) / ZC);
             if (pow2) hipLaunchKernelGGL((ftcs3d_kernel<T, true, ZC>), grid, dim3(64, 4), 0, st, in, out, a);
@@ -302,10 +408,11 @@ int ftcs_t(opf_field* f, double coef, int nsteps) {
 extern "C" int opf_ftcs(opf_field* u, double coef, int nsteps) {
     OPF_CHECK(u && nsteps >= 0, OPF_ERR_ARG, "opf_ftcs: bad args");
     if (nsteps == 0) return OPF_OK;
-    return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps) : ftcs_t<float>(u, coef, nsteps);
+    return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps, 0) : ftcs_t<float>(u, coef, nsteps, 0);
 }
 
 extern "C" int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock) {
-    OPF_CHECK(tblock >= 1, OPF_ERR_ARG, "tblock >= 1");
-    return opf_ftcs(u, coef, nsteps);
+    OPF_CHECK(u && nsteps >= 0 && tblock >= 0, OPF_ERR_ARG, "opf_ftcs_blocked: bad args");
+    if (nsteps == 0) return OPF_OK;
+    return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps, tblock) : ftcs_t<float>(u, coef, nsteps, tblock);
 }

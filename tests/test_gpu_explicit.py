@@ -94,6 +94,71 @@ def test_ftcs3d_periodic(ctx, dtype):
     u.close()
 
 
+def test_ftcs3d_periodic_ragged_blocked(ctx):
+    """Temporal blocking on a grid that no tile divides, odd step count (blocked
+    launches plus a trailing single step): equal bits to the oracle and to one
+    launch per step."""
+    n = [40, 36, 20]
+    steps = 13
+    mesh, om = _mesh(ctx, [m + 1 for m in n], [(0.0, m / 32.0) for m in n])
+    fields = []
+    for tblock in (0, 1):
+        u = build_field(ctx, mesh, 3, 1, [(PER, PER)] * 3)
+        init = oracle.update_padding(u.desc, om, random_storage(u, SEED + 9))
+        u.upload(init)
+        u.ftcs_blocked(0.2 / 32**2, steps, tblock)
+        fields.append(u.numpy())
+        u.close()
+    ref = oracle.ftcs(u.desc, om, init, 0.2 / 32**2, steps)
+    np.testing.assert_array_equal(fields[0], ref)
+    np.testing.assert_array_equal(fields[1], ref)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_ftcs3d_periodic_full_size_blocked(ctx, dtype):
+    """512^3 periodic (the benchmark shape): blocked and one-launch-per-step
+    results are identical, and a Fourier mode decays by lambda^N."""
+    n, steps = 512, 6
+    mesh, om = _mesh(ctx, [n + 1] * 3)
+    h = 1.0 / n
+    coef = 0.2 * h * h
+    xc = (np.arange(n) + 0.5) / n
+    modes = (1, 2, 3)
+    init = None
+    got = []
+    for tblock in (0, 1):
+        u = build_field(ctx, mesh, 3, 1, [(PER, PER)] * 3, dtype=dtype)
+        if init is None:
+            init = np.zeros(u.shape)
+            init[1:-1, 1:-1, 1:-1] = (np.cos(2 * np.pi * modes[2] * xc)[:, None, None] *
+                                      np.cos(2 * np.pi * modes[1] * xc)[None, :, None] *
+                                      np.cos(2 * np.pi * modes[0] * xc)[None, None, :])
+            init = oracle.update_padding(u.desc, om, init)
+        u.upload(init)
+        u.ftcs_blocked(coef, steps, tblock)
+        got.append(u.numpy())
+        u.close()
+    np.testing.assert_array_equal(got[0], got[1])
+    lam = 1 - 4 * coef / h**2 * sum(np.sin(np.pi * m * h) ** 2 for m in modes)
+    tol = 1e-12 if dtype == "f64" else 2e-6
+    assert rel_err(got[0], lam**steps * init) < tol
+
+
+def test_ftcs2d_blocked_equals_single_large(ctx):
+    """2048^2 Dirichlet/Neumann, 19 steps: 8-step blocked launches + 3 single steps == 19 single steps."""
+    n, steps = 2048, 19
+    mesh, om = _mesh(ctx, [n + 1, n + 1])
+    got = []
+    for tblock in (0, 1):
+        u = build_field(ctx, mesh, 2, 1, [(DIRC, NEUM), (NEUM, DIRC)], [(0.5, -1.0), (0.25, 0.0)])
+        init = oracle.update_padding(u.desc, om, random_storage(u, SEED + 12))
+        u.upload(init)
+        u.ftcs_blocked(0.2 / n**2, steps, tblock)
+        got.append(u.numpy())
+        u.close()
+    np.testing.assert_array_equal(got[0], got[1])
+
+
 def test_ftcs3d_dirichlet_eigenmode_full_size(ctx):
     """Size-independent property at 256^3 (Dirichlet): the discrete eigenmode decays by lambda^N."""
     n, steps = 256, 10

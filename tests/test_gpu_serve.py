@@ -86,13 +86,20 @@ def test_graph_replays_match_direct_launches(ctx):
 
 
 def test_memory_admission_queues(ctx):
-    # budget for roughly one 256^2 group: requests wait for memory and still finish
+    """Budget just above one 256^2 group (8 slots at N <= 2^18): the 128^2
+    requests cannot be co-resident with it, wait for it to finish and be
+    evicted, and still finish to tolerance."""
     desc = _desc([256, 256])
-    one = of.Server(ctx, 1 << 30)  # just to get the lib handle
-    one.close()
-    fp = ctx.lib.opf_gmg_footprint(desc, 1)
-    budget = int(fp * 4 * 1.6 + 256 * 256 * 16 * 4)
+    N = 256 * 256
+    group = ctx.lib.opf_gmg_footprint(desc, 1) * 8 + 8 * N * 16 + 4096
+    budget = int(group * 1.1)
     reqs = [([256, 256], s, 2) for s in range(3)] + [([128, 128], 10 + s, 3) for s in range(3)]
     outs, res = _serve(ctx, reqs, budget=budget)
     assert len(res) == len(reqs)
     assert all(r.state.converged for r in res.values())
+    first_done = min(res[k].t_done for k in range(3))
+    assert all(res[k].t_admit >= first_done for k in range(3, 6))
+    for k, (n, seed, nb) in enumerate(reqs):
+        b = of.gen_bumps_grid(_desc(n), seed, nb)
+        r = b - oracle.poisson_apply(n, [1.0 / n[0]] * 2, [(D, D)] * 2, outs[k])
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
