@@ -1,6 +1,6 @@
 // Ghost-position enumeration and source lookup of the closed-form
 // updatePadding kernel. Host- and device-callable so the index math can be
-// checked exhaustively on the CPU (tests/cpp/ghost_index_check.cpp).
+// checked exhaustively on the CPU (tests/cpp_host/index_check.cpp).
 #pragma once
 
 #if defined(__HIPCC__)

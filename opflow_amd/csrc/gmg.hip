@@ -32,42 +32,36 @@ __device__ __forceinline__ T face_coef(const GOp<T>& o, bool var, int d, int i, 
     return var ? o.af[d][face_index(d, i, j, k, o.n0, o.n1)] : o.a[d];
 }
 
-// (A x)_c and the diagonal D_c; xv(ii,jj,kk) returns x at a cell.
+// (A x)_c and the diagonal D_c at cell (i, j, k) = linear index t;
+// xv(l) returns x at linear cell index l.
 template <typename T, bool VAR, int DIM, typename XV>
-__device__ __forceinline__ T apply_op(const GOp<T>& o, int i, int j, int k, T xc, XV xv, T& D) {
+__device__ __forceinline__ T apply_op(const GOp<T>& o, int i, int j, int k, uint32_t t, T xc, XV xv, T& D) {
     T acc = 0;
-    D = VAR ? o.sg[i + (long long)o.n0 * (j + (long long)o.n1 * k)] : o.sigma;
-    int c[3] = {i, j, k};
-    int n[3] = {o.n0, o.n1, o.n2};
+    D = VAR ? o.sg[t] : o.sigma;
+    const int c[3] = {i, j, k};
+    const int n[3] = {o.n0, o.n1, o.n2};
+    const uint32_t st[3] = {1u, o.s1, o.s2};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+        int hi[3] = {i, j, k};
         hi[d] += 1;
-        T alo = face_coef<T>(o, VAR, d, lo[0], lo[1], lo[2]);
+        T alo = face_coef<T>(o, VAR, d, i, j, k);
         T ahi = face_coef<T>(o, VAR, d, hi[0], hi[1], hi[2]);
         if (c[d] > 0) {
-            int q[3] = {i, j, k};
-            q[d] -= 1;
             D += alo;
-            acc += alo * xv(q[0], q[1], q[2]);
+            acc += alo * xv(t - st[d]);
         } else if (o.per[d]) {
-            int q[3] = {i, j, k};
-            q[d] = n[d] - 1;
             D += alo;
-            acc += alo * xv(q[0], q[1], q[2]);
+            acc += alo * xv(t + (uint32_t)(n[d] - 1) * st[d]);
         } else {
             D += alo * o.bfac[d][0];
         }
         if (c[d] < n[d] - 1) {
-            int q[3] = {i, j, k};
-            q[d] += 1;
             D += ahi;
-            acc += ahi * xv(q[0], q[1], q[2]);
+            acc += ahi * xv(t + st[d]);
         } else if (o.per[d]) {
-            int q[3] = {i, j, k};
-            q[d] = 0;
             D += ahi;
-            acc += ahi * xv(q[0], q[1], q[2]);
+            acc += ahi * xv(t - (uint32_t)(n[d] - 1) * st[d]);
         } else {
             D += ahi * o.bfac[d][1];
         }
@@ -75,11 +69,13 @@ __device__ __forceinline__ T apply_op(const GOp<T>& o, int i, int j, int k, T xc
     return D * xc - acc;
 }
 
-__host__ __device__ __forceinline__ void cell_of(long long t, int n0, int n1, int& i, int& j, int& k) {
-    i = (int)(t % n0);
-    long long r = t / n0;
-    j = (int)(r % n1);
-    k = (int)(r / n1);
+// linear cell index -> (i, j, k) for extents n0, n1 (f0, f1 divide by them)
+__device__ __forceinline__ void cell_of(uint32_t t, const FastDiv& f0, const FastDiv& f1, int& i, int& j, int& k) {
+    const uint32_t r = f0.div(t);
+    i = (int)(t - r * f0.d);
+    const uint32_t q = f1.div(r);
+    j = (int)(r - q * f1.d);
+    k = (int)q;
 }
 
 // Batched execution: blockIdx.y selects a request slot; slot vectors are
@@ -122,20 +118,20 @@ template <typename T, typename TO, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
                                                TO* __restrict__ r, double* __restrict__ part, Bat bt) {
     BAT_SLOT(bt);
-    long long N = (long long)o.n0 * o.n1 * o.n2;
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
     if (x) x += vo;
     b += vo;
     if (r) r += vo;
     if (part) part += (long long)slot * bt.ps;
     double acc = 0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_of(t, o.n0, o.n1, i, j, k);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
         T v = b[t];
         if (x) {
-            auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+            int i, j, k;
+            cell_of(t, o.f0, o.f1, i, j, k);
+            auto xv = [&](uint32_t l) { return x[l]; };
             T D;
-            v = v - apply_op<T, VAR, DIM>(o, i, j, k, x[t], xv, D);
+            v = v - apply_op<T, VAR, DIM>(o, i, j, k, t, x[t], xv, D);
         }
         if (r) r[t] = (TO)v;
         acc += (double)v * (double)v;
@@ -149,14 +145,14 @@ __global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x
 // y = A x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_apply(GOp<T> o, const T* __restrict__ x, T* __restrict__ y) {
-    long long N = (long long)o.n0 * o.n1 * o.n2;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     int i, j, k;
-    cell_of(t, o.n0, o.n1, i, j, k);
-    auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+    cell_of(t, o.f0, o.f1, i, j, k);
+    auto xv = [&](uint32_t l) { return x[l]; };
     T D;
-    y[t] = apply_op<T, VAR, DIM>(o, i, j, k, x[t], xv, D);
+    y[t] = apply_op<T, VAR, DIM>(o, i, j, k, t, x[t], xv, D);
 }
 
 // weighted Jacobi: xo = xi + w (b - A xi)/D ; ZERO: xo = w b / D
@@ -164,23 +160,23 @@ template <typename T, bool VAR, int DIM, bool ZERO>
 __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
                                                 T* __restrict__ xo, T w, Bat bt) {
     BAT_SLOT(bt);
-    long long N = (long long)o.n0 * o.n1 * o.n2;
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
     if (xi) xi += vo;
     b += vo;
     xo += vo;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     int i, j, k;
-    cell_of(t, o.n0, o.n1, i, j, k);
+    cell_of(t, o.f0, o.f1, i, j, k);
     T D;
     if (ZERO) {
-        auto zv = [&](int, int, int) { return (T)0; };
-        apply_op<T, VAR, DIM>(o, i, j, k, (T)0, zv, D);
+        auto zv = [&](uint32_t) { return (T)0; };
+        apply_op<T, VAR, DIM>(o, i, j, k, t, (T)0, zv, D);
         xo[t] = w * b[t] / D;
     } else {
-        auto xv = [&](int a, int bb, int c) { return xi[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+        auto xv = [&](uint32_t l) { return xi[l]; };
         T xc = xi[t];
-        T ax = apply_op<T, VAR, DIM>(o, i, j, k, xc, xv, D);
+        T ax = apply_op<T, VAR, DIM>(o, i, j, k, t, xc, xv, D);
         xo[t] = xc + w * (b[t] - ax) / D;
     }
 }
@@ -188,43 +184,45 @@ __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ 
 // coarse rhs = average over the 2^d children of the fine residual b - A x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
-                                                  T* __restrict__ bc, int c0, int c1, int c2, Bat bt) {
+                                                  T* __restrict__ bc, int c0, int c1, int c2, FastDiv fc0,
+                                                  FastDiv fc1, Bat bt) {
     BAT_SLOT(bt);
-    long long NC = (long long)c0 * c1 * c2;
+    const uint32_t NC = (uint32_t)c0 * c1 * c2;
     x += vo;
     b += vo;
     bc += (long long)slot * NC;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NC) return;
     int I, J, K;
-    cell_of(t, c0, c1, I, J, K);
-    auto xv = [&](int a, int bb, int c) { return x[a + (long long)o.n0 * (bb + (long long)o.n1 * c)]; };
+    cell_of(t, fc0, fc1, I, J, K);
+    auto xv = [&](uint32_t l) { return x[l]; };
     T s = 0;
     const int nz = DIM == 3 ? 2 : 1;
     for (int dk = 0; dk < nz; ++dk)
         for (int dj = 0; dj < 2; ++dj)
             for (int di = 0; di < 2; ++di) {
-                int i = 2 * I + di, j = 2 * J + dj, k = DIM == 3 ? 2 * K + dk : 0;
-                long long f = i + (long long)o.n0 * (j + (long long)o.n1 * k);
+                const int i = 2 * I + di, j = 2 * J + dj, k = DIM == 3 ? 2 * K + dk : 0;
+                const uint32_t f = (uint32_t)i + o.s1 * j + o.s2 * k;
                 T D;
-                s += b[f] - apply_op<T, VAR, DIM>(o, i, j, k, x[f], xv, D);
+                s += b[f] - apply_op<T, VAR, DIM>(o, i, j, k, f, x[f], xv, D);
             }
     bc[t] = s * (T)(DIM == 3 ? 0.125 : 0.25);
 }
 
-// x_f += P x_c (linear interpolation, coarse ghosts: periodic wrap or alpha * boundary value)
+// x_f += P x_c (linear interpolation, coarse ghosts: periodic wrap or alpha * boundary value);
+// o is the fine level's operator
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, int n0, int n1, int n2, const T* __restrict__ xc,
-                                                 int c0, int c1, int c2, GOp<T> o, Bat bt) {
+__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, const T* __restrict__ xc, int c0, int c1, int c2,
+                                                 GOp<T> o, Bat bt) {
     BAT_SLOT(bt);
-    long long N = (long long)n0 * n1 * n2;
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
     xf += vo;
     xc += (long long)slot * ((long long)c0 * c1 * c2);
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     int i, j, k;
-    cell_of(t, n0, n1, i, j, k);
-    int f[3] = {i, j, k}, cn[3] = {c0, c1, c2};
+    cell_of(t, o.f0, o.f1, i, j, k);
+    const int f[3] = {i, j, k}, cn[3] = {c0, c1, c2};
     int par[3], oth[3];
     for (int d = 0; d < 3; ++d) {
         par[d] = f[d] >> 1;
@@ -250,7 +248,7 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, int n0, int
             }
             q[d] = c;
         }
-        s += w * fac * xc[q[0] + (long long)c0 * (q[1] + (long long)c1 * q[2])];
+        s += w * fac * xc[(uint32_t)q[0] + (uint32_t)c0 * ((uint32_t)q[1] + (uint32_t)c1 * q[2])];
     }
     xf[t] += s;
 }
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ 
                                                 T* __restrict__ pnew, T* __restrict__ q,
                                                 const double* __restrict__ s, double* __restrict__ part, Bat bt) {
     BAT_SLOT(bt);
-    long long N = (long long)o.n0 * o.n1 * o.n2;
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
     z += vo;
     pold += vo;
     pnew += vo;
@@ -326,16 +324,13 @@ __global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ 
     const T beta = (T)s[S_BETA];
     const T zm = (T)s[S_ZMEAN];
     double acc = 0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
         int i, j, k;
-        cell_of(t, o.n0, o.n1, i, j, k);
-        auto pv = [&](int a, int bb, int c) {
-            long long l = a + (long long)o.n0 * (bb + (long long)o.n1 * c);
-            return (z[l] - zm) + beta * pold[l];
-        };
-        T pc = pv(i, j, k);
+        cell_of(t, o.f0, o.f1, i, j, k);
+        auto pv = [&](uint32_t l) { return (z[l] - zm) + beta * pold[l]; };
+        T pc = pv(t);
         T D;
-        T qc = apply_op<T, VAR, DIM>(o, i, j, k, pc, pv, D);
+        T qc = apply_op<T, VAR, DIM>(o, i, j, k, t, pc, pv, D);
         pnew[t] = pc;
         q[t] = qc;
         acc += (double)pc * (double)qc;
@@ -447,6 +442,10 @@ static GOp<T> make_op(const Hierarchy& H, int l) {
     o.n0 = L.n[0];
     o.n1 = L.n[1];
     o.n2 = L.n[2];
+    o.s1 = (uint32_t)L.n[0];
+    o.s2 = (uint32_t)L.n[0] * (uint32_t)L.n[1];
+    o.f0 = FastDiv((uint32_t)L.n[0]);
+    o.f1 = FastDiv((uint32_t)L.n[1]);
     o.dim = H.dim;
     for (int d = 0; d < 3; ++d) {
         o.per[d] = H.per[d];
@@ -752,15 +751,16 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         std::swap(cur, tmp);
     }
     const GLevelDev& C = H.dev[l + 1];
+    const FastDiv fc0((uint32_t)C.n[0]), fc1((uint32_t)C.n[1]);
     GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N), B), dim3(256), 0, st, o,
-                                                  (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], bt));
+                                                  (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], fc0, fc1, bt));
     ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
     if (H.dim == 3)
-        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
-                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o, bt);
+        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
+                           C.n[2], o, bt);
     else
-        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, cur, L.n[0], L.n[1], L.n[2],
-                           (const T*)C.x, C.n[0], C.n[1], C.n[2], o, bt);
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
+                           C.n[2], o, bt);
     for (int s = 0; s < H.post; ++s) {
         GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256), 0,
                                                       st, o, (const T*)cur, rhs, tmp, w, bt));

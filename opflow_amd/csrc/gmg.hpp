@@ -12,6 +12,7 @@
 
 #include <vector>
 
+#include "fastdiv.hpp"
 #include "opf_internal.hpp"
 
 namespace opf {
@@ -24,6 +25,8 @@ constexpr int GMG_PSTRIDE = 4 * GMG_MAXB;  // reduction partials per slot
 template <typename T>
 struct GOp {
     int n0, n1, n2;
+    uint32_t s1, s2;  // linear strides of dims 1 and 2 (n0, n0 * n1); cells per slot < 2^31
+    FastDiv f0, f1;   // division by n0 and n1 (cell index -> coordinates)
     int dim;
     int per[3];
     T a[3];          // constant face coefficient per dim (const mode)
