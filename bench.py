@@ -85,6 +85,8 @@ def ftcs_field(of, ctx, n, dim, dtype, bc_type, seed, init="uniform"):
 
 
 def time_ftcs(of, ctx, ev, u, coef, steps, warmup):
+    """HIP-event time of `steps` FTCS steps on the library's stream (every
+    launch in the region is an FTCS kernel launch)."""
     u.ftcs(coef, warmup)
     ctx.sync()
     e0, e1 = ev.create(), ev.create()
@@ -126,6 +128,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only run 5 device copies of one 512^3 FP64 field (2 x nbytes of HBM traffic each): "
+                         "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: the engine is single-GPU by design "
@@ -136,6 +141,15 @@ def main():
     ctx = of.Context(0)
     ev = HipEvents()
     n = args.n
+    if args.probe_copy:
+        _, a = ftcs_field(of, ctx, n, 3, "f64", of.BCType.Periodic, 1)
+        _, b = ftcs_field(of, ctx, n, 3, "f64", of.BCType.Periodic, 2)
+        for _ in range(5):
+            b.copy_from(a)
+        ctx.sync()
+        nbytes = 8 * a.shape[0] * a.shape[1] * a.shape[2]
+        print(json.dumps({"probe": "copy", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes}))
+        return
     h = 1.0 / n
     coef = 0.2 * h * h  # dt = 0.2 h^2, alpha = 1
 
@@ -144,9 +158,13 @@ def main():
     ms, wall = time_ftcs(of, ctx, ev, u, coef, args.steps, args.warmup)
     cells = n ** 3
     value = cells * args.steps / (ms / 1e3) / 1e6
-    per_launch_s = ms / 1e3 / args.steps
-    alg_bytes = 16.0 * cells  # read u^n + write u^{n+1}, FP64
+    launches, steps_per_launch = u.ftcs_plan(args.steps)
+    per_launch_s = ms / 1e3 / launches
+    # algorithmic bytes of one launch: read u once and write it once (FP64),
+    # whatever number of steps the launch fuses
+    alg_bytes = 16.0 * cells
     achieved = alg_bytes / per_launch_s / 1e9
+    u64 = None if args.no_extra else u.numpy()[1:-1, 1:-1, 1:-1]
     u.close()
 
     extra = {}
@@ -154,8 +172,21 @@ def main():
         mesh32, u32 = ftcs_field(of, ctx, n, 3, "f32", of.BCType.Periodic, 0xC0FFEE + 2)
         ms32, _ = time_ftcs(of, ctx, ev, u32, coef, args.steps, args.warmup)
         v32 = cells * args.steps / (ms32 / 1e3) / 1e6
+        l32, k32 = u32.ftcs_plan(args.steps)
         extra["ftcs3d_512_fp32"] = {"Mcell_updates_per_s": v32, "ms_per_step": ms32 / args.steps,
-                                    "hbm_frac": 8.0 * cells / (ms32 / 1e3 / args.steps) / 1e9 / HBM_PEAK_GBS}
+                                    "steps_per_launch": k32,
+                                    "alg_GBps_per_launch": 8.0 * cells / (ms32 / 1e3 / l32) / 1e9}
+        extra["ftcs3d_512_fp32"]["hbm_frac"] = extra["ftcs3d_512_fp32"]["alg_GBps_per_launch"] / HBM_PEAK_GBS
+        # FP32 against FP64 after the same warmup + steps from the same input
+        # (the FP64 GPU sweep equals the oracle bit for bit), relative to max|u_FP64|
+        import numpy as np
+        d32 = u32.numpy()[1:-1, 1:-1, 1:-1].astype(np.float64) - u64
+        scale = float(np.max(np.abs(u64)))
+        extra["ftcs3d_512_fp32"].update({
+            "steps_compared": args.warmup + args.steps, "max_abs_u_fp64": scale,
+            "max_err_rel": float(np.max(np.abs(d32))) / scale,
+            "l2_err_rel": float(np.linalg.norm(d32) / np.linalg.norm(u64))})
+        del d32
         u32.close()
         n2 = 256
         h2 = 1.0 / n2
@@ -192,7 +223,11 @@ def main():
         "config": {"workload": "ftcs3d 512^3 cells fp64 periodic dt=0.2h^2 (one step = one fused sweep)",
                    "cells": cells, "grid": [n, n, n], "bc": "periodic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "step_equivalent_GBps": alg_bytes * args.steps / (ms / 1e3) / 1e9,
+                     "step_equivalent_frac": alg_bytes * args.steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "kernel": "ftcs3d_tb_kernel" if steps_per_launch > 1 else "ftcs3d_kernel",
+                     "steps_per_launch": steps_per_launch, "launches": launches},
         "cpu_baseline": cpu,
         "wall_s_timed_region": wall,
         "extra": extra,

@@ -180,6 +180,11 @@ int opf_ftcs(opf_field* u, double coef, int nsteps);
  * 3-D: 2 steps per launch when periodic in every dimension), which is what
  * opf_ftcs does. Results are bit-identical either way.                      */
 int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);
+/* Launch plan of opf_ftcs_blocked(u, ., nsteps, tblock) without running it:
+ * number of kernel launches and the most steps one launch advances. No
+ * counterpart in the reference (its FTCS loop is one rangeFor pass per step);
+ * it lets a caller relate launch timings to steps.                          */
+int opf_ftcs_plan(const opf_field* u, int nsteps, int tblock, int* launches, int* max_steps);
 
 /* ---- reductions ----------------------------------------------------------- */
 /* Replaces rangeReduce (Loops/RangeFor.hpp:86-121) for the common reducers;
@@ -191,6 +196,8 @@ int opf_buf_create(opf_ctx* ctx, size_t nbytes, void** dptr);
 int opf_buf_destroy(opf_ctx* ctx, void* dptr);
 int opf_buf_upload(opf_ctx* ctx, void* dptr, const void* host, size_t nbytes);
 int opf_buf_download(opf_ctx* ctx, const void* dptr, void* host, size_t nbytes);
+/* zero nbytes of a device buffer, asynchronously on the context's stream */
+int opf_buf_zero(opf_ctx* ctx, void* dptr, size_t nbytes);
 
 /* ---- synthetic inputs (host-only) ----------------------------------------- */
 /* SplitMix64 stream, u = (x >> 11) * 2^-53; lo + (hi-lo) u. */
@@ -291,7 +298,7 @@ typedef struct {
     uint64_t seed;       /* RHS = nbumps seeded Gaussian bumps                      */
     int nbumps;
     double noise;
-    const double* rhs;   /* optional host RHS (prod(n) doubles); NULL -> generated  */
+    const double* rhs;   /* optional host RHS (prod(n) doubles); NULL -> generated on the device from seed/nbumps/noise */
     double* out_x;       /* optional host buffer: the solution is copied here when done */
     double t_submit;     /* arrival time on the server clock (< 0: now)             */
 } opf_srv_request;
@@ -304,6 +311,12 @@ typedef struct {
 } opf_srv_result;
 /* bumps (+ noise) on the cell centres (h*i + h/2) of the unit-cube grid of d, x-fastest */
 int opf_gen_bumps_grid(const opf_poisson_desc* d, uint64_t seed, int nbumps, double noise, double* out);
+/* The same right-hand side generated on the device into dev_out (prod(n)
+ * doubles) on the context's stream; at most 32 bumps. Values equal the host
+ * generator's up to the last bit of exp/log/cos. Synthetic-input helper with
+ * no counterpart in the reference.                                          */
+int opf_gen_bumps_grid_dev(opf_ctx* ctx, const opf_poisson_desc* d, uint64_t seed, int nbumps, double noise,
+                           double* dev_out);
 int opf_srv_create(opf_ctx* ctx, const opf_srv_config* cfg, opf_srv** out);
 int opf_srv_submit(opf_srv* s, const opf_srv_request* req, uint64_t* id);
 /* advance the device work by one scheduling round; returns number of finished */

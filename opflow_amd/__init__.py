@@ -396,6 +396,13 @@ class CartesianField(Expr):
     def ftcs_blocked(self, coef, nsteps, tblock):
         L.check(self.lib.opf_ftcs_blocked(self.h, float(coef), int(nsteps), int(tblock)))
 
+    def ftcs_plan(self, nsteps, tblock=0):
+        """(kernel launches, most steps per launch) that ftcs/ftcs_blocked use for nsteps."""
+        import ctypes as C
+        nl, mx = C.c_int(), C.c_int()
+        L.check(self.lib.opf_ftcs_plan(self.h, int(nsteps), int(tblock), C.byref(nl), C.byref(mx)))
+        return nl.value, mx.value
+
     def device_ptr(self):
         return self.lib.opf_field_device_ptr(self.h)
 
@@ -502,4 +509,5 @@ def gen_uniform(seed, n, lo=0.0, hi=1.0):
 
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
-                    PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc, gen_bumps_grid)
+                    PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc, gen_bumps_grid,
+                    gen_bumps_grid_dev)

@@ -116,11 +116,13 @@ SIGNATURES = {
     "opf_sweep": (_I, [C.POINTER(ExprNode), _I, _I, C.POINTER(_P), _I, _P, _I]),
     "opf_ftcs": (_I, [_P, _D, _I]),
     "opf_ftcs_blocked": (_I, [_P, _D, _I, _I]),
+    "opf_ftcs_plan": (_I, [_P, _I, _I, _IP, _IP]),
     "opf_reduce": (_I, [_P, _I, _IP, _IP, _DP]),
     "opf_buf_create": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "opf_buf_destroy": (_I, [_P, _P]),
     "opf_buf_upload": (_I, [_P, _P, _P, C.c_size_t]),
     "opf_buf_download": (_I, [_P, _P, _P, C.c_size_t]),
+    "opf_buf_zero": (_I, [_P, _P, C.c_size_t]),
     "opf_gen_uniform": (_I, [C.c_uint64, C.c_size_t, _D, _D, _DP]),
     "opf_gen_bumps": (_I, [_P, C.POINTER(FieldDesc), C.c_uint64, _I, _D, _DP]),
     "opf_eqn_create": (_I, [C.POINTER(ExprNode), _I, _I, _I, C.POINTER(_P), _I, _P, C.POINTER(SolveParams),
@@ -137,6 +139,7 @@ SIGNATURES = {
     "opf_srv_step": (_I, [_P, _IP]),
     "opf_srv_poll": (_I, [_P, C.POINTER(SrvResult), _I, _IP]),
     "opf_gen_bumps_grid": (_I, [C.POINTER(PoissonDesc), C.c_uint64, _I, _D, _DP]),
+    "opf_gen_bumps_grid_dev": (_I, [_P, C.POINTER(PoissonDesc), C.c_uint64, _I, _D, _P]),
     "opf_srv_pending": (_I, [_P, _IP, _IP]),
     "opf_srv_clock": (_D, [_P]),
     "opf_srv_destroy": (_I, [_P]),

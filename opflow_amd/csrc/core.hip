@@ -123,6 +123,12 @@ extern "C" int opf_buf_download(opf_ctx* c, const void* dptr, void* host, size_t
     return OPF_OK;
 }
 
+extern "C" int opf_buf_zero(opf_ctx* c, void* dptr, size_t nbytes) {
+    OPF_CHECK(c && dptr, OPF_ERR_ARG, "opf_buf_zero: null arg");
+    OPF_HIP(hipMemsetAsync(dptr, 0, nbytes, c->stream));
+    return OPF_OK;
+}
+
 // ---------------------------------------------------------------- mesh
 extern "C" int opf_mesh_uniform_arrays(int nnodes, double lo, double hi, int pad, double* x, double* dx) {
     // MeshBuilder::set1DMesh(min, max, k) + setExtMesh(Symm) with range start 0
@@ -755,7 +761,85 @@ void gen_bumps_grid(int dim, const int n[3], uint64_t seed, int nbumps, double n
                 out[l] = v;
             }
 }
+
+// Device version of gen_bumps_grid: the same bump parameters (drawn on the
+// host from the same SplitMix64 stream) and the same per-cell arithmetic; the
+// noise of cell l uses draws P + 2l + 1 and P + 2l + 2 of the stream (P = the
+// parameter draws), which is where the sequential host generator takes them.
+// exp/log/cos are the device's, so values may differ from the host's in the
+// last bit.
+namespace {
+constexpr int DEV_MAX_BUMPS = 32;
+struct DevBumps {
+    int dim, nb;
+    int n[3];
+    double h[3];
+    double c[DEV_MAX_BUMPS][3], w[DEV_MAX_BUMPS], A[DEV_MAX_BUMPS];
+    double noise;
+    uint64_t seed, p;
+};
+
+__device__ __forceinline__ double smx_uniform(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+__global__ void gen_bumps_kernel(DevBumps a, double* __restrict__ out, long long N) {
+    for (long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x; l < N;
+         l += (long long)gridDim.x * blockDim.x) {
+        const int idx[3] = {(int)(l % a.n[0]), (int)((l / a.n[0]) % a.n[1]), (int)(l / ((long long)a.n[0] * a.n[1]))};
+        double xc[3];
+        for (int q = 0; q < 3; ++q) xc[q] = a.h[q] * idx[q] + .5 * a.h[q];
+        double v = 0.0;
+        for (int b = 0; b < a.nb; ++b) {
+            double r2 = 0.0;
+            for (int q = 0; q < a.dim; ++q) r2 += (xc[q] - a.c[b][q]) * (xc[q] - a.c[b][q]);
+            v += a.A[b] * exp(-r2 / (2.0 * a.w[b] * a.w[b]));
+        }
+        if (a.noise != 0.0) {
+            double u1 = smx_uniform(a.seed, a.p + 2 * (uint64_t)l + 1), u2 = smx_uniform(a.seed, a.p + 2 * (uint64_t)l + 2);
+            if (u1 <= 0.0) u1 = 0x1.0p-53;
+            v += a.noise * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        }
+        out[l] = v;
+    }
+}
+}  // namespace
+
+int gen_bumps_grid_dev(hipStream_t st, int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out) {
+    OPF_CHECK(nbumps >= 0 && nbumps <= DEV_MAX_BUMPS, OPF_ERR_ARG, "gen_bumps_grid_dev: 0..%d bumps", DEV_MAX_BUMPS);
+    DevBumps a{};
+    SplitMix64 g{seed};
+    a.dim = dim;
+    a.nb = nbumps;
+    for (int b = 0; b < nbumps; ++b) {
+        for (int k = 0; k < 3; ++k) a.c[b][k] = k < dim ? 0.2 + 0.6 * g.uniform() : 0.0;
+        a.w[b] = 0.03 + 0.07 * g.uniform();
+        a.A[b] = 0.5 + 0.5 * g.uniform();
+    }
+    for (int q = 0; q < 3; ++q) {
+        a.n[q] = q < dim ? n[q] : 1;
+        a.h[q] = 1.0 / a.n[q];
+    }
+    a.noise = noise;
+    a.seed = seed;
+    a.p = (uint64_t)nbumps * (dim + 2);
+    const long long N = (long long)a.n[0] * a.n[1] * a.n[2];
+    const int blocks = (int)std::min<long long>((N + 255) / 256, 8192);
+    hipLaunchKernelGGL(gen_bumps_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, a, out, N);
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
+}
 }  // namespace opf
+
+extern "C" int opf_gen_bumps_grid_dev(opf_ctx* ctx, const opf_poisson_desc* d, uint64_t seed, int nbumps, double noise,
+                                      double* dev_out) {
+    OPF_CHECK(ctx && d && dev_out && d->dim >= 1 && d->dim <= 3, OPF_ERR_ARG, "opf_gen_bumps_grid_dev: bad args");
+    return gen_bumps_grid_dev(ctx->stream, d->dim, d->n, seed, nbumps, noise, dev_out);
+}
 
 extern "C" int opf_gen_bumps(const opf_mesh* m, const opf_field_desc* d, uint64_t seed, int nbumps, double noise,
                              double* out) {

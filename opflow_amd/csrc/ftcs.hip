@@ -219,22 +219,28 @@ __global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T
 // landed, level L (1..K) computes its plane zz - L from level L-1's planes
 // zz-L-1 .. zz-L+1 (a ring of 3 planes per level) over an x-y window that
 // shrinks by one cell per level; level K is exactly the tile and goes straight
-// to global memory. The next level-0 plane is prefetched into registers while
-// the levels compute. Per cell and step the arithmetic is that of
+// to global memory. Level-0 planes are loaded PF planes ahead into registers,
+// so their HBM latency overlaps the compute of the planes before them. Reads
+// wrap to interior cells, so the input's ghosts are never read; the output's
+// ghosts are left stale and refilled by the host loop before anything reads
+// them (a branch-free body keeps the load waits counted). Per cell and step the arithmetic is that of
 // ftcs3d_kernel, so the result has the same bits as K single steps.
 template <int TX, int TY, int K>
 struct Ftcs3dTile {
     static constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
 };
 
-template <typename T, bool POW2, int TX, int TY, int K, int NT>
+template <typename T, bool POW2, int TX, int TY, int K, int NT, int PF>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
                                                        int zc) {
     static_assert(K >= 2, "level K must not share a barrier interval with the level-0 store");
     using G = Ftcs3dTile<TX, TY, K>;
     constexpr int RX = G::RX, RY = G::RY, RP = G::RP;
     constexpr int NL = (RP + NT - 1) / NT;
-    __shared__ T ring[K][3][RP];
+    // level 0 rows are padded to NL * NT so every thread loads and stores
+    // unconditionally (no branches around the loads: their waits stay counted)
+    __shared__ T ring0[3][NL * NT];
+    __shared__ T ringl[K - 1][3][RP];
     const int tilesx = (a.n[0] + TX - 1) / TX, tilesy = (a.n[1] + TY - 1) / TY;
     int b = blockIdx.x;
     const int x0 = (b % tilesx) * TX;
@@ -243,69 +249,100 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     const int z0 = (b / tilesy) * zc;
     const int z1 = min(z0 + zc, a.n[2]);
     const long long sz = (long long)a.e0 * a.e1;
-    // wrapped in-plane storage offsets of this thread's level-0 elements
+    // wrapped in-plane storage offsets of this thread's level-0 elements (padding
+    // elements load the region's first cell)
     int off[NL];
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
         int e = threadIdx.x + m * NT;
-        int lx = e % RX, ly = e / RX;
-        int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
-        int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
-        off[m] = e < RP ? (gx + 1) + a.e0 * (gy + 1) : -1;
+        if (e >= RP) e = 0;
+        const int lx = e % RX, ly = e / RX;
+        const int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
+        const int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
+        off[m] = (gx + 1) + a.e0 * (gy + 1);
     }
-    auto plane = [&](int z) -> const T* {
-        z = (z % a.n[2] + a.n[2]) % a.n[2];
-        return in + sz * (z + 1);
-    };
     auto slot = [](int z) { return (z + 3 * K) % 3; };  // z >= -K
-    T pre[NL];
-    {
-        const T* p = plane(z0 - K);
+    auto level = [&](int L, int s) -> T* { return L == 0 ? ring0[s] : ringl[L - 1][s]; };
+    // level-0 planes zb .. ze-1 stream through PF register buffers (loads run
+    // PF planes ahead of the plane being stored to LDS)
+    const int zb = z0 - K, ze = z1 + K;
+    T pre[PF][NL];
+    auto load = [&](T (&r)[NL], int z) {
+        z = (z % a.n[2] + a.n[2]) % a.n[2];
+        const T* p = in + sz * (z + 1);
 #pragma unroll
-        for (int m = 0; m < NL; ++m)
-            if (off[m] >= 0) pre[m] = p[off[m]];
-    }
-    for (int zz = z0 - K; zz < z1 + K; ++zz) {
-        T* r0 = ring[0][slot(zz)];
+        for (int m = 0; m < NL; ++m) r[m] = p[off[m]];
+    };
 #pragma unroll
-        for (int m = 0; m < NL; ++m)
-            if (off[m] >= 0) r0[threadIdx.x + m * NT] = pre[m];
-        if (zz + 1 < z1 + K) {
-            const T* p = plane(zz + 1);
+    for (int u = 0; u < PF; ++u) load(pre[u], min(zb + u, ze - 1));
+    for (int zz0 = zb; zz0 < ze; zz0 += PF) {
 #pragma unroll
-            for (int m = 0; m < NL; ++m)
-                if (off[m] >= 0) pre[m] = p[off[m]];
-        }
-        __syncthreads();
+        for (int u = 0; u < PF; ++u) {
+            const int zz = zz0 + u;
+            if (zz >= ze) break;  // uniform over the block
+            T* r0 = ring0[slot(zz)];
 #pragma unroll
-        for (int L = 1; L <= K; ++L) {
-            const int zl = zz - L;
-            if (zl < z0 - K + L) break;  // pipeline still filling (uniform over the block)
-            const T* pm = ring[L - 1][slot(zl - 1)];
-            const T* pc = ring[L - 1][slot(zl)];
-            const T* pp = ring[L - 1][slot(zl + 1)];
-            const int w = RX - 2 * L, h = RY - 2 * L;
-            for (int t = threadIdx.x; t < w * h; t += NT) {
-                const int lx = L + t % w, ly = L + t / w;
-                const int e = lx + RX * ly;
-                const T c = pc[e];
-                T s = d2<T, POW2>(pc[e - 1], c, pc[e + 1], a.hh[0]) + d2<T, POW2>(pc[e - RX], c, pc[e + RX], a.hh[1]);
-                s = s + d2<T, POW2>(pm[e], c, pp[e], a.hh[2]);
-                const T v = c + a.coef * s;
-                if (L < K) {
-                    ring[L][slot(zl)][e] = v;
-                } else {
-                    const int i = x0 + lx - K, j = y0 + ly - K;
-                    if (i < a.n[0] && j < a.n[1]) {
-                        int p[3] = {i, j, zl};
-                        out[ftcs_index<3>(a.e0, a.e1, p)] = v;
-                        write_images<T, 3>(out, a, p, v);
+            for (int m = 0; m < NL; ++m) r0[threadIdx.x + m * NT] = pre[u][m];
+            load(pre[u], min(zz + PF, ze - 1));
+            __syncthreads();
+#pragma unroll
+            for (int L = 1; L <= K; ++L) {
+                const int zl = zz - L;
+                if (zl < z0 - K + L) break;  // pipeline still filling (uniform over the block)
+                const T* pm = level(L - 1, slot(zl - 1));
+                const T* pc = level(L - 1, slot(zl));
+                const T* pp = level(L - 1, slot(zl + 1));
+                const int w = RX - 2 * L, h = RY - 2 * L;
+                for (int t = threadIdx.x; t < w * h; t += NT) {
+                    const int lx = L + t % w, ly = L + t / w;
+                    const int e = lx + RX * ly;
+                    const T c = pc[e];
+                    T s = d2<T, POW2>(pc[e - 1], c, pc[e + 1], a.hh[0]) +
+                          d2<T, POW2>(pc[e - RX], c, pc[e + RX], a.hh[1]);
+                    s = s + d2<T, POW2>(pm[e], c, pp[e], a.hh[2]);
+                    const T v = c + a.coef * s;
+                    if (L < K) {
+                        ringl[L - 1][slot(zl)][e] = v;
+                    } else {
+                        const int i = x0 + lx - K, j = y0 + ly - K;
+                        if (i < a.n[0] && j < a.n[1]) {
+                            const int p[3] = {i, j, zl};
+                            out[ftcs_index<3>(a.e0, a.e1, p)] = v;
+                        }
                     }
                 }
+                if (L < K) __syncthreads();
             }
-            if (L < K) __syncthreads();
         }
     }
+}
+
+// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch;
+// 3-D single steps march FT_ZC planes per block
+constexpr int FT_ZC = 16, FT_TS2 = 32, FT_TB2 = 8;
+
+// 3-D blocked launch shape per precision (measured at 512^3 on MI355X):
+// TX x TY columns, K steps per launch, NT threads, PF planes of prefetch, z
+// chunks of ZC planes
+template <typename T>
+struct Ftcs3dCfg;
+template <>
+struct Ftcs3dCfg<double> {
+    static constexpr int TX = 32, TY = 16, K = 2, NT = 256, PF = 2, ZC = 64;
+};
+template <>
+struct Ftcs3dCfg<float> {
+    static constexpr int TX = 64, TY = 8, K = 2, NT = 256, PF = 3, ZC = 32;
+};
+
+// Steps the next launch advances: the launch plan shared by the sweep and
+// opf_ftcs_plan. tblock = 1 forces single steps.
+int launch_steps(int dim, int dtype, const int n[3], const int per[3], long long plane, int remaining, int tblock) {
+    if (tblock == 1) return 1;
+    if (dim == 2 && remaining >= FT_TB2 && n[0] >= FT_TB2 && n[1] >= FT_TB2) return FT_TB2;
+    const int k3 = dtype == OPF_F64 ? Ftcs3dCfg<double>::K : Ftcs3dCfg<float>::K;
+    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= k3) return k3;
+    return 1;
 }
 
 bool is_pow2(double h) {
@@ -364,41 +401,41 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         OPF_HIP(hipMemcpyAsync(f->spare, f->data, f->nbytes, hipMemcpyDeviceToDevice, st));
         f->spare_valid = true;
     }
-    constexpr int ZC = 16;
-    constexpr int TS2 = 32, TB2 = 8;
-    constexpr int TX3 = 32, TY3 = 32, K3 = 2, ZC3 = 64;
-    const bool tb3 = tblock != 1 && d.dim == 3 && a.per[0] && a.per[1] && a.per[2] &&
-                     (long long)a.e0 * a.e1 < (1LL << 31);
-    const bool tb2 = tblock != 1;
-    for (int it = 0; it < nsteps; ++it) {
+    bool ghosts_stale = false;  // the 3-D blocked kernel writes interior cells only
+    for (int it = 0; it < nsteps;) {
         const T* in = (const T*)f->data;
         T* out = (T*)f->spare;
-        if (d.dim == 2 && tb2 && nsteps - it >= TB2 && a.n[0] >= TB2 && a.n[1] >= TB2) {
-            // temporally blocked: TB2 steps per launch
-            int tiles = ((a.n[0] + TS2 - 1) / TS2) * ((a.n[1] + TS2 - 1) / TS2);
-            if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, TS2, TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, TB2);
-            else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, TS2, TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, TB2);
-            it += TB2 - 1;
+        const int k = launch_steps(d.dim, d.dtype, a.n, a.per, (long long)a.e0 * a.e1, nsteps - it, tblock);
+        if (ghosts_stale && !(d.dim == 3 && k > 1)) {
+            OPF_TRY(launch_ghost_fill(f, f->data));
+            ghosts_stale = false;
+        }
+        if (d.dim == 2 && k == FT_TB2) {
+            int tiles = ((a.n[0] + FT_TS2 - 1) / FT_TS2) * ((a.n[1] + FT_TS2 - 1) / FT_TS2);
+            if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+            else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
         } else if (d.dim == 2) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
             if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);
             else hipLaunchKernelGGL((ftcs2d_kernel<T, false>), grid, dim3(64, 4), 0, st, in, out, a);
-        } else if (d.dim == 3 && tb3 && nsteps - it >= K3) {
-            // temporally blocked: K3 steps per launch
-            int nb = ((a.n[0] + TX3 - 1) / TX3) * ((a.n[1] + TY3 - 1) / TY3) * ((a.n[2] + ZC3 - 1) / ZC3);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, TX3, TY3, K3, 256>), dim3(nb), dim3(256), 0, st, in, out, a, ZC3);
-            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, TX3, TY3, K3, 256>), dim3(nb), dim3(256), 0, st, in, out, a, ZC3);
-            it += K3 - 1;
+        } else if (d.dim == 3 && k > 1) {
+            using C3 = Ftcs3dCfg<T>;
+            int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::K, C3::NT, C3::PF>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::K, C3::NT, C3::PF>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            ghosts_stale = true;
         } else if (d.dim == 3) {
-            dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + ZC - 1) / ZC);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_kernel<T, true, ZC>), grid, dim3(64, 4), 0, st, in, out, a);
-            else hipLaunchKernelGGL((ftcs3d_kernel<T, false, ZC>), grid, dim3(64, 4), 0, st, in, out, a);
+            dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + FT_ZC - 1) / FT_ZC);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_kernel<T, true, FT_ZC>), grid, dim3(64, 4), 0, st, in, out, a);
+            else hipLaunchKernelGGL((ftcs3d_kernel<T, false, FT_ZC>), grid, dim3(64, 4), 0, st, in, out, a);
         } else {
             set_error("ftcs: dim %d not supported by the fast path", d.dim);
             return OPF_ERR_UNSUPPORTED;
         }
+        it += k;
         std::swap(f->data, f->spare);
     }
+    if (ghosts_stale) OPF_TRY(launch_ghost_fill(f, f->data));
     OPF_HIP(hipGetLastError());
     return OPF_OK;
 }
@@ -409,6 +446,25 @@ extern "C" int opf_ftcs(opf_field* u, double coef, int nsteps) {
     OPF_CHECK(u && nsteps >= 0, OPF_ERR_ARG, "opf_ftcs: bad args");
     if (nsteps == 0) return OPF_OK;
     return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps, 0) : ftcs_t<float>(u, coef, nsteps, 0);
+}
+
+extern "C" int opf_ftcs_plan(const opf_field* u, int nsteps, int tblock, int* launches, int* max_steps) {
+    OPF_CHECK(u && nsteps >= 0 && tblock >= 0 && launches && max_steps, OPF_ERR_ARG, "opf_ftcs_plan: bad args");
+    const opf_field_desc& d = u->d;
+    int n[3] = {1, 1, 1}, per[3] = {0, 0, 0};
+    for (int q = 0; q < d.dim; ++q) {
+        n[q] = d.local_end[q] - d.local_start[q];
+        per[q] = d.bc_type[q][0] == OPF_BC_PERIODIC;
+    }
+    *launches = 0;
+    *max_steps = 0;
+    for (int it = 0; it < nsteps;) {
+        int k = launch_steps(d.dim, d.dtype, n, per, (long long)d.extent[0] * d.extent[1], nsteps - it, tblock);
+        ++*launches;
+        *max_steps = std::max(*max_steps, k);
+        it += k;
+    }
+    return OPF_OK;
 }
 
 extern "C" int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock) {

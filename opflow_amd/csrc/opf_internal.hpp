@@ -99,6 +99,8 @@ int launch_ghost_fill(const opf_field* f, void* data);
 
 // seeded Gaussian bumps on a unit-cube cell-centred grid (host)
 void gen_bumps_grid(int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out);
+// the same field generated on the device into `out` (device array, N doubles), on stream st
+int gen_bumps_grid_dev(hipStream_t st, int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out);
 
 // device program cache
 int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev);

@@ -233,9 +233,15 @@ int try_admit(opf_srv* s) {
         sl.t_submit = p.t_submit;
         sl.t_admit = s->now();
         sl.out_x = p.req.out_x;
-        OPF_HIP(hipMemcpyAsync(g->b64 + (size_t)k * g->N, p.rhs.data(), 8 * g->N, hipMemcpyHostToDevice, st));
+        double* bk = g->b64 + (size_t)k * g->N;
         OPF_HIP(hipMemsetAsync(g->x64 + (size_t)k * g->N, 0, 8 * g->N, st));
-        OPF_HIP(hipStreamSynchronize(st));  // the host RHS buffer is released below
+        if (p.rhs.empty()) {
+            // synthetic request: its bumps are generated in place on the device
+            OPF_TRY(gen_bumps_grid_dev(st, p.req.desc.dim, p.req.desc.n, p.req.seed, p.req.nbumps, p.req.noise, bk));
+        } else {
+            OPF_HIP(hipMemcpyAsync(bk, p.rhs.data(), 8 * g->N, hipMemcpyHostToDevice, st));
+            OPF_HIP(hipStreamSynchronize(st));  // the host RHS buffer is released below
+        }
         ++s->inflight;
         it = s->queue.erase(it);
     }
@@ -429,9 +435,11 @@ extern "C" int opf_srv_submit(opf_srv* s, const opf_srv_request* req, uint64_t* 
         OPF_CHECK(req->desc.n[q] >= 2, OPF_ERR_ARG, "opf_srv_submit: n >= 2");
         N *= req->desc.n[q];
     }
-    p.rhs.resize(N);
-    if (req->rhs) std::copy(req->rhs, req->rhs + N, p.rhs.begin());
-    else gen_bumps_grid(req->desc.dim, req->desc.n, req->seed, req->nbumps, req->noise, p.rhs.data());
+    if (req->rhs) {
+        p.rhs.assign(req->rhs, req->rhs + N);
+    } else {
+        OPF_CHECK(req->nbumps >= 0 && req->nbumps <= 32, OPF_ERR_ARG, "opf_srv_submit: 0..32 bumps");
+    }
     p.t_submit = req->t_submit >= 0 ? req->t_submit : s->now();
     s->queue.push_back(std::move(p));
     *id = s->queue.back().id;

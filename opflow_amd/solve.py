@@ -122,6 +122,11 @@ def gen_bumps_grid(desc, seed, nbumps, noise=0.0):
     return out
 
 
+def gen_bumps_grid_dev(ctx, desc, seed, nbumps, noise, buf):
+    """gen_bumps_grid's field written on the device into DeviceBuffer `buf` (context stream)."""
+    L.check(ctx.lib.opf_gen_bumps_grid_dev(ctx.h, C.byref(desc), C.c_uint64(seed), nbumps, noise, buf.ptr))
+
+
 def poisson_desc(dim, n, h, bc, sigma=0.0):
     d = L.PoissonDesc()
     d.dim = dim
@@ -152,6 +157,9 @@ class DeviceBuffer:
         out = np.empty(self.n)
         L.check(self.lib.opf_buf_download(self.ctx.h, self.ptr, out.ctypes.data, out.nbytes))
         return out
+
+    def zero(self):
+        L.check(self.lib.opf_buf_zero(self.ctx.h, self.ptr, self.n * 8))
 
     def close(self):
         if self.ptr:

@@ -106,6 +106,11 @@ def test_ftcs3d_periodic_ragged_blocked(ctx):
         u = build_field(ctx, mesh, 3, 1, [(PER, PER)] * 3)
         init = oracle.update_padding(u.desc, om, random_storage(u, SEED + 9))
         u.upload(init)
+        nl, k = u.ftcs_plan(steps, tblock)
+        if tblock == 1:
+            assert (nl, k) == (steps, 1)
+        else:
+            assert k > 1 and nl == steps // k + steps % k
         u.ftcs_blocked(0.2 / 32**2, steps, tblock)
         fields.append(u.numpy())
         u.close()

@@ -103,3 +103,15 @@ def test_memory_admission_queues(ctx):
         b = of.gen_bumps_grid(_desc(n), seed, nb)
         r = b - oracle.poisson_apply(n, [1.0 / n[0]] * 2, [(D, D)] * 2, outs[k])
         assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
+
+
+def test_device_rhs_generator_matches_host(ctx):
+    """Device bumps + noise equal the host generator's (same SplitMix64 draws) to the last bits of exp/log/cos."""
+    for n, nb, noise in (([96, 80], 5, 0.0), ([24, 20, 16], 8, 1e-3), ([64, 64], 32, 0.5)):
+        desc = _desc(n)
+        host = of.gen_bumps_grid(desc, 1234 + nb, nb, noise)
+        buf = of.DeviceBuffer(ctx, host.size)
+        of.gen_bumps_grid_dev(ctx, desc, 1234 + nb, nb, noise, buf)
+        dev = buf.numpy()
+        buf.close()
+        assert np.max(np.abs(dev - host)) <= 1e-14 * np.max(np.abs(host)), (n, nb, noise)
