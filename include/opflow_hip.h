@@ -261,6 +261,16 @@ int opf_eqn_create(const opf_expr_node* nodes, int nnodes, int lhs_root, int rhs
  * regenerate b (and A unless static_mat), solve from the target's current
  * values, write the solution back, updatePadding.                          */
 int opf_eqn_solve(opf_eqn* e, opf_solve_state* state);
+/* The equation's linear system in the CSR form CSRMatrixGenerator::generate /
+ * generate_rhs produce (CSRMatrixGenerator.hpp:55-220, the AMGCL path of
+ * AMGCLEqnSolveHandler.hpp:92-99): one row per unknown, x-fastest over the
+ * target's assignable range; row = StencilPad of lhs - rhs with columns
+ * ascending; rhs = -bias; with params.pin_value the LAST row becomes
+ * x_last = 0 (rhs 0). The pads are evaluated on the device from the current
+ * field values; the rows are assembled on the host. Call with null arrays to
+ * get nrows / nnz, then with ptr[nrows+1], col[nnz], val[nnz], rhs[nrows].  */
+int opf_eqn_csr(opf_eqn* e, long long* nrows, long long* nnz, long long* ptr, long long* col, double* val,
+                double* rhs);
 int opf_eqn_destroy(opf_eqn* e);
 
 /* Direct multigrid API on device vectors (used by the serving loop and the

@@ -128,6 +128,7 @@ SIGNATURES = {
     "opf_eqn_create": (_I, [C.POINTER(ExprNode), _I, _I, _I, C.POINTER(_P), _I, _P, C.POINTER(SolveParams),
                             C.POINTER(_P)]),
     "opf_eqn_solve": (_I, [_P, C.POINTER(SolveState)]),
+    "opf_eqn_csr": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _P, _P, _P, _P]),
     "opf_eqn_destroy": (_I, [_P]),
     "opf_gmg_create": (_I, [_P, C.POINTER(PoissonDesc), _I, C.POINTER(_P)]),
     "opf_gmg_solve": (_I, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveState)]),

@@ -496,6 +496,66 @@ extern "C" int opf_eqn_solve(opf_eqn* e, opf_solve_state* st) {
     return OPF_OK;
 }
 
+extern "C" int opf_eqn_csr(opf_eqn* e, long long* nrows, long long* nnz, long long* ptr, long long* col, double* val,
+                           double* rhs) {
+    // CSRMatrixGenerator::generate / generate_rhs (CSRMatrixGenerator.hpp:55-220) for one equation:
+    // row r = the StencilPad of lhs - rhs at unknown r (x-fastest over the target's
+    // assignable range), columns ascending, rhs = -bias; with pinValue the last row
+    // becomes x_last = 0.
+    OPF_CHECK(e && nrows && nnz, OPF_ERR_ARG, "opf_eqn_csr: null arg");
+    OPF_TRY(eqn_eval_pads(e));
+    const long long N = e->N;
+    const int dim = e->target->d.dim;
+    std::vector<double> c((size_t)PAD_K * N), bias((size_t)N);
+    OPF_HIP(hipMemcpyAsync(c.data(), e->coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, e->ctx->stream));
+    OPF_HIP(hipMemcpyAsync(bias.data(), e->bias, sizeof(double) * N, hipMemcpyDeviceToHost, e->ctx->stream));
+    OPF_HIP(hipStreamSynchronize(e->ctx->stream));
+    const bool pin = e->params.pin_value != 0;
+    const long long st[3] = {1, e->n[0], (long long)e->n[0] * e->n[1]};
+    long long cnt = 0;
+    if (ptr) ptr[0] = 0;
+    for (long long r = 0; r < N; ++r) {
+        std::pair<long long, double> ent[7];
+        int ne = 0;
+        if (pin && r == N - 1) {
+            ent[ne++] = {r, 1.0};
+        } else {
+            ent[ne++] = {r, c[r]};
+            long long rem = r;
+            int p[3] = {0, 0, 0};
+            for (int d = 0; d < 3; ++d) {
+                p[d] = (int)(rem % e->n[d]);
+                rem /= e->n[d];
+            }
+            for (int d = 0; d < dim; ++d)
+                for (int sd = 0; sd < 2; ++sd) {
+                    const double a = c[(size_t)(1 + 2 * d + sd) * N + r];
+                    const bool edge = sd == 0 ? p[d] == 0 : p[d] == e->n[d] - 1;
+                    if (edge && !e->per[d]) continue;  // folded into the diagonal and bias by the BC
+                    long long q = r + (sd == 0 ? -st[d] : st[d]);
+                    if (edge) q = r + (sd == 0 ? 1 : -1) * (long long)(e->n[d] - 1) * st[d];
+                    bool merged = false;  // a periodic extent of 1 or 2 meets the same cell twice
+                    for (int m = 0; m < ne; ++m)
+                        if (ent[m].first == q) {
+                            ent[m].second += a;
+                            merged = true;
+                        }
+                    if (!merged) ent[ne++] = {q, a};
+                }
+            std::sort(ent, ent + ne);
+        }
+        for (int m = 0; m < ne; ++m, ++cnt) {
+            if (col) col[cnt] = ent[m].first;
+            if (val) val[cnt] = ent[m].second;
+        }
+        if (ptr) ptr[r + 1] = cnt;
+        if (rhs) rhs[r] = (pin && r == N - 1) ? 0.0 : -bias[r];
+    }
+    *nrows = N;
+    *nnz = cnt;
+    return OPF_OK;
+}
+
 extern "C" int opf_eqn_destroy(opf_eqn* e) {
     if (!e) return OPF_OK;
     hipStreamSynchronize(e->ctx->stream);

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "fastdiv.hpp"
 #include "opf_internal.hpp"
 
 using namespace opf;
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
     constexpr int W = Ftcs2dTile<TS, TB>::W;
     __shared__ T buf[2][W][W];
     const int tilesx = (a.n[0] + TS - 1) / TS;
-    const int x0 = (blockIdx.x % tilesx) * TS, y0 = (blockIdx.x / tilesx) * TS;
+    const int tb = (int)xcd_block(blockIdx.x, gridDim.x);  // x-neighbouring tiles share an XCD's L2
+    const int x0 = (tb % tilesx) * TS, y0 = (tb / tilesx) * TS;
     // region origin in global cell coordinates; non-periodic sides clamp to the ghost layer
     int lo[2], hi[2];
     const int org[2] = {x0, y0};
@@ -230,7 +232,7 @@ struct Ftcs3dTile {
     static constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
 };
 
-template <typename T, bool POW2, int TX, int TY, int K, int NT, int PF>
+template <typename T, bool POW2, int TX, int TY, int K, int NT, int PF, bool XCD>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
                                                        int zc) {
     static_assert(K >= 2, "level K must not share a barrier interval with the level-0 store");
@@ -242,7 +244,8 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     __shared__ T ring0[3][NL * NT];
     __shared__ T ringl[K - 1][3][RP];
     const int tilesx = (a.n[0] + TX - 1) / TX, tilesy = (a.n[1] + TY - 1) / TY;
-    int b = blockIdx.x;
+    // XCD: neighbouring tiles share an XCD's L2 (halo reuse)
+    int b = XCD ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int x0 = (b % tilesx) * TX;
     b /= tilesx;
     const int y0 = (b % tilesy) * TY;
@@ -329,10 +332,12 @@ struct Ftcs3dCfg;
 template <>
 struct Ftcs3dCfg<double> {
     static constexpr int TX = 32, TY = 16, K = 2, NT = 256, PF = 2, ZC = 64;
+    static constexpr bool XCD = true;
 };
 template <>
 struct Ftcs3dCfg<float> {
     static constexpr int TX = 64, TY = 8, K = 2, NT = 256, PF = 3, ZC = 32;
+    static constexpr bool XCD = true;
 };
 
 // Steps the next launch advances: the launch plan shared by the sweep and
@@ -421,8 +426,8 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         } else if (d.dim == 3 && k > 1) {
             using C3 = Ftcs3dCfg<T>;
             int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::K, C3::NT, C3::PF>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::K, C3::NT, C3::PF>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::K, C3::NT, C3::PF, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::K, C3::NT, C3::PF, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
             ghosts_stale = true;
         } else if (d.dim == 3) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + FT_ZC - 1) / FT_ZC);

@@ -69,6 +69,20 @@ __device__ __forceinline__ T apply_op(const GOp<T>& o, int i, int j, int k, uint
     return D * xc - acc;
 }
 
+// cell of this thread in a one-cell-per-thread launch (XCD-contiguous blocks)
+__device__ __forceinline__ uint32_t cell_id() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
+
+// contiguous range of cells of this block in a fixed-grid reduction launch
+// (ranges are multiples of the block size; the partial-sum order depends only
+// on N and the grid, so results are reproducible)
+__device__ __forceinline__ void block_range(uint32_t N, uint32_t& beg, uint32_t& end) {
+    const uint32_t G = gridDim.x;
+    const uint32_t per = (((N + G - 1) / G) + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const uint32_t lb = xcd_block(blockIdx.x, G);
+    beg = min(N, lb * per);
+    end = min(N, beg + per);
+}
+
 // linear cell index -> (i, j, k) for extents n0, n1 (f0, f1 divide by them)
 __device__ __forceinline__ void cell_of(uint32_t t, const FastDiv& f0, const FastDiv& f1, int& i, int& j, int& k) {
     const uint32_t r = f0.div(t);
@@ -124,7 +138,9 @@ __global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x
     if (r) r += vo;
     if (part) part += (long long)slot * bt.ps;
     double acc = 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+    uint32_t tb, te;
+    block_range(N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x) {
         T v = b[t];
         if (x) {
             int i, j, k;
@@ -146,7 +162,7 @@ __global__ __launch_bounds__(256) void k_resid(GOp<T> o, const T* __restrict__ x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_apply(GOp<T> o, const T* __restrict__ x, T* __restrict__ y) {
     const uint32_t N = o.s2 * (uint32_t)o.n2;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = cell_id();
     if (t >= N) return;
     int i, j, k;
     cell_of(t, o.f0, o.f1, i, j, k);
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ 
     if (xi) xi += vo;
     b += vo;
     xo += vo;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = cell_id();
     if (t >= N) return;
     int i, j, k;
     cell_of(t, o.f0, o.f1, i, j, k);
@@ -191,7 +207,7 @@ __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict_
     x += vo;
     b += vo;
     bc += (long long)slot * NC;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = cell_id();
     if (t >= NC) return;
     int I, J, K;
     cell_of(t, fc0, fc1, I, J, K);
@@ -218,7 +234,7 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, const T* __
     const uint32_t N = o.s2 * (uint32_t)o.n2;
     xf += vo;
     xc += (long long)slot * ((long long)c0 * c1 * c2);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = cell_id();
     if (t >= N) return;
     int i, j, k;
     cell_of(t, o.f0, o.f1, i, j, k);
@@ -324,7 +340,9 @@ __global__ __launch_bounds__(256) void k_spmv_p(GOp<T> o, const T* __restrict__ 
     const T beta = (T)s[S_BETA];
     const T zm = (T)s[S_ZMEAN];
     double acc = 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+    uint32_t tb, te;
+    block_range(N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x) {
         int i, j, k;
         cell_of(t, o.f0, o.f1, i, j, k);
         auto pv = [&](uint32_t l) { return (z[l] - zm) + beta * pold[l]; };
@@ -353,7 +371,9 @@ __global__ __launch_bounds__(256) void k_axpy2(long long N, T* __restrict__ x, c
     part += (long long)slot * bt.ps;
     const T alpha = (T)s[S_ALPHA];
     double acc = 0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+    uint32_t tb, te;
+    block_range((uint32_t)N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x) {
         x[t] += alpha * p[t];
         T rv = r[t] - alpha * q[t];
         r[t] = rv;
@@ -373,7 +393,9 @@ __global__ __launch_bounds__(256) void k_dot3(long long N, const T* __restrict__
     q += vo;
     part += (long long)slot * bt.ps;
     double a = 0, b = 0, c = 0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+    uint32_t tb, te;
+    block_range((uint32_t)N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x) {
         double zv = (double)z[t];
         a += (double)r[t] * zv;
         b += zv * (double)q[t];
@@ -395,16 +417,16 @@ __global__ void k_add_to64(long long N, double* __restrict__ x, const TE* __rest
     BAT_SLOT(bt);
     x += vo;
     e += vo;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) x[t] += (double)e[t];
+    const uint32_t t = cell_id();
+    if (t < (uint32_t)N) x[t] += (double)e[t];
 }
 
 template <typename T>
 __global__ void k_fill(long long N, T* __restrict__ x, T v, Bat bt) {
     BAT_SLOT(bt);
     x += vo;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) x[t] = v;
+    const uint32_t t = cell_id();
+    if (t < (uint32_t)N) x[t] = v;
 }
 
 // subtract a device scalar mean (singular problems): x -= s[slot] * scale
@@ -414,8 +436,8 @@ __global__ void k_sub_mean(long long N, T* __restrict__ x, const double* __restr
     BAT_SLOT(bt);
     x += vo;
     s += (long long)slot * bt.ss;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) x[t] -= (T)(s[which] * scale);
+    const uint32_t t = cell_id();
+    if (t < (uint32_t)N) x[t] -= (T)(s[which] * scale);
 }
 
 template <typename T>
@@ -424,7 +446,9 @@ __global__ void k_sum(long long N, const T* __restrict__ x, double* __restrict__
     x += vo;
     part += (long long)slot * bt.ps;
     double a = 0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x)
+    uint32_t tb, te;
+    block_range((uint32_t)N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x)
         a += (double)x[t];
     a = gmg_block_sum(a);
     if (threadIdx.x == 0) part[blockIdx.x] = a;
@@ -590,6 +614,8 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
     }
     H.dim = fine.n[2] > 1 ? 3 : 2;
     if (fine.n[1] == 1) return (set_error("gmg: 1-D grids are not supported"), OPF_ERR_UNSUPPORTED);
+    OPF_CHECK(fine.N < (1LL << 31), OPF_ERR_UNSUPPORTED, "gmg: %lld cells per system (kernels index cells with 32 bits)",
+              fine.N);
     H.omega = H.dim == 3 ? 6.0 / 7.0 : 0.8;
     // singular: no Dirichlet-like face and no diagonal shift
     bool shift = fine.var ? false : fine.sigma != 0.0;

@@ -13,7 +13,8 @@
 // an FP32 flexible-CG inner solve on A e = r, x += e. One scheduling round
 //   1. admits queued requests into free slots while the memory budget allows,
 //   2. runs the FP64 outer step for the slots that need it (batched),
-//   3. replays the group's captured hipGraph of two batched inner iterations,
+//   3. replays the group's captured hipGraph of two batched inner iterations
+//      (1-3 times by problem size),
 //   4. reads every slot's scalars with one copy and advances the state machines.
 // Requests that arrive while others run are admitted in the next round, so
 // their work overlaps on the device with requests already in flight.
@@ -63,6 +64,8 @@ struct Group {
     std::vector<Slot> slots;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t st = nullptr;  // the group's own stream: groups overlap on the device
+    int replays = 1;           // graph replays (2 inner iterations each) per round
     size_t bytes = 0;
     double last_used = 0;
     int busy() const {
@@ -89,6 +92,7 @@ struct opf_srv {
     std::deque<Pending> queue;
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<opf_srv_result> done;
+    std::vector<hipStream_t> done_streams;  // stream carrying each done result's out_x copy
     uint64_t next_id = 1;
     size_t used = 0;
     int inflight = 0;
@@ -108,6 +112,7 @@ size_t group_bytes(const opf_poisson_desc& d, int precision, int B, long long N)
 }
 
 void free_group(Group* g) {
+    if (g->st) hipStreamSynchronize(g->st);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     gmg_free(g->H);
@@ -115,7 +120,12 @@ void free_group(Group* g) {
     if (g->b64) hipFree(g->b64);
     if (g->dmask) hipFree(g->dmask);
     if (g->hmask) hipHostFree(g->hmask);
+    if (g->st) hipStreamDestroy(g->st);
 }
+
+// Inner iterations per round by problem size only (never by the mix, so a
+// request's iteration sequence, and its result, do not depend on other requests)
+int replays_for(long long N) { return N <= (1 << 16) ? 3 : N <= (1 << 20) ? 2 : 1; }
 
 int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     int per[3] = {0, 0, 0};
@@ -140,6 +150,8 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     g->d = d;
     g->N = L.N;
     g->B = slots_for(*s, L.N);
+    g->replays = replays_for(L.N);
+    OPF_HIP(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
     int prec = s->cfg.precision == OPF_PREC_MIXED ? OPF_PREC_MIXED : OPF_PREC_FP64;
     int rc = gmg_build(s->ctx, g->H, L, per, alpha, prec, g->B);
     if (rc != OPF_OK) {
@@ -170,7 +182,7 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
 // capture two batched inner iterations (the p ping-pong returns to its start)
 int capture(opf_srv* s, Group* g) {
     if (!s->cfg.batched || g->exec) return OPF_OK;
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = g->st;
     const int* m = g->dmask + M_ITER * g->B;
     OPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     if (g->H.precision == OPF_PREC_MIXED) {
@@ -186,7 +198,6 @@ int capture(opf_srv* s, Group* g) {
 }
 
 int try_admit(opf_srv* s) {
-    hipStream_t st = s->ctx->stream;
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         if (s->inflight >= s->cfg.max_inflight) break;
         Pending& p = *it;
@@ -211,9 +222,10 @@ int try_admit(opf_srv* s) {
                         victim = (int)q;
                     }
                 if (victim < 0) break;
-                OPF_HIP(hipStreamSynchronize(st));
                 s->used -= s->groups[victim]->bytes;
-                free_group(s->groups[victim].get());
+                free_group(s->groups[victim].get());  // waits for its stream (pending out_x copies too)
+                for (auto& ds : s->done_streams)
+                    if (ds == s->groups[victim]->st) ds = nullptr;
                 s->groups.erase(s->groups.begin() + victim);
             }
             if (s->used + need > s->cfg.mem_budget) {
@@ -233,6 +245,7 @@ int try_admit(opf_srv* s) {
         sl.t_submit = p.t_submit;
         sl.t_admit = s->now();
         sl.out_x = p.req.out_x;
+        hipStream_t st = g->st;
         double* bk = g->b64 + (size_t)k * g->N;
         OPF_HIP(hipMemsetAsync(g->x64 + (size_t)k * g->N, 0, 8 * g->N, st));
         if (p.rhs.empty()) {
@@ -266,19 +279,23 @@ void set_masks(Group* g) {
 }
 
 int round(opf_srv* s, int* nfinished) {
-    hipStream_t st = s->ctx->stream;
     *nfinished = 0;
     OPF_TRY(try_admit(s));
     const double tol = s->cfg.tol > 0 ? s->cfg.tol : 1e-8;
     const int maxit = s->cfg.max_iter > 0 ? s->cfg.max_iter : 200;
-    // ---- outer FP64 step for NEW / OUTER slots
+    auto needs = [](const Group* g, int a, int b) {
+        for (auto& sl : g->slots)
+            if (sl.phase == a || sl.phase == b) return true;
+        return false;
+    };
+    // ---- outer FP64 step for NEW / OUTER slots: enqueue every group's work on
+    // its own stream first, then wait for them all
     bool any_outer = false;
     for (auto& gp : s->groups) {
         Group* g = gp.get();
-        bool need = false;
-        for (auto& sl : g->slots) need |= sl.phase == NEWREQ || sl.phase == OUTER;
-        if (!need) continue;
+        if (!needs(g, NEWREQ, OUTER)) continue;
         any_outer = true;
+        hipStream_t st = g->st;
         set_masks(g);
         OPF_HIP(hipMemcpyAsync(g->dmask, g->hmask, sizeof(int) * M_COUNT * g->B, hipMemcpyHostToDevice, st));
         const int* mk = g->dmask;
@@ -290,19 +307,14 @@ int round(opf_srv* s, int* nfinished) {
             ph_add64<double>(g->H, st, mk + M_ADD * g->B, g->x64, (const double*)g->H.e);
             ph_resid64<double>(g->H, st, mk + M_OUTER * g->B, g->x64, g->b64, (double*)g->H.r, FIN_RR0);
         }
+        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost, st));
     }
     if (any_outer) {
+        for (auto& gp : s->groups)
+            if (needs(gp.get(), NEWREQ, OUTER)) OPF_HIP(hipStreamSynchronize(gp->st));
         for (auto& gp : s->groups) {
             Group* g = gp.get();
-            bool need = false;
-            for (auto& sl : g->slots) need |= sl.phase == NEWREQ || sl.phase == OUTER;
-            if (need)
-                OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B,
-                                       hipMemcpyDeviceToHost, st));
-        }
-        OPF_HIP(hipStreamSynchronize(st));
-        for (auto& gp : s->groups) {
-            Group* g = gp.get();
+            hipStream_t st = g->st;
             bool start = false;
             for (int k = 0; k < g->B; ++k) {
                 Slot& sl = g->slots[k];
@@ -315,7 +327,7 @@ int round(opf_srv* s, int* nfinished) {
                 sl.rr0 = sc[S_RR0];
                 sl.pending_add = false;
                 if (!(sl.rr0 > sl.target) || sl.bb == 0 || sl.its >= maxit || sl.nref >= 40) {
-                    // finished: report, copy out, free the slot
+                    // finished: report, copy out (completes before the result is polled), free the slot
                     opf_srv_result r{};
                     r.id = sl.id;
                     r.status = OPF_OK;
@@ -331,6 +343,7 @@ int round(opf_srv* s, int* nfinished) {
                                                st));
                     r.t_done = s->now();
                     s->done.push_back(r);
+                    s->done_streams.push_back(sl.out_x ? st : nullptr);
                     sl.phase = FREE;
                     --s->inflight;
                     ++*nfinished;
@@ -356,41 +369,42 @@ int round(opf_srv* s, int* nfinished) {
                 }
             }
         }
-        OPF_HIP(hipStreamSynchronize(st));  // out_x copies complete before results are polled
     }
-    // ---- batched inner iterations (graph replay)
+    // ---- batched inner iterations (graph replays), all groups enqueued before any wait
     bool any_iter = false;
     for (auto& gp : s->groups) {
         Group* g = gp.get();
-        bool need = false;
-        for (auto& sl : g->slots) need |= sl.phase == ITER;
-        if (!need) continue;
+        if (!needs(g, ITER, ITER)) continue;
         any_iter = true;
+        hipStream_t st = g->st;
         set_masks(g);
         OPF_HIP(hipMemcpyAsync(g->dmask + M_ITER * g->B, g->hmask + M_ITER * g->B, sizeof(int) * g->B,
                                hipMemcpyHostToDevice, st));
-        if (s->cfg.batched) {
-            OPF_TRY(capture(s, g));
-            OPF_HIP(hipGraphLaunch(g->exec, st));
-        } else {
-            const int* m = g->dmask + M_ITER * g->B;
-            for (int pc = 0; pc < 2; ++pc) {
-                if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
-                else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
+        for (int r = 0; r < g->replays; ++r) {
+            if (s->cfg.batched) {
+                OPF_TRY(capture(s, g));
+                OPF_HIP(hipGraphLaunch(g->exec, st));
+            } else {
+                const int* m = g->dmask + M_ITER * g->B;
+                for (int pc = 0; pc < 2; ++pc) {
+                    if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
+                    else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
+                }
             }
         }
         OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost,
                                st));
     }
     if (any_iter) {
-        OPF_HIP(hipStreamSynchronize(st));
+        for (auto& gp : s->groups)
+            if (needs(gp.get(), ITER, ITER)) OPF_HIP(hipStreamSynchronize(gp->st));
         for (auto& gp : s->groups) {
             Group* g = gp.get();
             for (int k = 0; k < g->B; ++k) {
                 Slot& sl = g->slots[k];
                 if (sl.phase != ITER) continue;
-                sl.its += 2;
-                sl.inner_its += 2;
+                sl.its += 2 * g->replays;
+                sl.inner_its += 2 * g->replays;
                 double rr = g->H.hscal[(size_t)k * GMG_SSTRIDE + S_RR];
                 // FP32 inner solve: reduce the residual by ~1e-4 per refinement (or reach the target)
                 double inner = std::max(1e-4, std::sqrt(sl.target / sl.rr0) * 0.5);
@@ -454,8 +468,12 @@ extern "C" int opf_srv_step(opf_srv* s, int* nfinished) {
 extern "C" int opf_srv_poll(opf_srv* s, opf_srv_result* out, int max, int* n) {
     OPF_CHECK(s && n && (out || max == 0), OPF_ERR_ARG, "opf_srv_poll: null arg");
     int k = std::min<int>(max, (int)s->done.size());
-    for (int q = 0; q < k; ++q) out[q] = s->done[q];
+    for (int q = 0; q < k; ++q) {
+        if (s->done_streams[q]) OPF_HIP(hipStreamSynchronize(s->done_streams[q]));  // its out_x copy
+        out[q] = s->done[q];
+    }
     s->done.erase(s->done.begin(), s->done.begin() + k);
+    s->done_streams.erase(s->done_streams.begin(), s->done_streams.begin() + k);
     *n = k;
     return OPF_OK;
 }
@@ -471,7 +489,6 @@ extern "C" double opf_srv_clock(opf_srv* s) { return s ? s->now() : 0.0; }
 
 extern "C" int opf_srv_destroy(opf_srv* s) {
     if (!s) return OPF_OK;
-    hipStreamSynchronize(s->ctx->stream);
     for (auto& g : s->groups) free_group(g.get());
     delete s;
     return OPF_OK;

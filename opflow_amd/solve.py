@@ -93,6 +93,18 @@ class _Handler:
         L.check(self.lib.opf_eqn_solve(self.h, C.byref(st)))
         return EqnSolveState(st.niter, st.relerr, st.abserr, st.nrefine, bool(st.converged))
 
+    def csr(self):
+        """(ptr, col, val, rhs) of the system as CSRMatrixGenerator builds it (pin last when pinValue)."""
+        nr, nz = C.c_longlong(), C.c_longlong()
+        L.check(self.lib.opf_eqn_csr(self.h, C.byref(nr), C.byref(nz), None, None, None, None))
+        ptr = np.zeros(nr.value + 1, dtype=np.int64)
+        col = np.zeros(nz.value, dtype=np.int64)
+        val = np.zeros(nz.value)
+        rhs = np.zeros(nr.value)
+        L.check(self.lib.opf_eqn_csr(self.h, C.byref(nr), C.byref(nz), ptr.ctypes.data, col.ctypes.data,
+                                     val.ctypes.data, rhs.ctypes.data))
+        return ptr, col, val, rhs
+
     def close(self):
         if self.h:
             self.lib.opf_eqn_destroy(self.h)

@@ -123,5 +123,33 @@ int main() {
         auto st = Solve(c.poisson_eqn(), c.p, params);
         report("DircEqnTest.MixedPrecision", c.max_rel_err(), 1e-8, st);
     }
+    {  // CSRMatrixGeneratorTest.cpp:57-89 (SimplePoisson): the system in CSR form
+        using Mesh = CartesianMesh<Meta::int_<2>>;
+        using Field = CartesianField<Real, Mesh>;
+        auto m = MeshBuilder<Mesh>().newMesh(5, 5).setMeshOfDim(0, 0., 4.).setMeshOfDim(1, 0., 4.).build();
+        auto p = ExprBuilder<Field>()
+                         .setName("p")
+                         .setMesh(m)
+                         .setBC(0, DimPos::start, BCType::Dirc, 0.)
+                         .setBC(0, DimPos::end, BCType::Dirc, 0.)
+                         .setBC(1, DimPos::start, BCType::Dirc, 0.)
+                         .setBC(1, DimPos::end, BCType::Dirc, 0.)
+                         .setLoc({LocOnMesh::Center, LocOnMesh::Center})
+                         .setExt(1)
+                         .build();
+        HIPGMGEqnSolveHandler<std::function<Equation(ExprH)>, Field,
+                              SolverBox<StructSolverType::GMRES, StructSolverType::None>>
+                direct([](ExprH e) { return 1.0 == d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e); },
+                       p, SolverBox<StructSolverType::GMRES, StructSolverType::None>{});
+        auto mat = direct.generateCSR();
+        // golden: row 0 = {0: 6, 1: -1, 4: -1}, row 5 = {1,4,5,6,9} with diagonal 4, nnz 64, rhs = -1
+        bool ok = mat.nnz() == 64 && mat.row.size() == 17 && mat.row[1] == 3 && mat.col[0] == 0 && mat.col[1] == 1 &&
+                  mat.col[2] == 4 && mat.val[0] == 6.0 && mat.val[1] == -1.0 && mat.val[2] == -1.0;
+        long long r5 = mat.row[5];
+        ok = ok && mat.row[6] - r5 == 5 && mat.col[r5 + 2] == 5 && mat.val[r5 + 2] == 4.0;
+        for (double v : mat.rhs) ok = ok && v == -1.0;
+        std::printf("%s CSRMatrixGeneratorTest.SimplePoisson: nnz %d\n", ok ? "PASS" : "FAIL", mat.nnz());
+        failures += !ok;
+    }
     return failures ? 1 : 0;
 }

@@ -1,8 +1,10 @@
-// CPU check of opf::FastDiv against the hardware divide: every divisor up to
-// 4096 over edge and random dividends, plus large and power-of-two divisors.
+// CPU check of opf::FastDiv against the hardware divide (every divisor up to
+// 4096 over edge and random dividends, plus large and power-of-two divisors)
+// and of the XCD block mapping (a bijection with contiguous per-XCD ranges).
 #include <cstdio>
 #include <cstdint>
 #include <random>
+#include <vector>
 
 #include "fastdiv.hpp"
 
@@ -40,6 +42,23 @@ int main() {
         test(d, (uint32_t)rng());
         test(d, d);
         test(d, d - 1);
+    }
+    // xcd_block is a bijection of [0, G) whose XCD ranges are contiguous
+    for (uint32_t G = 1; G <= 5000; ++G) {
+        std::vector<int> seen(G, 0);
+        for (uint32_t b = 0; b < G; ++b) {
+            uint32_t l = opf::xcd_block(b, G);
+            ++checks;
+            if (l >= G || seen[l]++) {
+                if (bad < 10) printf("FAIL xcd_block G=%u b=%u -> %u\n", G, b, l);
+                ++bad;
+            }
+            // consecutive hardware blocks of one XCD get consecutive logical blocks
+            if (b >= 8 && opf::xcd_block(b, G) != opf::xcd_block(b - 8, G) + 1) {
+                if (bad < 10) printf("FAIL xcd_block contiguity G=%u b=%u\n", G, b);
+                ++bad;
+            }
+        }
     }
     printf("%lld checks, %lld failures\n", checks, bad);
     if (bad) return 1;

@@ -113,3 +113,24 @@ def test_simple_poisson_golden(ctx):
     assert a[1, 1] == pytest.approx(-0.375, abs=1e-12)
     assert a[2, 2] == pytest.approx(-1.125, abs=1e-12)
     assert a[0, 1] == pytest.approx(0.375, abs=1e-12)  # ghost = -mirror after updatePadding
+
+
+@pytest.mark.parametrize("name,bc,pin", [("SimplePoisson", BC.Dirc, False), ("SimplePoisson_Neum", BC.Neum, True),
+                                         ("SimplePoisson_Periodic", BC.Periodic, True)])
+def test_device_csr_matches_golden(ctx, name, bc, pin):
+    """CSRMatrixGeneratorTest.cpp:57-161: the device-extracted system in CSRMatrixGenerator's layout
+    (pin last) equals the reference's golden ptr / col / val / rhs arrays exactly."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "csr_simple_poisson.json")))["cases"][name]
+    mesh = of.MeshBuilder(ctx).newMesh(5, 5).setMeshOfDim(0, 0., 4.).setMeshOfDim(1, 0., 4.).build()
+    p = _field(ctx, mesh, bc, 0.0, "p")
+    h = of.makeEqnSolveHandler(
+        lambda e: 1.0 == of.d2x(of.D2SecondOrderCentered, e) + of.d2y(of.D2SecondOrderCentered, e), p,
+        of.PrecondStructSolver(of.StructSolverParams(pinValue=pin)))
+    ptr, col, val, rhs = h.csr()
+    h.close()
+    np.testing.assert_array_equal(ptr, gold["ptr"])
+    np.testing.assert_array_equal(col, gold["col"])
+    np.testing.assert_array_equal(val, gold["val"])
+    np.testing.assert_array_equal(rhs, gold["rhs"])
