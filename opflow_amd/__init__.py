@@ -226,8 +226,13 @@ class Expr:
 
 
 class Equation:
+    """lhs == rhs (Equation/Equation.hpp:27-55). Python evaluates `1.0 == expr`
+    as `expr == 1.0`, which swaps the sides; solves are unaffected (the operator's
+    orientation is normalised) but the CSR form of a system is lhs - rhs, so
+    build `Equation(1.0, expr)` to keep a constant on the left."""
+
     def __init__(self, lhs, rhs):
-        self.lhs, self.rhs = lhs, rhs
+        self.lhs, self.rhs = _wrap(lhs), _wrap(rhs)
 
 
 def _node_array(nodes):

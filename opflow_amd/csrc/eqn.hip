@@ -531,7 +531,11 @@ extern "C" int opf_eqn_csr(opf_eqn* e, long long* nrows, long long* nnz, long lo
                 for (int sd = 0; sd < 2; ++sd) {
                     const double a = c[(size_t)(1 + 2 * d + sd) * N + r];
                     const bool edge = sd == 0 ? p[d] == 0 : p[d] == e->n[d] - 1;
-                    if (edge && !e->per[d]) continue;  // folded into the diagonal and bias by the BC
+                    if (edge && !e->per[d]) {
+                        // the slot holds the ghost, alpha * (this cell) + beta; beta is in the bias already
+                        ent[0].second += e->alpha[d][sd] * a;
+                        continue;
+                    }
                     long long q = r + (sd == 0 ? -st[d] : st[d]);
                     if (edge) q = r + (sd == 0 ? 1 : -1) * (long long)(e->n[d] - 1) * st[d];
                     bool merged = false;  // a periodic extent of 1 or 2 meets the same cell twice

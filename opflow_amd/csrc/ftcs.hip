@@ -336,7 +336,7 @@ struct Ftcs3dCfg<double> {
 };
 template <>
 struct Ftcs3dCfg<float> {
-    static constexpr int TX = 64, TY = 8, K = 2, NT = 256, PF = 3, ZC = 32;
+    static constexpr int TX = 64, TY = 8, K = 2, NT = 256, PF = 4, ZC = 64;
     static constexpr bool XCD = true;
 };
 

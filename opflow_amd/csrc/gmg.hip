@@ -197,6 +197,386 @@ __global__ __launch_bounds__(256) void k_jacobi(GOp<T> o, const T* __restrict__ 
     }
 }
 
+// Diagonal of the constant-coefficient operator at cell (i, j, k), accumulated
+// in exactly apply_op's order (so it has the same bits).
+template <typename T, int DIM>
+__device__ __forceinline__ T diag_const(const GOp<T>& o, int i, int j, int k) {
+    T D = o.sigma;
+    const int c[3] = {i, j, k};
+    const int n[3] = {o.n0, o.n1, o.n2};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (c[d] > 0 || o.per[d]) D += o.a[d];
+        else D += o.a[d] * o.bfac[d][0];
+        if (c[d] < n[d] - 1 || o.per[d]) D += o.a[d];
+        else D += o.a[d] * o.bfac[d][1];
+    }
+    return D;
+}
+
+// The first two pre-smoothing sweeps from x = 0 in one pass (constant
+// coefficients): x1 = w b / D at every cell, x2 = x1 + w (b - A x1) / D. x1 at
+// the neighbours is recomputed from b there instead of going through memory;
+// the arithmetic is that of k_jacobi<ZERO> followed by k_jacobi, so x2 has the
+// same bits as the two-launch sequence.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_jacobi2z(GOp<T> o, const T* __restrict__ b, T* __restrict__ xo, T w, Bat bt) {
+    BAT_SLOT(bt);
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
+    b += vo;
+    xo += vo;
+    const uint32_t t = cell_id();
+    if (t >= N) return;
+    int i, j, k;
+    cell_of(t, o.f0, o.f1, i, j, k);
+    const int c[3] = {i, j, k};
+    const int n[3] = {o.n0, o.n1, o.n2};
+    const uint32_t st[3] = {1u, o.s1, o.s2};
+    auto x1 = [&](uint32_t l, int d, int cd) {
+        int q[3] = {i, j, k};
+        q[d] = cd;
+        return w * b[l] / diag_const<T, DIM>(o, q[0], q[1], q[2]);
+    };
+    const T bc = b[t];
+    T D = o.sigma, acc = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (c[d] > 0) {
+            D += o.a[d];
+            acc += o.a[d] * x1(t - st[d], d, c[d] - 1);
+        } else if (o.per[d]) {
+            D += o.a[d];
+            acc += o.a[d] * x1(t + (uint32_t)(n[d] - 1) * st[d], d, n[d] - 1);
+        } else {
+            D += o.a[d] * o.bfac[d][0];
+        }
+        if (c[d] < n[d] - 1) {
+            D += o.a[d];
+            acc += o.a[d] * x1(t + st[d], d, c[d] + 1);
+        } else if (o.per[d]) {
+            D += o.a[d];
+            acc += o.a[d] * x1(t - (uint32_t)(n[d] - 1) * st[d], d, 0);
+        } else {
+            D += o.a[d] * o.bfac[d][1];
+        }
+    }
+    const T xc = w * bc / D;
+    const T ax = D * xc - acc;
+    xo[t] = xc + w * (bc - ax) / D;
+}
+
+// last post-smoothing sweep of the finest level fused with the dot products
+// the flexible CG needs next: out = xi + w (b - A xi)/D, then partials of
+// (b, out), (out, q) and sum(out) (b is the residual r the V-cycle smooths).
+// Same block ranges and per-thread order as k_dot3, so the sums are identical.
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_jacobi_dot3(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
+                                                     T* __restrict__ xo, T w, const T* __restrict__ q,
+                                                     double* __restrict__ part, Bat bt) {
+    BAT_SLOT(bt);
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
+    xi += vo;
+    b += vo;
+    xo += vo;
+    q += vo;
+    part += (long long)slot * bt.ps;
+    double sa = 0, sb = 0, sc = 0;
+    uint32_t tb, te;
+    block_range(N, tb, te);
+    for (uint32_t t = tb + threadIdx.x; t < te; t += blockDim.x) {
+        int i, j, k;
+        cell_of(t, o.f0, o.f1, i, j, k);
+        auto xv = [&](uint32_t l) { return xi[l]; };
+        T D;
+        const T xc = xi[t];
+        const T bv = b[t];
+        const T ax = apply_op<T, VAR, DIM>(o, i, j, k, t, xc, xv, D);
+        const T zv = xc + w * (bv - ax) / D;
+        xo[t] = zv;
+        const double zd = (double)zv;
+        sa += (double)bv * zd;
+        sb += zd * (double)q[t];
+        sc += zd;
+    }
+    sa = gmg_block_sum(sa);
+    sb = gmg_block_sum(sb);
+    sc = gmg_block_sum(sc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sa;
+        part[GMG_MAXB + blockIdx.x] = sb;
+        part[2 * GMG_MAXB + blockIdx.x] = sc;
+    }
+}
+
+// ---- constant-coefficient kernels over 64 x 4 tiles (3-D: marching in z)
+// A block of 64 x 4 threads owns a 64 x 4 column of the x-y plane and (3-D) walks ZC
+// planes in z, keeping the stencilled vector's values at k-1, k, k+1 in
+// registers (the z neighbours come for free). Cells with no boundary face take
+// a straight-line path (interior diagonal, omega / D precomputed); the others
+// take apply_op's general path. Launches are a fixed grid striding over
+// (tile, z chunk) items, so the partial sums of reductions are reproducible.
+struct ZTile {
+    int tx, ty, nz, zc;  // tiles in x, y, z chunks, planes per chunk
+    int items;
+};
+
+__device__ __forceinline__ void ztile_item(const ZTile& z, int item, int& i, int& j, int& k0, int& k1, int n0, int n1,
+                                           int n2) {
+    const int x = item % z.tx;
+    const int r = item / z.tx;
+    const int y = r % z.ty;
+    const int c = r / z.ty;
+    i = x * 64 + (int)(threadIdx.x & 63);
+    j = y * 4 + (int)(threadIdx.x >> 6);
+    k0 = c * z.zc;
+    k1 = min(k0 + z.zc, n2);
+    if (i >= n0 || j >= n1) k1 = k0;  // lanes past the grid edge do no cells
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ bool interior3(const GOp<T>& o, int i, int j, int k) {
+    return i > 0 && i < o.n0 - 1 && j > 0 && j < o.n1 - 1 && (DIM == 2 || (k > 0 && k < o.n2 - 1));
+}
+
+// (A v)_c for an interior cell: x/y neighbours through the accessor, z neighbours from the window
+template <typename T, int DIM, typename V>
+__device__ __forceinline__ T apply_in3(const GOp<T>& o, uint32_t t, T vc, T vm, T vp, V v) {
+    T acc = o.a[0] * v(t - 1);
+    acc += o.a[0] * v(t + 1);
+    acc += o.a[1] * v(t - o.s1);
+    acc += o.a[1] * v(t + o.s1);
+    if (DIM == 3) {
+        acc += o.a[2] * vm;
+        acc += o.a[2] * vp;
+    }
+    return o.dint * vc - acc;
+}
+
+// weighted Jacobi sweep xo = xi + w (b - A xi) / D
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void zk_jacobi(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
+                                                 T* __restrict__ xo, T w, ZTile zt, Bat bt) {
+    BAT_SLOT(bt);
+    xi += vo;
+    b += vo;
+    xo += vo;
+    auto xv = [&](uint32_t l) { return xi[l]; };
+    for (int item = (int)xcd_block(blockIdx.x, gridDim.x); item < zt.items; item += gridDim.x) {
+        int i, j, k0, k1;
+        ztile_item(zt, item, i, j, k0, k1, o.n0, o.n1, o.n2);
+        if (k0 >= k1) continue;
+        uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j + o.s2 * (uint32_t)k0;
+        T vm = (DIM == 3 && k0 > 0) ? xi[t - o.s2] : (T)0, vc = xi[t];
+        for (int k = k0; k < k1; ++k, t += o.s2) {
+            const T vp = (DIM == 3 && k + 1 < o.n2) ? xi[t + o.s2] : (T)0;
+            const T bv = b[t];
+            if (interior3<T, DIM>(o, i, j, k)) {
+                xo[t] = vc + o.wdi * (bv - apply_in3<T, DIM>(o, t, vc, vm, vp, xv));
+            } else {
+                T D;
+                const T ax = apply_op<T, false, DIM>(o, i, j, k, t, vc, xv, D);
+                xo[t] = vc + w * (bv - ax) / D;
+            }
+            vm = vc;
+            vc = vp;
+        }
+    }
+}
+
+// last sweep of the finest level + partials of (b, out), (out, q), sum(out)
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void zk_jacobi_dot3(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ b,
+                                                      T* __restrict__ xo, T w, const T* __restrict__ q,
+                                                      double* __restrict__ part, ZTile zt, Bat bt) {
+    BAT_SLOT(bt);
+    xi += vo;
+    b += vo;
+    xo += vo;
+    q += vo;
+    part += (long long)slot * bt.ps;
+    auto xv = [&](uint32_t l) { return xi[l]; };
+    double sa = 0, sb = 0, sc = 0;
+    for (int item = (int)xcd_block(blockIdx.x, gridDim.x); item < zt.items; item += gridDim.x) {
+        int i, j, k0, k1;
+        ztile_item(zt, item, i, j, k0, k1, o.n0, o.n1, o.n2);
+        if (k0 >= k1) continue;
+        uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j + o.s2 * (uint32_t)k0;
+        T vm = (DIM == 3 && k0 > 0) ? xi[t - o.s2] : (T)0, vc = xi[t];
+        for (int k = k0; k < k1; ++k, t += o.s2) {
+            const T vp = (DIM == 3 && k + 1 < o.n2) ? xi[t + o.s2] : (T)0;
+            const T bv = b[t];
+            T zv;
+            if (interior3<T, DIM>(o, i, j, k)) {
+                zv = vc + o.wdi * (bv - apply_in3<T, DIM>(o, t, vc, vm, vp, xv));
+            } else {
+                T D;
+                const T ax = apply_op<T, false, DIM>(o, i, j, k, t, vc, xv, D);
+                zv = vc + w * (bv - ax) / D;
+            }
+            xo[t] = zv;
+            const double zd = (double)zv;
+            sa += (double)bv * zd;
+            sb += zd * (double)q[t];
+            sc += zd;
+            vm = vc;
+            vc = vp;
+        }
+    }
+    sa = gmg_block_sum(sa);
+    sb = gmg_block_sum(sb);
+    sc = gmg_block_sum(sc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sa;
+        part[GMG_MAXB + blockIdx.x] = sb;
+        part[2 * GMG_MAXB + blockIdx.x] = sc;
+    }
+}
+
+// two sweeps from x = 0 (see k_jacobi2z); cells two or more cells away from
+// every boundary take the straight-line path with x1 = (omega / D) b
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void zk_jacobi2z(GOp<T> o, const T* __restrict__ b, T* __restrict__ xo, T w,
+                                                   ZTile zt, Bat bt) {
+    BAT_SLOT(bt);
+    b += vo;
+    xo += vo;
+    auto x1v = [&](uint32_t l) { return o.wdi * b[l]; };
+    for (int item = (int)xcd_block(blockIdx.x, gridDim.x); item < zt.items; item += gridDim.x) {
+        int i, j, k0, k1;
+        ztile_item(zt, item, i, j, k0, k1, o.n0, o.n1, o.n2);
+        if (k0 >= k1) continue;
+        uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j + o.s2 * (uint32_t)k0;
+        T bm = (DIM == 3 && k0 > 0) ? b[t - o.s2] : (T)0, bc = b[t];
+        const bool in2xy = i > 1 && i < o.n0 - 2 && j > 1 && j < o.n1 - 2;
+        for (int k = k0; k < k1; ++k, t += o.s2) {
+            const T bp = (DIM == 3 && k + 1 < o.n2) ? b[t + o.s2] : (T)0;
+            if (in2xy && (DIM == 2 || (k > 1 && k < o.n2 - 2))) {
+                const T xc = o.wdi * bc;
+                const T ax = apply_in3<T, DIM>(o, t, xc, o.wdi * bm, o.wdi * bp, x1v);
+                xo[t] = xc + o.wdi * (bc - ax);
+            } else {
+                const int c[3] = {i, j, k};
+                const int n[3] = {o.n0, o.n1, o.n2};
+                const uint32_t st[3] = {1u, o.s1, o.s2};
+                auto x1 = [&](uint32_t l, int d, int cd) {
+                    int qq[3] = {i, j, k};
+                    qq[d] = cd;
+                    return w * b[l] / diag_const<T, DIM>(o, qq[0], qq[1], qq[2]);
+                };
+                T D = o.sigma, acc = 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    if (c[d] > 0) {
+                        D += o.a[d];
+                        acc += o.a[d] * x1(t - st[d], d, c[d] - 1);
+                    } else if (o.per[d]) {
+                        D += o.a[d];
+                        acc += o.a[d] * x1(t + (uint32_t)(n[d] - 1) * st[d], d, n[d] - 1);
+                    } else {
+                        D += o.a[d] * o.bfac[d][0];
+                    }
+                    if (c[d] < n[d] - 1) {
+                        D += o.a[d];
+                        acc += o.a[d] * x1(t + st[d], d, c[d] + 1);
+                    } else if (o.per[d]) {
+                        D += o.a[d];
+                        acc += o.a[d] * x1(t - (uint32_t)(n[d] - 1) * st[d], d, 0);
+                    } else {
+                        D += o.a[d] * o.bfac[d][1];
+                    }
+                }
+                const T xc = w * bc / D;
+                xo[t] = xc + w * (bc - (D * xc - acc)) / D;
+            }
+            bm = bc;
+            bc = bp;
+        }
+    }
+}
+
+// p_new = (z - zmean) + beta p_old ; q = A p_new ; partial (p_new, q)
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void zk_spmv_p(GOp<T> o, const T* __restrict__ z, const T* __restrict__ pold,
+                                                 T* __restrict__ pnew, T* __restrict__ q,
+                                                 const double* __restrict__ s, double* __restrict__ part, ZTile zt,
+                                                 Bat bt) {
+    BAT_SLOT(bt);
+    z += vo;
+    pold += vo;
+    pnew += vo;
+    q += vo;
+    s += (long long)slot * bt.ss;
+    part += (long long)slot * bt.ps;
+    const T beta = (T)s[S_BETA];
+    const T zm = (T)s[S_ZMEAN];
+    auto pv = [&](uint32_t l) { return (z[l] - zm) + beta * pold[l]; };
+    double acc = 0;
+    for (int item = (int)xcd_block(blockIdx.x, gridDim.x); item < zt.items; item += gridDim.x) {
+        int i, j, k0, k1;
+        ztile_item(zt, item, i, j, k0, k1, o.n0, o.n1, o.n2);
+        if (k0 >= k1) continue;
+        uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j + o.s2 * (uint32_t)k0;
+        T vm = (DIM == 3 && k0 > 0) ? pv(t - o.s2) : (T)0, vc = pv(t);
+        for (int k = k0; k < k1; ++k, t += o.s2) {
+            const T vp = (DIM == 3 && k + 1 < o.n2) ? pv(t + o.s2) : (T)0;
+            T qc;
+            if (interior3<T, DIM>(o, i, j, k)) {
+                qc = apply_in3<T, DIM>(o, t, vc, vm, vp, pv);
+            } else {
+                T D;
+                qc = apply_op<T, false, DIM>(o, i, j, k, t, vc, pv, D);
+            }
+            pnew[t] = vc;
+            q[t] = qc;
+            acc += (double)vc * (double)qc;
+            vm = vc;
+            vc = vp;
+        }
+    }
+    acc = gmg_block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// r = b - A x (x may be null), partials of r^2
+template <typename T, typename TO, int DIM>
+__global__ __launch_bounds__(256) void zk_resid(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
+                                                TO* __restrict__ r, double* __restrict__ part, ZTile zt, Bat bt) {
+    BAT_SLOT(bt);
+    if (x) x += vo;
+    b += vo;
+    if (r) r += vo;
+    if (part) part += (long long)slot * bt.ps;
+    auto xv = [&](uint32_t l) { return x[l]; };
+    double acc = 0;
+    for (int item = (int)xcd_block(blockIdx.x, gridDim.x); item < zt.items; item += gridDim.x) {
+        int i, j, k0, k1;
+        ztile_item(zt, item, i, j, k0, k1, o.n0, o.n1, o.n2);
+        if (k0 >= k1) continue;
+        uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j + o.s2 * (uint32_t)k0;
+        T vm = (DIM == 3 && x && k0 > 0) ? x[t - o.s2] : (T)0, vc = x ? x[t] : (T)0;
+        for (int k = k0; k < k1; ++k, t += o.s2) {
+            const T vp = (DIM == 3 && x && k + 1 < o.n2) ? x[t + o.s2] : (T)0;
+            T v = b[t];
+            if (x) {
+                if (interior3<T, DIM>(o, i, j, k)) {
+                    v = v - apply_in3<T, DIM>(o, t, vc, vm, vp, xv);
+                } else {
+                    T D;
+                    v = v - apply_op<T, false, DIM>(o, i, j, k, t, vc, xv, D);
+                }
+            }
+            if (r) r[t] = (TO)v;
+            acc += (double)v * (double)v;
+            vm = vc;
+            vc = vp;
+        }
+    }
+    if (part) {
+        acc = gmg_block_sum(acc);
+        if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    }
+}
+
 // coarse rhs = average over the 2^d children of the fine residual b - A x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
@@ -214,6 +594,26 @@ __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict_
     auto xv = [&](uint32_t l) { return x[l]; };
     T s = 0;
     const int nz = DIM == 3 ? 2 : 1;
+    if (!VAR && I > 0 && I < c0 - 1 && J > 0 && J < c1 - 1 && (DIM == 2 || (K > 0 && K < c2 - 1))) {
+        // every child has six (four) neighbours inside the fine grid: straight-line residuals
+        for (int dk = 0; dk < nz; ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    const uint32_t f = (uint32_t)(2 * I + di) + o.s1 * (uint32_t)(2 * J + dj) +
+                                       o.s2 * (uint32_t)(DIM == 3 ? 2 * K + dk : 0);
+                    T acc = o.a[0] * x[f - 1];
+                    acc += o.a[0] * x[f + 1];
+                    acc += o.a[1] * x[f - o.s1];
+                    acc += o.a[1] * x[f + o.s1];
+                    if (DIM == 3) {
+                        acc += o.a[2] * x[f - o.s2];
+                        acc += o.a[2] * x[f + o.s2];
+                    }
+                    s += b[f] - (o.dint * x[f] - acc);
+                }
+        bc[t] = s * (T)(DIM == 3 ? 0.125 : 0.25);
+        return;
+    }
     for (int dk = 0; dk < nz; ++dk)
         for (int dj = 0; dj < 2; ++dj)
             for (int di = 0; di < 2; ++di) {
@@ -243,6 +643,24 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, const T* __
     for (int d = 0; d < 3; ++d) {
         par[d] = f[d] >> 1;
         oth[d] = (f[d] & 1) ? par[d] + 1 : par[d] - 1;
+    }
+    const bool inner = i > 0 && i < o.n0 - 1 && j > 0 && j < o.n1 - 1 && (DIM == 2 || (k > 0 && k < o.n2 - 1));
+    if (inner) {
+        // every coarse cell used is inside the coarse grid: fixed (bi/tri)linear weights
+        const uint32_t cs1 = (uint32_t)c0, cs2 = (uint32_t)c0 * (uint32_t)c1;
+        const uint32_t p0 = (uint32_t)par[0] + cs1 * (uint32_t)par[1] + cs2 * (uint32_t)par[2];
+        const int o0 = oth[0] - par[0], o1 = (oth[1] - par[1]) * (int)cs1;
+        if (DIM == 2) {
+            const T v = (T)0.5625 * xc[p0] + (T)0.1875 * (xc[p0 + o0] + xc[p0 + o1]) + (T)0.0625 * xc[p0 + o0 + o1];
+            xf[t] += v;
+        } else {
+            const int o2 = (oth[2] - par[2]) * (int)cs2;
+            const T v = (T)0.421875 * xc[p0] + (T)0.140625 * (xc[p0 + o0] + xc[p0 + o1] + xc[p0 + o2]) +
+                        (T)0.046875 * (xc[p0 + o0 + o1] + xc[p0 + o0 + o2] + xc[p0 + o1 + o2]) +
+                        (T)0.015625 * xc[p0 + o0 + o1 + o2];
+            xf[t] += v;
+        }
+        return;
     }
     T s = 0;
     const int ncomb = 1 << DIM;
@@ -293,13 +711,25 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int mode, doub
     BAT_SLOT(bt);
     part += (long long)slot * bt.ps;
     s += (long long)slot * bt.ss;
-    double v[3];
-    for (int q = 0; q < 3; ++q) {
-        double acc = 0;
-        for (int t = threadIdx.x; t < nb; t += blockDim.x) acc += part[q * GMG_MAXB + t];
-        v[q] = gmg_block_sum(acc);
-        if (mode != FIN_DOT3 && mode != FIN_DOT3_FIRST) break;
+    // up to 16384 partials per value: independent loads, four accumulators per
+    // value per thread (a fixed order: the sums are reproducible)
+    const int nv = (mode == FIN_DOT3 || mode == FIN_DOT3_FIRST) ? 3 : 1;
+    double v[3] = {0, 0, 0};
+    for (int q = 0; q < nv; ++q) {
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        const double* pq = part + q * GMG_MAXB;
+        const int step = blockDim.x;
+        int t = threadIdx.x;
+        for (; t + 3 * step < nb; t += 4 * step) {
+            a0 += pq[t];
+            a1 += pq[t + step];
+            a2 += pq[t + 2 * step];
+            a3 += pq[t + 3 * step];
+        }
+        for (; t < nb; t += step) a0 += pq[t];
+        v[q] = (a0 + a1) + (a2 + a3);
     }
+    for (int q = 0; q < nv; ++q) v[q] = gmg_block_sum(v[q]);
     if (threadIdx.x != 0) return;
     switch (mode) {
         case FIN_PQ:
@@ -456,7 +886,27 @@ __global__ void k_sum(long long N, const T* __restrict__ x, double* __restrict__
 
 // ------------------------------------------------------------------ host side
 static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
-static inline int rblk(long long n) { return (int)std::min<long long>(GMG_MAXB, (n + 255) / 256); }
+static inline int rblk(long long n) { return (int)std::min<long long>(GMG_RBLK, (n + 255) / 256); }
+
+// z-marching kernels for the big 3-D constant-coefficient levels
+static bool use_z(const Hierarchy& H, int l) {
+    const GLevelHost& L = H.host[l];
+    return !H.var && L.n[0] >= 64 && L.N >= (H.dim == 3 ? (1 << 18) : (1 << 16));
+}
+static ZTile ztile(const GLevelHost& L) {
+    ZTile z;
+    z.zc = L.n[2] == 1 ? 1 : L.n[2] >= 256 ? 8 : 4;
+    z.tx = (L.n[0] + 63) / 64;
+    z.ty = (L.n[1] + 3) / 4;
+    z.nz = (L.n[2] + z.zc - 1) / z.zc;
+    z.items = z.tx * z.ty * z.nz;
+    return z;
+}
+// blocks (= partial sums per value) of every finest-level reduction launch
+static int red_count(const Hierarchy& H) {
+    if (use_z(H, 0)) return std::min(GMG_MAXB, ztile(H.host[0]).items);
+    return rblk(H.host[0].N);
+}
 
 template <typename T>
 static GOp<T> make_op(const Hierarchy& H, int l) {
@@ -480,6 +930,14 @@ static GOp<T> make_op(const Hierarchy& H, int l) {
     }
     o.sigma = (T)L.sigma;
     o.sg = std::is_same<T, double>::value ? (const T*)D.sg64 : (const T*)D.sg32;
+    // interior diagonal, accumulated in precision T in the order the kernels use
+    T di = o.sigma;
+    for (int d = 0; d < H.dim; ++d) {
+        di += o.a[d];
+        di += o.a[d];
+    }
+    o.dint = di;
+    o.wdi = (T)H.omega / di;
     return o;
 }
 
@@ -752,13 +1210,17 @@ static inline Bat bat(const Hierarchy& H, const int* mask, long long vs) {
 }
 
 template <typename T>
-void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out) {
+void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out,
+               const T* dot_q = nullptr) {
     const GLevelDev& D = H.dev[l];
     const GLevelHost& L = H.host[l];
     const long long N = L.N;
     const unsigned B = (unsigned)H.B;
     if (l == (int)H.dev.size() - 1) {
         hipLaunchKernelGGL(k_dense<T>, dim3(1, B), dim3(256), 0, st, H.cinv, H.cN, rhs, out, bat(H, mask, N));
+        if (dot_q)  // a one-level hierarchy: the dot products follow the dense solve
+            hipLaunchKernelGGL(k_dot3<T>, dim3(red_count(H), B), dim3(256), 0, st, N, rhs, (const T*)out, dot_q,
+                               H.part, bat(H, mask, N));
         return;
     }
     GOp<T> o = make_op<T>(H, l);
@@ -769,11 +1231,33 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     T* tmp = (T*)D.t;
     if (((H.pre - 1) + H.post) & 1) std::swap(cur, tmp);
     const Bat bt = bat(H, mask, N);
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, true>), dim3(nblk(N), B), dim3(256), 0, st, o,
-                                                  (const T*)nullptr, rhs, cur, w, bt));
-    for (int s = 1; s < H.pre; ++s) {
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256), 0,
-                                                      st, o, (const T*)cur, rhs, tmp, w, bt));
+    const bool zm = use_z(H, l);
+    const ZTile zt = ztile(L);
+    int s0 = 1;
+    if (!H.var && H.pre >= 2) {
+        // sweeps 1 and 2 from zero in one pass, landing where the second sweep of
+        // the two-launch sequence would (the buffer after `cur`)
+        if (zm && H.dim == 3)
+            hipLaunchKernelGGL((zk_jacobi2z<T, 3>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
+        else if (zm)
+            hipLaunchKernelGGL((zk_jacobi2z<T, 2>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
+        else if (H.dim == 3)
+            hipLaunchKernelGGL((k_jacobi2z<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, o, rhs, tmp, w, bt);
+        else
+            hipLaunchKernelGGL((k_jacobi2z<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, o, rhs, tmp, w, bt);
+        std::swap(cur, tmp);
+        s0 = 2;
+    } else {
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, true>), dim3(nblk(N), B), dim3(256), 0, st,
+                                                      o, (const T*)nullptr, rhs, cur, w, bt));
+    }
+    for (int s = s0; s < H.pre; ++s) {
+        if (zm)
+            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
+                                                          (const T*)cur, rhs, tmp, w, zt, bt));
+        else
+            GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256),
+                                                          0, st, o, (const T*)cur, rhs, tmp, w, bt));
         std::swap(cur, tmp);
     }
     const GLevelDev& C = H.dev[l + 1];
@@ -788,16 +1272,30 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
                            C.n[2], o, bt);
     for (int s = 0; s < H.post; ++s) {
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256), 0,
-                                                      st, o, (const T*)cur, rhs, tmp, w, bt));
+        if (dot_q && s == H.post - 1) {
+            // last sweep also forms the partials of (r, z), (z, q), sum(z) (finest level only)
+            const int G = red_count(H);
+            if (zm)
+                GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi_dot3<T, D_>), dim3(G, B), dim3(256), 0, st, o,
+                                                              (const T*)cur, rhs, tmp, w, dot_q, H.part, zt, bt));
+            else
+                GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi_dot3<T, V_, D_>), dim3(G, B), dim3(256), 0,
+                                                              st, o, (const T*)cur, rhs, tmp, w, dot_q, H.part, bt));
+        } else if (zm) {
+            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
+                                                          (const T*)cur, rhs, tmp, w, zt, bt));
+        } else {
+            GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256),
+                                                          0, st, o, (const T*)cur, rhs, tmp, w, bt));
+        }
         std::swap(cur, tmp);
     }
 }
 
 static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular) {
     const long long N = H.host[0].N;
-    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(256), 0, st, H.part, rblk(N), mode, H.scal, 1.0 / N,
-                       singular, bat(H, mask, N));
+    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, red_count(H), mode, H.scal,
+                       1.0 / N, singular, bat(H, mask, N));
 }
 
 // start of an inner FCG solve on r: p_old = 0, z = M r, rz = (r, z)
@@ -807,9 +1305,14 @@ void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask) {
     const unsigned B = (unsigned)H.B;
     const Bat bt = bat(H, mask, N);
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N), B), dim3(256), 0, st, N, (T*)H.p[0], (T)0, bt);
-    ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z);
-    hipLaunchKernelGGL(k_dot3<T>, dim3(rblk(N), B), dim3(256), 0, st, N, (const T*)H.r, (const T*)H.z,
-                       (const T*)H.z, H.part, bt);
+    if (H.post > 0) {
+        // (z, q) is not used by the first finish: any vector other than z serves as q
+        ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z, (const T*)H.r);
+    } else {
+        ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z);
+        hipLaunchKernelGGL(k_dot3<T>, dim3(red_count(H), B), dim3(256), 0, st, N, (const T*)H.r, (const T*)H.z,
+                           (const T*)H.z, H.part, bt);
+    }
     ph_finish(H, st, mask, FIN_DOT3_FIRST, H.singular);
 }
 
@@ -818,7 +1321,7 @@ template <typename T>
 void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int pc) {
     const long long N = H.host[0].N;
     const unsigned B = (unsigned)H.B;
-    const int G = rblk(N);
+    const int G = red_count(H);
     const Bat bt = bat(H, mask, N);
     GOp<T> o = make_op<T>(H, 0);
     T* z = (T*)H.z;
@@ -826,14 +1329,23 @@ void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int 
     T* r = (T*)H.r;
     T* pold = (T*)H.p[pc];
     T* pnew = (T*)H.p[1 - pc];
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o, (const T*)z,
-                                                  (const T*)pold, pnew, q, H.scal, H.part, bt));
+    if (use_z(H, 0))
+        GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_spmv_p<T, D_>), dim3(G, B), dim3(256), 0, st, o, (const T*)z,
+                                                      (const T*)pold, pnew, q, H.scal, H.part, ztile(H.host[0]), bt));
+    else
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o,
+                                                      (const T*)z, (const T*)pold, pnew, q, H.scal, H.part, bt));
     ph_finish(H, st, mask, FIN_PQ, H.singular);
     hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
                        bt);
     ph_finish(H, st, mask, FIN_RR, H.singular);
-    ph_vcycle<T>(H, st, mask, 0, (const T*)r, z);
-    hipLaunchKernelGGL(k_dot3<T>, dim3(G, B), dim3(256), 0, st, N, (const T*)r, (const T*)z, (const T*)q, H.part, bt);
+    if (H.post > 0) {
+        ph_vcycle<T>(H, st, mask, 0, (const T*)r, z, (const T*)q);  // last sweep forms the dot products
+    } else {
+        ph_vcycle<T>(H, st, mask, 0, (const T*)r, z);
+        hipLaunchKernelGGL(k_dot3<T>, dim3(G, B), dim3(256), 0, st, N, (const T*)r, (const T*)z, (const T*)q, H.part,
+                           bt);
+    }
     ph_finish(H, st, mask, FIN_DOT3, H.singular);
 }
 
@@ -843,8 +1355,13 @@ void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const doubl
                 int mode) {
     GOp<double> o = make_op<double>(H, 0);
     const long long N = H.host[0].N;
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(rblk(N), (unsigned)H.B),
-                                                  dim3(256), 0, st, o, x, b, r, H.part, bat(H, mask, N)));
+    if (use_z(H, 0))
+        GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_resid<double, TO, D_>), dim3(red_count(H), (unsigned)H.B),
+                                                      dim3(256), 0, st, o, x, b, r, H.part, ztile(H.host[0]),
+                                                      bat(H, mask, N)));
+    else
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(red_count(H), (unsigned)H.B),
+                                                      dim3(256), 0, st, o, x, b, r, H.part, bat(H, mask, N)));
     ph_finish(H, st, mask, mode, H.singular);
 }
 
@@ -852,7 +1369,7 @@ template <typename T>
 void ph_project(const Hierarchy& H, hipStream_t st, const int* mask, T* r) {
     const long long N = H.host[0].N;
     const Bat bt = bat(H, mask, N);
-    hipLaunchKernelGGL(k_sum<T>, dim3(rblk(N), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
+    hipLaunchKernelGGL(k_sum<T>, dim3(red_count(H), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
     ph_finish(H, st, mask, FIN_RR0, 1);
     hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, r, H.scal, (int)S_RR0,
                        1.0 / N, bt);
@@ -943,8 +1460,9 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
             ph_zero<float>(H, s, nullptr, e);
             int its = 0;
             double rrf = 0;
-            // reduce the residual by ~1e-4 per refinement (FP32 resolves ~1e-6)
-            double inner = std::max(1e-4, std::sqrt(target / rr0) * 0.5);
+            // reduce the residual by what is still needed (with a margin), at most by
+            // 1e-5 per refinement: the FP32 inner solve resolves ~1e-6 of its right-hand side
+            double inner = std::max(1e-5, std::sqrt(target / rr0) * 0.5);
             OPF_TRY(fcg<float>(H, s, e, rr0, std::min(inner, 0.5), std::max(1, maxit - total), &its, &rrf));
             total += its;
             ph_add64<float>(H, s, nullptr, x, (const float*)e);

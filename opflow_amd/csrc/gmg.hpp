@@ -17,7 +17,8 @@
 
 namespace opf {
 
-constexpr int GMG_MAXB = 2048;  // max blocks of a reduction kernel (partials per value)
+constexpr int GMG_MAXB = 16384;  // max blocks of a reduction kernel (partials per value)
+constexpr int GMG_RBLK = 2048;   // blocks of a one-dimensional (streaming) reduction launch
 constexpr int GMG_SSTRIDE = 16;            // scalars per slot
 constexpr int GMG_PSTRIDE = 4 * GMG_MAXB;  // reduction partials per slot
 
@@ -34,6 +35,8 @@ struct GOp {
     T bfac[3][2];    // boundary face factor (1 - alpha) per dim/side (non-periodic)
     const T* af[3];  // per-face coefficients, face array dims n + e_d (variable mode)
     const T* sg;     // per-cell sigma (variable mode)
+    T dint;          // diagonal of a cell with no boundary face (const mode)
+    T wdi;           // omega / dint: the smoother's interior scaling (const mode)
 };
 
 // Host description of one level (double master copy of the coefficients).
@@ -81,7 +84,7 @@ struct Hierarchy {
     void* q = nullptr;
     void* e = nullptr;      // inner solution (mixed)
     double* r64 = nullptr;  // outer residual (mixed) / work
-    double* part = nullptr; // reduction partials [8][GMG_MAXB]
+    double* part = nullptr; // reduction partials [4][GMG_MAXB] per slot
     double* scal = nullptr; // device scalars
     double* hscal = nullptr;  // pinned host mirror
     size_t bytes = 0;

@@ -12,10 +12,12 @@
 // Each slot runs FP32-inner / FP64-refinement: an FP64 residual r = b - A x,
 // an FP32 flexible-CG inner solve on A e = r, x += e. One scheduling round
 //   1. admits queued requests into free slots while the memory budget allows,
-//   2. runs the FP64 outer step for the slots that need it (batched),
+//   2. runs the FP64 outer step for the slots that need it (batched) and, on
+//      the device, starts an inner solve for every slot still above tolerance,
 //   3. replays the group's captured hipGraph of two batched inner iterations
-//      (1-3 times by problem size),
-//   4. reads every slot's scalars with one copy and advances the state machines.
+//      (1-3 times by problem size) for iterating and just-started slots,
+//   4. reads every slot's scalars with one copy per group, after one wait for
+//      all groups, and advances the state machines.
 // Requests that arrive while others run are admitted in the next round, so
 // their work overlaps on the device with requests already in flight.
 #include <algorithm>
@@ -33,7 +35,7 @@ using namespace opf;
 namespace {
 
 enum Phase { FREE = 0, NEWREQ, OUTER, ITER };
-enum { M_NEW = 0, M_ADD, M_OUTER, M_START, M_ITER, M_COUNT };
+enum { M_NEW = 0, M_ADD, M_OUTER, M_ALLOW, M_START, M_ITER, M_COUNT };
 
 struct Pending {
     uint64_t id;
@@ -127,6 +129,8 @@ void free_group(Group* g) {
 // request's iteration sequence, and its result, do not depend on other requests)
 int replays_for(long long N) { return N <= (1 << 16) ? 3 : N <= (1 << 20) ? 2 : 1; }
 
+int capture(opf_srv* s, Group* g);
+
 int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     int per[3] = {0, 0, 0};
     double alpha[3][2];
@@ -173,6 +177,14 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     }
     g->slots.assign(g->B, Slot{});
     g->bytes = g->H.bytes + 16 * BN;
+    {
+        // capture the group's iteration graph now, while its stream is idle
+        int rc2 = capture(s, g.get());
+        if (rc2 != OPF_OK) {
+            free_group(g.get());
+            return rc2;
+        }
+    }
     *out = g.get();
     s->used += g->bytes;
     s->groups.push_back(std::move(g));
@@ -261,7 +273,7 @@ int try_admit(opf_srv* s) {
     return OPF_OK;
 }
 
-void set_masks(Group* g) {
+void set_masks(Group* g, int maxit) {
     for (int m = 0; m < M_COUNT; ++m)
         for (int k = 0; k < g->B; ++k) g->hmask[m * g->B + k] = 0;
     for (int k = 0; k < g->B; ++k) {
@@ -274,59 +286,105 @@ void set_masks(Group* g) {
             g->hmask[M_OUTER * g->B + k] = 1;
             if (sl.pending_add) g->hmask[M_ADD * g->B + k] = 1;
         }
+        if (sl.phase == NEWREQ || sl.phase == OUTER) g->hmask[M_ALLOW * g->B + k] = sl.its < maxit && sl.nref < 40;
         if (sl.phase == ITER) g->hmask[M_ITER * g->B + k] = 1;
     }
+}
+
+// The outer step's decision for every slot of a group, on the device (so the
+// inner solve can start in the same round): a slot whose FP64 residual is still
+// above tol^2 * ||b||^2 (and that may iterate) starts an inner solve and joins
+// this round's iterations. The host repeats the same test on the same scalars.
+__global__ void k_decide(int B, const int* __restrict__ m_outer, const int* __restrict__ m_allow,
+                         int* __restrict__ m_start, int* __restrict__ m_iter, const double* __restrict__ scal,
+                         double tol2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    const double* sc = scal + (size_t)k * GMG_SSTRIDE;
+    int st = 0;
+    if (m_outer[k] && m_allow[k]) {
+        const double bb = sc[S_BB];
+        st = bb > 0 && sc[S_RR0] > tol2 * bb;
+    }
+    m_start[k] = st;
+    if (st) m_iter[k] = 1;
 }
 
 int round(opf_srv* s, int* nfinished) {
     *nfinished = 0;
     OPF_TRY(try_admit(s));
     const double tol = s->cfg.tol > 0 ? s->cfg.tol : 1e-8;
+    const double tol2 = tol * tol;
     const int maxit = s->cfg.max_iter > 0 ? s->cfg.max_iter : 200;
-    auto needs = [](const Group* g, int a, int b) {
-        for (auto& sl : g->slots)
-            if (sl.phase == a || sl.phase == b) return true;
-        return false;
-    };
-    // ---- outer FP64 step for NEW / OUTER slots: enqueue every group's work on
-    // its own stream first, then wait for them all
-    bool any_outer = false;
+    // ---- enqueue one round for every group with work, each on its own stream:
+    // FP64 outer step (NEW / OUTER slots) -> device decision -> inner starts ->
+    // graph replays for iterating and just-started slots -> scalars to the host
+    std::vector<Group*> active;
     for (auto& gp : s->groups) {
         Group* g = gp.get();
-        if (!needs(g, NEWREQ, OUTER)) continue;
-        any_outer = true;
-        hipStream_t st = g->st;
-        set_masks(g);
-        OPF_HIP(hipMemcpyAsync(g->dmask, g->hmask, sizeof(int) * M_COUNT * g->B, hipMemcpyHostToDevice, st));
-        const int* mk = g->dmask;
-        ph_resid64<double>(g->H, st, mk + M_NEW * g->B, nullptr, g->b64, (double*)nullptr, FIN_BB);
-        if (g->H.precision == OPF_PREC_MIXED) {
-            ph_add64<float>(g->H, st, mk + M_ADD * g->B, g->x64, (const float*)g->H.e);
-            ph_resid64<float>(g->H, st, mk + M_OUTER * g->B, g->x64, g->b64, (float*)g->H.r, FIN_RR0);
-        } else {
-            ph_add64<double>(g->H, st, mk + M_ADD * g->B, g->x64, (const double*)g->H.e);
-            ph_resid64<double>(g->H, st, mk + M_OUTER * g->B, g->x64, g->b64, (double*)g->H.r, FIN_RR0);
+        bool outer = false, iter = false;
+        for (auto& sl : g->slots) {
+            outer |= sl.phase == NEWREQ || sl.phase == OUTER;
+            iter |= sl.phase == ITER;
         }
-        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost, st));
+        if (!outer && !iter) continue;
+        active.push_back(g);
+        hipStream_t st = g->st;
+        const int B = g->B;
+        set_masks(g, maxit);
+        OPF_HIP(hipMemcpyAsync(g->dmask, g->hmask, sizeof(int) * M_COUNT * B, hipMemcpyHostToDevice, st));
+        const int* mk = g->dmask;
+        if (outer) {
+            ph_resid64<double>(g->H, st, mk + M_NEW * B, nullptr, g->b64, (double*)nullptr, FIN_BB);
+            if (g->H.precision == OPF_PREC_MIXED) {
+                ph_add64<float>(g->H, st, mk + M_ADD * B, g->x64, (const float*)g->H.e);
+                ph_resid64<float>(g->H, st, mk + M_OUTER * B, g->x64, g->b64, (float*)g->H.r, FIN_RR0);
+            } else {
+                ph_add64<double>(g->H, st, mk + M_ADD * B, g->x64, (const double*)g->H.e);
+                ph_resid64<double>(g->H, st, mk + M_OUTER * B, g->x64, g->b64, (double*)g->H.r, FIN_RR0);
+            }
+            hipLaunchKernelGGL(k_decide, dim3((B + 63) / 64), dim3(64), 0, st, B, mk + M_OUTER * B, mk + M_ALLOW * B,
+                               g->dmask + M_START * B, g->dmask + M_ITER * B, (const double*)g->H.scal, tol2);
+            const int* m = g->dmask + M_START * B;
+            if (g->H.precision == OPF_PREC_MIXED) {
+                ph_zero<float>(g->H, st, m, (float*)g->H.e);
+                ph_fcg_start<float>(g->H, st, m);
+            } else {
+                ph_zero<double>(g->H, st, m, (double*)g->H.e);
+                ph_fcg_start<double>(g->H, st, m);
+            }
+        }
+        for (int r = 0; r < g->replays; ++r) {
+            if (s->cfg.batched) {
+                OPF_HIP(hipGraphLaunch(g->exec, st));
+            } else {
+                const int* m = g->dmask + M_ITER * B;
+                for (int pc = 0; pc < 2; ++pc) {
+                    if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
+                    else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
+                }
+            }
+        }
+        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * B, hipMemcpyDeviceToHost, st));
     }
-    if (any_outer) {
-        for (auto& gp : s->groups)
-            if (needs(gp.get(), NEWREQ, OUTER)) OPF_HIP(hipStreamSynchronize(gp->st));
-        for (auto& gp : s->groups) {
-            Group* g = gp.get();
-            hipStream_t st = g->st;
-            bool start = false;
-            for (int k = 0; k < g->B; ++k) {
-                Slot& sl = g->slots[k];
-                if (sl.phase != NEWREQ && sl.phase != OUTER) continue;
-                const double* sc = g->H.hscal + (size_t)k * GMG_SSTRIDE;
+    for (Group* g : active) OPF_HIP(hipStreamSynchronize(g->st));
+    // ---- advance every slot's state machine from the scalars
+    for (Group* g : active) {
+        hipStream_t st = g->st;
+        for (int k = 0; k < g->B; ++k) {
+            Slot& sl = g->slots[k];
+            if (sl.phase == FREE) continue;
+            const double* sc = g->H.hscal + (size_t)k * GMG_SSTRIDE;
+            if (sl.phase == NEWREQ || sl.phase == OUTER) {
                 if (sl.phase == NEWREQ) {
                     sl.bb = sc[S_BB];
-                    sl.target = tol * tol * sl.bb;
+                    sl.target = tol2 * sl.bb;
                 }
                 sl.rr0 = sc[S_RR0];
                 sl.pending_add = false;
-                if (!(sl.rr0 > sl.target) || sl.bb == 0 || sl.its >= maxit || sl.nref >= 40) {
+                const bool allow = sl.its < maxit && sl.nref < 40;
+                const bool start = allow && sl.bb > 0 && sl.rr0 > tol2 * sl.bb;  // k_decide's test
+                if (!start) {
                     // finished: report, copy out (completes before the result is polled), free the slot
                     opf_srv_result r{};
                     r.id = sl.id;
@@ -350,68 +408,20 @@ int round(opf_srv* s, int* nfinished) {
                     g->last_used = s->now();
                     continue;
                 }
-                sl.phase = ITER;
+                sl.phase = ITER;  // started on the device and iterated this round
                 sl.nref++;
                 sl.inner_its = 0;
-                g->hmask[M_START * g->B + k] = 1;
-                start = true;
             }
-            if (start) {
-                OPF_HIP(hipMemcpyAsync(g->dmask + M_START * g->B, g->hmask + M_START * g->B, sizeof(int) * g->B,
-                                       hipMemcpyHostToDevice, st));
-                const int* m = g->dmask + M_START * g->B;
-                if (g->H.precision == OPF_PREC_MIXED) {
-                    ph_zero<float>(g->H, st, m, (float*)g->H.e);
-                    ph_fcg_start<float>(g->H, st, m);
-                } else {
-                    ph_zero<double>(g->H, st, m, (double*)g->H.e);
-                    ph_fcg_start<double>(g->H, st, m);
-                }
-            }
-        }
-    }
-    // ---- batched inner iterations (graph replays), all groups enqueued before any wait
-    bool any_iter = false;
-    for (auto& gp : s->groups) {
-        Group* g = gp.get();
-        if (!needs(g, ITER, ITER)) continue;
-        any_iter = true;
-        hipStream_t st = g->st;
-        set_masks(g);
-        OPF_HIP(hipMemcpyAsync(g->dmask + M_ITER * g->B, g->hmask + M_ITER * g->B, sizeof(int) * g->B,
-                               hipMemcpyHostToDevice, st));
-        for (int r = 0; r < g->replays; ++r) {
-            if (s->cfg.batched) {
-                OPF_TRY(capture(s, g));
-                OPF_HIP(hipGraphLaunch(g->exec, st));
-            } else {
-                const int* m = g->dmask + M_ITER * g->B;
-                for (int pc = 0; pc < 2; ++pc) {
-                    if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
-                    else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
-                }
-            }
-        }
-        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost,
-                               st));
-    }
-    if (any_iter) {
-        for (auto& gp : s->groups)
-            if (needs(gp.get(), ITER, ITER)) OPF_HIP(hipStreamSynchronize(gp->st));
-        for (auto& gp : s->groups) {
-            Group* g = gp.get();
-            for (int k = 0; k < g->B; ++k) {
-                Slot& sl = g->slots[k];
-                if (sl.phase != ITER) continue;
-                sl.its += 2 * g->replays;
-                sl.inner_its += 2 * g->replays;
-                double rr = g->H.hscal[(size_t)k * GMG_SSTRIDE + S_RR];
-                // FP32 inner solve: reduce the residual by ~1e-4 per refinement (or reach the target)
-                double inner = std::max(1e-4, std::sqrt(sl.target / sl.rr0) * 0.5);
-                if (!(rr > inner * inner * sl.rr0) || !std::isfinite(rr) || sl.inner_its >= 40 || sl.its >= maxit) {
-                    sl.phase = OUTER;
-                    sl.pending_add = true;
-                }
+            // ITER (including slots started this round): the round's replays ran
+            sl.its += 2 * g->replays;
+            sl.inner_its += 2 * g->replays;
+            const double rr = sc[S_RR];
+            // FP32 inner solve: reduce the residual by what is still needed (with a
+            // margin), at most by 1e-5 per refinement
+            const double inner = std::max(1e-5, std::sqrt(sl.target / sl.rr0) * 0.5);
+            if (!(rr > inner * inner * sl.rr0) || !std::isfinite(rr) || sl.inner_its >= 40 || sl.its >= maxit) {
+                sl.phase = OUTER;
+                sl.pending_add = true;
             }
         }
     }

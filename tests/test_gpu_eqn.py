@@ -126,7 +126,7 @@ def test_device_csr_matches_golden(ctx, name, bc, pin):
     mesh = of.MeshBuilder(ctx).newMesh(5, 5).setMeshOfDim(0, 0., 4.).setMeshOfDim(1, 0., 4.).build()
     p = _field(ctx, mesh, bc, 0.0, "p")
     h = of.makeEqnSolveHandler(
-        lambda e: 1.0 == of.d2x(of.D2SecondOrderCentered, e) + of.d2y(of.D2SecondOrderCentered, e), p,
+        lambda e: of.Equation(1.0, of.d2x(of.D2SecondOrderCentered, e) + of.d2y(of.D2SecondOrderCentered, e)), p,
         of.PrecondStructSolver(of.StructSolverParams(pinValue=pin)))
     ptr, col, val, rhs = h.csr()
     h.close()
