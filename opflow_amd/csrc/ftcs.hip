@@ -417,8 +417,16 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         }
         if (d.dim == 2 && k == FT_TB2) {
             int tiles = ((a.n[0] + FT_TS2 - 1) / FT_TS2) * ((a.n[1] + FT_TS2 - 1) / FT_TS2);
-            if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
-            else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+            if (tiles < 512) {
+                // small grids: 16 x 16 tiles so that every CU gets work (more halo recompute, fewer idle CUs)
+                int t16 = ((a.n[0] + 15) / 16) * ((a.n[1] + 15) / 16);
+                if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, 16, FT_TB2>), dim3(t16), dim3(256), 0, st, in, out, a, FT_TB2);
+                else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, 16, FT_TB2>), dim3(t16), dim3(256), 0, st, in, out, a, FT_TB2);
+            } else if (pow2) {
+                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+            } else {
+                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+            }
         } else if (d.dim == 2) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
             if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);

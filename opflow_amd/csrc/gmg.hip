@@ -1444,7 +1444,7 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
         // iterative refinement: FP64 residual / update, FP32 inner FCG
         float* e = (float*)H.e;
         bool first = true;
-        double target = 0;
+        double target = 0, floor = 1e-5, rr_prev = 0, used = 0;
         for (nref = 0; nref < 40; ++nref) {
             ph_resid64<float>(H, s, nullptr, x, b, (float*)H.r, FIN_RR0);
             OPF_TRY(read_scal(H, s));
@@ -1456,14 +1456,14 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
             double rr0 = H.hscal[S_RR0];
             if (verbose) fprintf(stderr, "[gmg mixed] refine %d relres %.3e\n", nref, std::sqrt(rr0 / bb));
             if (!(rr0 > target) || bb == 0 || total >= maxit) break;
+            if (nref > 0) floor = next_floor(floor, rr_prev, rr0, used);
+            rr_prev = rr0;
             if (H.singular) ph_project<float>(H, s, nullptr, (float*)H.r);
             ph_zero<float>(H, s, nullptr, e);
             int its = 0;
             double rrf = 0;
-            // reduce the residual by what is still needed (with a margin), at most by
-            // 1e-5 per refinement: the FP32 inner solve resolves ~1e-6 of its right-hand side
-            double inner = std::max(1e-5, std::sqrt(target / rr0) * 0.5);
-            OPF_TRY(fcg<float>(H, s, e, rr0, std::min(inner, 0.5), std::max(1, maxit - total), &its, &rrf));
+            used = inner_target(target, rr0, floor);
+            OPF_TRY(fcg<float>(H, s, e, rr0, used, std::max(1, maxit - total), &its, &rrf));
             total += its;
             ph_add64<float>(H, s, nullptr, x, (const float*)e);
         }

@@ -10,6 +10,7 @@
 // per-face arrays (variable coefficients).
 #pragma once
 
+#include <cmath>
 #include <vector>
 
 #include "fastdiv.hpp"
@@ -93,6 +94,19 @@ struct Hierarchy {
 int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
               int precision, int slots = 1);
 int read_scal(const Hierarchy& H, hipStream_t st);
+// Inner (FP32) reduction target of a refinement: what is still needed, with a
+// margin, but not below `floor`, the reduction FP32 can deliver. floor starts
+// at 1e-5 and follows the true reductions the refinements achieve (an
+// ill-conditioned operator limits an FP32 correction to fewer digits).
+inline double inner_target(double target, double rr0, double floor) {
+    double v = std::sqrt(target / rr0) * 0.5;
+    return std::min(0.5, std::max(floor, v));
+}
+inline double next_floor(double floor, double rr0_prev, double rr0_new, double used) {
+    const double got = std::sqrt(rr0_new / rr0_prev);  // true reduction of the last refinement
+    if (got > 10.0 * used) floor = std::max(floor, std::min(0.1, 0.5 * got));
+    return floor;
+}
 // batched phases over all slots selected by a device mask (null: all slots)
 template <typename T>
 void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask);

@@ -49,6 +49,7 @@ struct Slot {
     uint64_t id = 0;
     double t_submit = 0, t_admit = 0;
     double bb = 0, target = 0, rr0 = 0;
+    double floor = 1e-5, used = 0;  // inner reduction floor and the last inner target
     int its = 0, nref = 0, inner_its = 0;
     bool pending_add = false;
     double* out_x = nullptr;
@@ -155,7 +156,7 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     g->N = L.N;
     g->B = slots_for(*s, L.N);
     g->replays = replays_for(L.N);
-    OPF_HIP(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    OPF_HIP(hipStreamCreate(&g->st));  // default flags (faster dispatch than a non-blocking stream)
     int prec = s->cfg.precision == OPF_PREC_MIXED ? OPF_PREC_MIXED : OPF_PREC_FP64;
     int rc = gmg_build(s->ctx, g->H, L, per, alpha, prec, g->B);
     if (rc != OPF_OK) {
@@ -379,6 +380,8 @@ int round(opf_srv* s, int* nfinished) {
                 if (sl.phase == NEWREQ) {
                     sl.bb = sc[S_BB];
                     sl.target = tol2 * sl.bb;
+                } else {
+                    sl.floor = next_floor(sl.floor, sl.rr0, sc[S_RR0], sl.used);
                 }
                 sl.rr0 = sc[S_RR0];
                 sl.pending_add = false;
@@ -411,14 +414,13 @@ int round(opf_srv* s, int* nfinished) {
                 sl.phase = ITER;  // started on the device and iterated this round
                 sl.nref++;
                 sl.inner_its = 0;
+                sl.used = inner_target(sl.target, sl.rr0, sl.floor);
             }
             // ITER (including slots started this round): the round's replays ran
             sl.its += 2 * g->replays;
             sl.inner_its += 2 * g->replays;
             const double rr = sc[S_RR];
-            // FP32 inner solve: reduce the residual by what is still needed (with a
-            // margin), at most by 1e-5 per refinement
-            const double inner = std::max(1e-5, std::sqrt(sl.target / sl.rr0) * 0.5);
+            const double inner = sl.used;  // FP32 inner reduction target of this refinement
             if (!(rr > inner * inner * sl.rr0) || !std::isfinite(rr) || sl.inner_its >= 40 || sl.its >= maxit) {
                 sl.phase = OUTER;
                 sl.pending_add = true;
