@@ -140,7 +140,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: the engine is single-GPU by design "
-              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(a slab decomposition along z with ghost exchange is the planned sharding, "
+              "not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import opflow_amd as of
@@ -210,7 +211,7 @@ def main():
         u2.close()
         try:
             from bench_solver import solver_extra
-            extra.update(solver_extra(of, ctx, ev))
+            extra.update(solver_extra(of, ctx, ev, cpu=not args.no_cpu))
         except ImportError:
             pass
 

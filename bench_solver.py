@@ -1,7 +1,9 @@
 """Implicit-solver and serving legs of bench.py (extra keys of the JSON line).
 
 cfg3: backward-Euler/Poisson solves at 2048^2 and 256^3 (Dirichlet, bumps +
-1e-3 noise RHS) to relative residual 1e-10 -- FP64 GMG-FCG vs FP32-inner/FP64-IR.
+1e-3 noise RHS) to relative residual 1e-10 -- FP64 GMG-FCG vs FP32-inner/FP64-IR
+(and, with cpu=True, the CPU restatement of the reference's solver).
+cfg4: lid-driven cavity 1024^2, 500 steps, mixed vs FP64 pressure solves.
 cfg5: served Poisson solves of mixed sizes at tolerance 1e-8 -- solves/s closed
 loop and p50/p99 latency open loop, next to the plain FP64 solver one at a time.
 """
@@ -29,7 +31,7 @@ def _solve_timed(of, ctx, ev, desc, b, precision, tol, reps=3):
     return x, st, float(np.median(times))
 
 
-def cfg3(of, ctx, ev):
+def cfg3(of, ctx, ev, cpu=False):
     out = {}
     for n in ([2048, 2048], [256, 256, 256]):
         dim = len(n)
@@ -45,7 +47,42 @@ def cfg3(of, ctx, ev):
                       "max_err_vs_fp64": err},
             "mixed_speedup": t64 / t32,
         }
+        if cpu:
+            # reported CPU baseline: the oracle's C/OpenMP restatement of the reference's
+            # CPU solver (flexible PCG + V(1,1) weighted-Jacobi multigrid, PFMG defaults)
+            import oracle
+            t0 = time.perf_counter()
+            xc, itc, relc = oracle.pcg_mg(n, [1.0 / n[0]] * dim, [(1, 1)] * dim, b, tol=1e-10, maxit=400)
+            tc = time.perf_counter() - t0
+            out[f"poisson_{key}_relres1e-10"]["cpu_baseline"] = {
+                "s_per_solve": tc, "iters": itc, "relres": relc, "cores": oracle.num_threads(), "kind": "port",
+                "max_err_vs_gpu_fp64": float(np.max(np.abs(xc - x64)) / np.max(np.abs(x64)))}
     return out
+
+
+def cfg4(of, ctx, ev, n=1024, steps=500):
+    """Lid-driven cavity 1024^2 MAC, Re 1000, 500 steps (the C++ OpFlow program
+    opflow_amd/examples/lid_driven2d.cpp): mixed-precision pressure solves vs FP64."""
+    import json
+    import os
+    import subprocess
+    import tempfile
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "opflow_amd", "bin", "lid_driven2d")
+    if not os.path.exists(exe):
+        return {"lid_driven_1024sq_500steps": {"skipped": "opflow_amd/bin/lid_driven2d not built"}}
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        for mixed in (0, 1):
+            path = os.path.join(td, f"u{mixed}.bin")
+            r = subprocess.run([exe, str(n), str(steps), str(mixed), path], capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                return {"lid_driven_1024sq_500steps": {"error": r.stderr[-400:]}}
+            res[mixed] = (json.loads(r.stdout.strip().splitlines()[-1]), np.fromfile(path))
+    (a, ua), (b, ub) = res[0], res[1]
+    return {"lid_driven_1024sq_500steps": {
+        "fp64": {k: a[k] for k in ("s_per_step", "poisson_iters", "worst_relres", "max_div", "dt")},
+        "mixed": {k: b[k] for k in ("s_per_step", "poisson_iters", "worst_relres", "max_div")},
+        "velocity_max_rel_diff_mixed_vs_fp64": float(np.max(np.abs(ua - ub)) / np.max(np.abs(ua)))}}
 
 
 def _requests(count, seed):
@@ -160,8 +197,9 @@ def cfg5(of, ctx, ev, count=48):
         "throughput_ratio": t_64 / t_srv}}
 
 
-def solver_extra(of, ctx, ev):
+def solver_extra(of, ctx, ev, cpu=False):
     out = {}
-    out.update(cfg3(of, ctx, ev))
+    out.update(cfg3(of, ctx, ev, cpu))
+    out.update(cfg4(of, ctx, ev))
     out.update(cfg5(of, ctx, ev))
     return out

@@ -68,7 +68,9 @@ def test_dirc_eqn_manual_unified_solve(ctx, precision):
     err = _check(p, pt, 1e-10)
     print(f"Dirc {precision}: iters {st.niter} relerr {st.relerr:.2e} max pointwise rel err {err:.2e}", flush=True)
     assert st.relerr <= 1e-10
-    assert err <= 1e-8  # see DESIGN.md: residual tolerance vs pointwise error
+    # the 1e-10 tolerance bounds the relative residual; the pointwise error of the
+    # solution is larger by up to the operator's condition number, hence 1e-8 here
+    assert err <= 1e-8
     # solving twice through a handler (HandlerSolveTwice pattern)
     h = of.makeEqnSolveHandler(eqn, p, of.PrecondStructSolver(of.StructSolverParams(tol=1e-10, staticMat=True)))
     p.assign(0.0)
