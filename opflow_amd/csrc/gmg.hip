@@ -577,6 +577,8 @@ __global__ __launch_bounds__(256) void zk_resid(GOp<T> o, const T* __restrict__ 
     }
 }
 
+#include "gmg_zp.hpp"
+
 // coarse rhs = average over the 2^d children of the fine residual b - A x
 template <typename T, bool VAR, int DIM>
 __global__ __launch_bounds__(256) void k_restrict(GOp<T> o, const T* __restrict__ x, const T* __restrict__ b,
@@ -902,8 +904,38 @@ static ZTile ztile(const GLevelHost& L) {
     z.items = z.tx * z.ty * z.nz;
     return z;
 }
-// blocks (= partial sums per value) of every finest-level reduction launch
+// z-pipelined LDS-ring passes (gmg_zp.hpp) for big 3-D constant-coefficient levels
+template <typename T>
+static ZpGrid zp_grid(const GLevelHost& L) {
+    using C = ZpCfg<T>;
+    ZpGrid g;
+    g.tx = (L.n[0] + C::TX - 1) / C::TX;
+    g.ty = (L.n[1] + C::TY - 1) / C::TY;
+    const long long tiles = (long long)g.tx * g.ty;
+    // chunks of 16 planes, fewer when that leaves under ~8 workgroups per CU
+    int zc = 16;
+    while (zc > 2 && tiles * ((L.n[2] + zc - 1) / zc) < 2048) zc /= 2;
+    while (zc < L.n[2] && tiles * ((L.n[2] + zc - 1) / zc) > GMG_MAXB) zc *= 2;
+    g.zc = zc;
+    g.nch = (L.n[2] + zc - 1) / zc;
+    g.blocks = (int)std::min<long long>(tiles * g.nch, 1LL << 30);
+    return g;
+}
+static bool use_zp(const Hierarchy& H, int l) {
+    const GLevelHost& L = H.host[l];
+    return H.dim == 3 && !H.var && L.n[0] >= 64 && L.n[1] >= 16 && L.n[2] >= 8 && L.N >= (1 << 18) &&
+           zp_grid<double>(L).blocks <= GMG_MAXB && zp_grid<float>(L).blocks <= GMG_MAXB;
+}
+// the fused restriction also needs even extents (children pair up inside a tile and a chunk)
+static bool use_zp_restrict(const Hierarchy& H, int l) {
+    const GLevelHost& L = H.host[l];
+    return use_zp(H, l) && L.n[0] % 2 == 0 && L.n[1] % 2 == 0 && L.n[2] % 2 == 0;
+}
+
+// blocks (= partial sums per value) of every finest-level reduction launch in precision T
+template <typename T>
 static int red_count(const Hierarchy& H) {
+    if (use_zp(H, 0)) return zp_grid<T>(H.host[0]).blocks;
     if (use_z(H, 0)) return std::min(GMG_MAXB, ztile(H.host[0]).items);
     return rblk(H.host[0].N);
 }
@@ -1209,6 +1241,16 @@ static inline Bat bat(const Hierarchy& H, const int* mask, long long vs) {
     return Bat{mask, vs, GMG_SSTRIDE, GMG_PSTRIDE};
 }
 
+// one z-pipelined pass over level l (every slot of the batch)
+template <typename T, int MODE, typename TO = T>
+static void zp_launch(const Hierarchy& H, int l, hipStream_t st, const Bat& bt, const T* in0, const T* in1,
+                      const T* cb, const T* cq, T* out0, TO* out1, double* part, long long cvs = 0) {
+    const ZpGrid g = zp_grid<T>(H.host[l]);
+    const GOp<T> o = make_op<T>(H, l);
+    hipLaunchKernelGGL((zp_pass<T, MODE, TO>), dim3(g.blocks, (unsigned)H.B), dim3(ZpCfg<T>::NT), 0, st, o, in0, in1,
+                       cb, cq, out0, out1, (const double*)H.scal, part, (T)H.omega, g, cvs, bt);
+}
+
 template <typename T>
 void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out,
                const T* dot_q = nullptr) {
@@ -1219,7 +1261,7 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     if (l == (int)H.dev.size() - 1) {
         hipLaunchKernelGGL(k_dense<T>, dim3(1, B), dim3(256), 0, st, H.cinv, H.cN, rhs, out, bat(H, mask, N));
         if (dot_q)  // a one-level hierarchy: the dot products follow the dense solve
-            hipLaunchKernelGGL(k_dot3<T>, dim3(red_count(H), B), dim3(256), 0, st, N, rhs, (const T*)out, dot_q,
+            hipLaunchKernelGGL(k_dot3<T>, dim3(red_count<T>(H), B), dim3(256), 0, st, N, rhs, (const T*)out, dot_q,
                                H.part, bat(H, mask, N));
         return;
     }
@@ -1232,12 +1274,15 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     if (((H.pre - 1) + H.post) & 1) std::swap(cur, tmp);
     const Bat bt = bat(H, mask, N);
     const bool zm = use_z(H, l);
+    const bool zp = use_zp(H, l);
     const ZTile zt = ztile(L);
     int s0 = 1;
     if (!H.var && H.pre >= 2) {
         // sweeps 1 and 2 from zero in one pass, landing where the second sweep of
         // the two-launch sequence would (the buffer after `cur`)
-        if (zm && H.dim == 3)
+        if (zp)
+            zp_launch<T, ZP_JAC2Z>(H, l, st, bt, rhs, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
+        else if (zm && H.dim == 3)
             hipLaunchKernelGGL((zk_jacobi2z<T, 3>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
         else if (zm)
             hipLaunchKernelGGL((zk_jacobi2z<T, 2>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
@@ -1252,7 +1297,9 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
                                                       o, (const T*)nullptr, rhs, cur, w, bt));
     }
     for (int s = s0; s < H.pre; ++s) {
-        if (zm)
+        if (zp)
+            zp_launch<T, ZP_JAC>(H, l, st, bt, (const T*)cur, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
+        else if (zm)
             GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
                                                           (const T*)cur, rhs, tmp, w, zt, bt));
         else
@@ -1262,8 +1309,13 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     }
     const GLevelDev& C = H.dev[l + 1];
     const FastDiv fc0((uint32_t)C.n[0]), fc1((uint32_t)C.n[1]);
-    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N), B), dim3(256), 0, st, o,
-                                                  (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], fc0, fc1, bt));
+    if (use_zp_restrict(H, l))
+        zp_launch<T, ZP_RESTRICT>(H, l, st, bt, (const T*)cur, nullptr, rhs, nullptr, (T*)C.b, (T*)nullptr, nullptr,
+                                  C.N);
+    else
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_restrict<T, V_, D_>), dim3(nblk(C.N), B), dim3(256), 0, st,
+                                                      o, (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], fc0,
+                                                      fc1, bt));
     ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
     if (H.dim == 3)
         hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
@@ -1274,13 +1326,17 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     for (int s = 0; s < H.post; ++s) {
         if (dot_q && s == H.post - 1) {
             // last sweep also forms the partials of (r, z), (z, q), sum(z) (finest level only)
-            const int G = red_count(H);
-            if (zm)
+            const int G = red_count<T>(H);
+            if (zp)
+                zp_launch<T, ZP_JAC_DOT3>(H, l, st, bt, (const T*)cur, nullptr, rhs, dot_q, tmp, (T*)nullptr, H.part);
+            else if (zm)
                 GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi_dot3<T, D_>), dim3(G, B), dim3(256), 0, st, o,
                                                               (const T*)cur, rhs, tmp, w, dot_q, H.part, zt, bt));
             else
                 GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi_dot3<T, V_, D_>), dim3(G, B), dim3(256), 0,
                                                               st, o, (const T*)cur, rhs, tmp, w, dot_q, H.part, bt));
+        } else if (zp) {
+            zp_launch<T, ZP_JAC>(H, l, st, bt, (const T*)cur, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
         } else if (zm) {
             GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
                                                           (const T*)cur, rhs, tmp, w, zt, bt));
@@ -1292,9 +1348,11 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     }
 }
 
+// finish a reduction whose partials came from a finest-level launch in precision T
+template <typename T>
 static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular) {
     const long long N = H.host[0].N;
-    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, red_count(H), mode, H.scal,
+    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, red_count<T>(H), mode, H.scal,
                        1.0 / N, singular, bat(H, mask, N));
 }
 
@@ -1310,10 +1368,10 @@ void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask) {
         ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z, (const T*)H.r);
     } else {
         ph_vcycle<T>(H, st, mask, 0, (const T*)H.r, (T*)H.z);
-        hipLaunchKernelGGL(k_dot3<T>, dim3(red_count(H), B), dim3(256), 0, st, N, (const T*)H.r, (const T*)H.z,
+        hipLaunchKernelGGL(k_dot3<T>, dim3(red_count<T>(H), B), dim3(256), 0, st, N, (const T*)H.r, (const T*)H.z,
                            (const T*)H.z, H.part, bt);
     }
-    ph_finish(H, st, mask, FIN_DOT3_FIRST, H.singular);
+    ph_finish<T>(H, st, mask, FIN_DOT3_FIRST, H.singular);
 }
 
 // one FCG iteration on (e, r) with p ping-pong index pc
@@ -1321,7 +1379,7 @@ template <typename T>
 void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int pc) {
     const long long N = H.host[0].N;
     const unsigned B = (unsigned)H.B;
-    const int G = red_count(H);
+    const int G = red_count<T>(H);
     const Bat bt = bat(H, mask, N);
     GOp<T> o = make_op<T>(H, 0);
     T* z = (T*)H.z;
@@ -1329,16 +1387,18 @@ void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int 
     T* r = (T*)H.r;
     T* pold = (T*)H.p[pc];
     T* pnew = (T*)H.p[1 - pc];
-    if (use_z(H, 0))
+    if (use_zp(H, 0))
+        zp_launch<T, ZP_SPMV_P>(H, 0, st, bt, (const T*)z, (const T*)pold, nullptr, nullptr, pnew, q, H.part);
+    else if (use_z(H, 0))
         GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_spmv_p<T, D_>), dim3(G, B), dim3(256), 0, st, o, (const T*)z,
                                                       (const T*)pold, pnew, q, H.scal, H.part, ztile(H.host[0]), bt));
     else
         GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o,
                                                       (const T*)z, (const T*)pold, pnew, q, H.scal, H.part, bt));
-    ph_finish(H, st, mask, FIN_PQ, H.singular);
+    ph_finish<T>(H, st, mask, FIN_PQ, H.singular);
     hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
                        bt);
-    ph_finish(H, st, mask, FIN_RR, H.singular);
+    ph_finish<T>(H, st, mask, FIN_RR, H.singular);
     if (H.post > 0) {
         ph_vcycle<T>(H, st, mask, 0, (const T*)r, z, (const T*)q);  // last sweep forms the dot products
     } else {
@@ -1346,7 +1406,7 @@ void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int 
         hipLaunchKernelGGL(k_dot3<T>, dim3(G, B), dim3(256), 0, st, N, (const T*)r, (const T*)z, (const T*)q, H.part,
                            bt);
     }
-    ph_finish(H, st, mask, FIN_DOT3, H.singular);
+    ph_finish<T>(H, st, mask, FIN_DOT3, H.singular);
 }
 
 // r_T = b - A x in FP64 (x may be null), partial sums of r^2 -> finish `mode`
@@ -1355,22 +1415,25 @@ void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const doubl
                 int mode) {
     GOp<double> o = make_op<double>(H, 0);
     const long long N = H.host[0].N;
-    if (use_z(H, 0))
-        GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_resid<double, TO, D_>), dim3(red_count(H), (unsigned)H.B),
+    if (x && use_zp(H, 0))
+        zp_launch<double, ZP_RESID, TO>(H, 0, st, bat(H, mask, N), x, nullptr, b, nullptr, (double*)nullptr, r,
+                                        H.part);
+    else if (use_z(H, 0))
+        GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_resid<double, TO, D_>), dim3(red_count<double>(H), (unsigned)H.B),
                                                       dim3(256), 0, st, o, x, b, r, H.part, ztile(H.host[0]),
                                                       bat(H, mask, N)));
     else
-        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(red_count(H), (unsigned)H.B),
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(red_count<double>(H), (unsigned)H.B),
                                                       dim3(256), 0, st, o, x, b, r, H.part, bat(H, mask, N)));
-    ph_finish(H, st, mask, mode, H.singular);
+    ph_finish<double>(H, st, mask, mode, H.singular);
 }
 
 template <typename T>
 void ph_project(const Hierarchy& H, hipStream_t st, const int* mask, T* r) {
     const long long N = H.host[0].N;
     const Bat bt = bat(H, mask, N);
-    hipLaunchKernelGGL(k_sum<T>, dim3(red_count(H), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
-    ph_finish(H, st, mask, FIN_RR0, 1);
+    hipLaunchKernelGGL(k_sum<T>, dim3(red_count<T>(H), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
+    ph_finish<T>(H, st, mask, FIN_RR0, 1);
     hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, r, H.scal, (int)S_RR0,
                        1.0 / N, bt);
 }
@@ -1486,6 +1549,12 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
 int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y) {
     GOp<double> o = make_op<double>(H, 0);
     const long long N = H.host[0].N;
+    if (use_zp(H, 0) && H.B == 1) {
+        zp_launch<double, ZP_APPLY>(H, 0, ctx->stream, bat(H, nullptr, N), x, nullptr, nullptr, nullptr, y,
+                                    (double*)nullptr, nullptr);
+        OPF_HIP(hipGetLastError());
+        return OPF_OK;
+    }
     GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_apply<double, V_, D_>), dim3(nblk(N)), dim3(256), 0,
                                                   ctx->stream, o, x, y));
     OPF_HIP(hipGetLastError());

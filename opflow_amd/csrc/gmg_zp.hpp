@@ -1,0 +1,331 @@
+// Z-pipelined stencil passes over 3-D constant-coefficient multigrid levels
+// (included by gmg.hip after its helpers; namespace opf).
+//
+// A workgroup owns a TX x TY column of the x-y plane and a chunk of planes
+// [z0, z1). It streams the planes of the stencilled vector, each with a
+// one-cell halo in x and y, through a 4-plane LDS ring: the loads of plane
+// z + PF are in flight (registers) while plane z - 1 is computed from planes
+// z - 2 .. z of the ring. Four slots let one barrier per plane separate the
+// store of plane z from the last reads of plane z - 4.
+//
+// Boundary conditions are folded into the halo: a periodic side wraps and a
+// Dirichlet / Neumann side stores the ghost value alpha * (boundary cell)
+// (alpha = 1 - bfac: -1 Dirichlet, +1 Neumann). Then
+//   (A v)_c = dint v_c - sum_faces a_d v_nb
+// holds at every cell with one straight-line expression; only the
+// smoother's diagonal differs at boundary cells:
+//   D_c = dint - sum over the cell's boundary faces of a_d alpha.
+// Every load is unconditional (padding lanes and lanes past a ragged edge read
+// a valid cell), so the compiler keeps counted waits on the prefetch.
+//
+// Modes (one kernel template, fused epilogues):
+//   ZP_JAC      xo = x + (w / D)(b - A x)
+//   ZP_JAC_DOT3 the same + partials of (b, xo), (xo, q), sum(xo)
+//   ZP_JAC2Z    two sweeps from zero: x1 = (w / D) b in the ring, xo = x1 + (w / D)(b - A x1)
+//   ZP_SPMV_P   p = (z - zmean) + beta p_old in the ring; p_new = p, q = A p, partial (p, q)
+//   ZP_RESID    r = b - A x (optional store, precision TO), partial r^2
+//   ZP_RESTRICT coarse b = average over the 2 x 2 x 2 children of b - A x (even extents)
+//   ZP_APPLY    y = A x
+
+template <typename T>
+struct ZpCfg;
+template <>
+struct ZpCfg<double> {
+    static constexpr int TX = 32, TY = 16, NT = 256, PF = 2;
+};
+template <>
+struct ZpCfg<float> {
+    static constexpr int TX = 64, TY = 8, NT = 256, PF = 4;
+};
+
+struct ZpGrid {
+    int tx, ty;  // tiles in x and y
+    int zc;      // planes per chunk (even)
+    int nch;     // chunks in z
+    int blocks;  // tx * ty * nch
+};
+
+enum { ZP_JAC = 0, ZP_JAC_DOT3, ZP_JAC2Z, ZP_SPMV_P, ZP_RESID, ZP_RESTRICT, ZP_APPLY };
+
+template <typename T>
+__device__ __forceinline__ T zp_alpha(const GOp<T>& o, int d, int s) {
+    return (T)1 - o.bfac[d][s];
+}
+
+// in-grid cell whose value stands at coordinate g in [-1, n] of dimension d
+// (times f: the ghost factor); coordinates past n (ragged tiles) clamp to a
+// valid cell whose value is never used
+template <typename T>
+__device__ __forceinline__ int zp_mirror(const GOp<T>& o, int d, int g, int n, T& f) {
+    if (g < 0) {
+        if (o.per[d]) return g + n;
+        f *= zp_alpha(o, d, 0);
+        return 0;
+    }
+    if (g >= n) {
+        if (g == n) {
+            if (o.per[d]) return 0;
+            f *= zp_alpha(o, d, 1);
+        }
+        return n - 1;
+    }
+    return g;
+}
+
+// boundary-face part of the diagonal of a cell at coordinate c (0 <= c < n)
+template <typename T>
+__device__ __forceinline__ T zp_dadj(const GOp<T>& o, int d, int c, int n) {
+    if (o.per[d]) return (T)0;
+    T r = 0;
+    if (c == 0) r += o.a[d] * zp_alpha(o, d, 0);
+    if (c == n - 1) r += o.a[d] * zp_alpha(o, d, 1);
+    return r;
+}
+
+template <typename T, int MODE, typename TO>
+__global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ in0, const T* __restrict__ in1,
+                                               const T* __restrict__ cb, const T* __restrict__ cq,
+                                               T* __restrict__ out0, TO* __restrict__ out1,
+                                               const double* __restrict__ scal, double* __restrict__ part, T w,
+                                               ZpGrid g, long long cvs, Bat bt) {
+    using C = ZpCfg<T>;
+    constexpr int TX = C::TX, TY = C::TY, NT = C::NT, PF = C::PF;
+    constexpr int RX = TX + 2, RY = TY + 2, RP = RX * RY;
+    constexpr int NL = (RP + NT - 1) / NT;  // ring elements per thread
+    constexpr int CPT = TX * TY / NT;       // output cells per thread and plane
+    constexpr int RS = NT / TX;             // rows between a thread's output cells
+    static_assert((TX * TY) % NT == 0 && NT % TX == 0 && TX % 2 == 0 && TY % 2 == 0, "tile shape");
+    constexpr bool RING1 = MODE == ZP_SPMV_P;
+    constexpr bool SMOOTH = MODE == ZP_JAC || MODE == ZP_JAC_DOT3 || MODE == ZP_JAC2Z;
+    constexpr bool HASB = MODE != ZP_SPMV_P && MODE != ZP_APPLY;
+    constexpr bool HASQ = MODE == ZP_JAC_DOT3;
+    constexpr bool RED1 = MODE == ZP_SPMV_P || MODE == ZP_RESID;
+    __shared__ T ring[4][NL * NT];
+    __shared__ T rbuf[MODE == ZP_RESTRICT ? 2 * TX * TY : 1];
+
+    BAT_SLOT(bt);
+    in0 += vo;
+    if (RING1) in1 += vo;
+    if (HASB) cb += vo;
+    if (HASQ) cq += vo;
+    if (MODE == ZP_RESTRICT) out0 += (long long)slot * cvs;
+    else if (out0) out0 += vo;
+    if (out1) out1 += vo;
+    if (part) part += (long long)slot * bt.ps;
+    T beta = 0, zmean = 0;
+    if (RING1) {
+        const double* sc = scal + (long long)slot * bt.ss;
+        beta = (T)sc[S_BETA];
+        zmean = (T)sc[S_ZMEAN];
+    }
+
+    const int tid = threadIdx.x;
+    int bi = (int)xcd_block(blockIdx.x, gridDim.x);  // neighbouring tiles share an XCD's L2
+    const int x0 = (bi % g.tx) * TX;
+    bi /= g.tx;
+    const int y0 = (bi % g.ty) * TY;
+    const int z0 = (bi / g.ty) * g.zc;
+    const int z1 = min(z0 + g.zc, o.n2);
+
+    // ring elements of this thread: in-plane offset of the cell it reads and its ghost factor
+    uint32_t off[NL];
+    T fac[NL];
+    T dxe[NL], wde[NL];  // ZP_JAC2Z: x/y diagonal part and w / D of the cell read (interior z)
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        int e = tid + m * NT;
+        if (e >= RP) e = 0;
+        const int lx = e % RX, ly = e / RX;
+        T f = 1;
+        const int mx = zp_mirror(o, 0, x0 - 1 + lx, o.n0, f);
+        const int my = zp_mirror(o, 1, y0 - 1 + ly, o.n1, f);
+        off[m] = (uint32_t)mx + o.s1 * (uint32_t)my;
+        fac[m] = f;
+        if (MODE == ZP_JAC2Z) {
+            dxe[m] = zp_dadj(o, 0, mx, o.n0) + zp_dadj(o, 1, my, o.n1);
+            wde[m] = dxe[m] != (T)0 ? w / (o.dint - dxe[m]) : o.wdi;
+        }
+    }
+    // output cells of this thread: (lx, ly0 + m * RS)
+    const int lx = tid % TX, ly0 = tid / TX;
+    uint32_t coff[CPT];
+    bool cval[CPT];
+    T dxc[CPT], wdc[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int gx = x0 + lx, gy = y0 + ly0 + m * RS;
+        cval[m] = gx < o.n0 && gy < o.n1;
+        const int cx = min(gx, o.n0 - 1), cy = min(gy, o.n1 - 1);
+        coff[m] = (uint32_t)cx + o.s1 * (uint32_t)cy;
+        if (SMOOTH) {
+            dxc[m] = zp_dadj(o, 0, cx, o.n0) + zp_dadj(o, 1, cy, o.n1);
+            wdc[m] = dxc[m] != (T)0 ? w / (o.dint - dxc[m]) : o.wdi;
+        }
+    }
+
+    const int zb = z0 - 1, ze = z1 + 1;  // ring planes zb .. ze - 1
+    T pre[PF][NL];
+    T pre1[RING1 ? PF : 1][NL];
+    T fzp[PF];
+    int mzp[PF];
+    T cbp[HASB ? PF : 1][CPT];
+    T cqp[HASQ ? PF : 1][CPT];
+    auto ring_load = [&](T(&r)[NL], T(&r1)[NL], T& fz, int& mzo, int z) {
+        T f = 1;
+        const int mz = zp_mirror(o, 2, z, o.n2, f);
+        fz = f;
+        mzo = mz;
+        const T* p = in0 + (size_t)o.s2 * (uint32_t)mz;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) r[m] = p[off[m]];
+        if (RING1) {
+            const T* p1 = in1 + (size_t)o.s2 * (uint32_t)mz;
+#pragma unroll
+            for (int m = 0; m < NL; ++m) r1[m] = p1[off[m]];
+        }
+    };
+    auto ctr_load = [&](T(&rb)[CPT], T(&rq)[CPT], int z) {
+        z = min(max(z, z0), z1 - 1);
+        const size_t po = (size_t)o.s2 * (uint32_t)z;
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            if (HASB) rb[m] = cb[po + coff[m]];
+            if (HASQ) rq[m] = cq[po + coff[m]];
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        ring_load(pre[u], pre1[RING1 ? u : 0], fzp[u], mzp[u], min(zb + u, ze - 1));
+        ctr_load(cbp[HASB ? u : 0], cqp[HASQ ? u : 0], zb + u - 1);
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int zz0 = zb; zz0 < ze; zz0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int zz = zz0 + u;
+            if (zz >= ze) break;  // uniform over the block
+            T* rs = ring[(zz - zb) & 3];
+            {
+                const T fz = fzp[u];
+                if (MODE == ZP_JAC2Z) {
+                    const int mz = mzp[u];
+                    const T dz = zp_dadj(o, 2, mz, o.n2);
+                    if (dz != (T)0) {  // a boundary plane of a non-periodic z (uniform over the block)
+#pragma unroll
+                        for (int m = 0; m < NL; ++m)
+                            rs[tid + m * NT] = (fac[m] * fz) * ((w / (o.dint - (dxe[m] + dz))) * pre[u][m]);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < NL; ++m) rs[tid + m * NT] = (fac[m] * fz) * (wde[m] * pre[u][m]);
+                    }
+                } else if (RING1) {
+#pragma unroll
+                    for (int m = 0; m < NL; ++m)
+                        rs[tid + m * NT] = (fac[m] * fz) * ((pre[u][m] - zmean) + beta * pre1[u][m]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NL; ++m) rs[tid + m * NT] = (fac[m] * fz) * pre[u][m];
+                }
+            }
+            ring_load(pre[u], pre1[RING1 ? u : 0], fzp[u], mzp[u], min(zz + PF, ze - 1));
+            __syncthreads();
+            const int zl = zz - 1;  // output plane
+            if (zl < z0) {          // pipeline still filling (uniform)
+                ctr_load(cbp[HASB ? u : 0], cqp[HASQ ? u : 0], zl + PF);
+                continue;
+            }
+            const T* pm = ring[(zl - 1 - zb) & 3];
+            const T* pc = ring[(zl - zb) & 3];
+            const T* pp = ring[(zl + 1 - zb) & 3];
+            const size_t po = (size_t)o.s2 * (uint32_t)zl;
+            T wdp[CPT];  // w / D of this plane's output cells
+            if (SMOOTH) {
+                const T dz = zp_dadj(o, 2, zl, o.n2);
+                if (dz != (T)0) {  // boundary plane of a non-periodic z (uniform)
+#pragma unroll
+                    for (int m = 0; m < CPT; ++m) wdp[m] = w / (o.dint - (dxc[m] + dz));
+                } else {
+#pragma unroll
+                    for (int m = 0; m < CPT; ++m) wdp[m] = wdc[m];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int ly = ly0 + m * RS;
+                const int e = (lx + 1) + RX * (ly + 1);
+                const T c = pc[e];
+                T acc = o.a[0] * pc[e - 1];
+                acc += o.a[0] * pc[e + 1];
+                acc += o.a[1] * pc[e - RX];
+                acc += o.a[1] * pc[e + RX];
+                acc += o.a[2] * pm[e];
+                acc += o.a[2] * pp[e];
+                const T av = o.dint * c - acc;
+                if (SMOOTH) {
+                    const T bv = cbp[HASB ? u : 0][m];
+                    const T xo = c + wdp[m] * (bv - av);
+                    if (cval[m]) {
+                        out0[po + coff[m]] = xo;
+                        if (MODE == ZP_JAC_DOT3) {
+                            const double zd = (double)xo;
+                            s0 += (double)bv * zd;
+                            s1 += zd * (double)cqp[HASQ ? u : 0][m];
+                            s2 += zd;
+                        }
+                    }
+                } else if (MODE == ZP_SPMV_P) {
+                    if (cval[m]) {
+                        out0[po + coff[m]] = c;
+                        out1[po + coff[m]] = (TO)av;
+                        s0 += (double)c * (double)av;
+                    }
+                } else if (MODE == ZP_RESID) {
+                    const T v = cbp[HASB ? u : 0][m] - av;
+                    if (cval[m]) {
+                        if (out1) out1[po + coff[m]] = (TO)v;
+                        s0 += (double)v * (double)v;
+                    }
+                } else if (MODE == ZP_APPLY) {
+                    if (cval[m]) out0[po + coff[m]] = av;
+                } else if (MODE == ZP_RESTRICT) {
+                    const T v = cbp[HASB ? u : 0][m] - av;
+                    T* rb = rbuf + ((zl >> 1) & 1) * (TX * TY) + lx + TX * ly;
+                    if (zl & 1) *rb += v;
+                    else *rb = v;
+                }
+            }
+            if (MODE == ZP_RESTRICT && (zl & 1)) {
+                // both fine planes of coarse plane zl / 2 are in rbuf: sum the 2 x 2 x 2 children
+                __syncthreads();
+                constexpr int CX = TX / 2, CY = TY / 2;
+                const int c0 = o.n0 >> 1, c1 = o.n1 >> 1;
+                const T* rb = rbuf + ((zl >> 1) & 1) * (TX * TY);
+                for (int t = tid; t < CX * CY; t += NT) {
+                    const int I = t % CX, J = t / CX;
+                    const int gI = (x0 >> 1) + I, gJ = (y0 >> 1) + J;
+                    const T* r0 = rb + 2 * I + TX * (2 * J);
+                    const T s = (r0[0] + r0[1]) + (r0[TX] + r0[TX + 1]);
+                    if (gI < c0 && gJ < c1)
+                        out0[(uint32_t)gI + (uint32_t)c0 * ((uint32_t)gJ + (uint32_t)c1 * (uint32_t)(zl >> 1))] =
+                            s * (T)0.125;
+                }
+            }
+            ctr_load(cbp[HASB ? u : 0], cqp[HASQ ? u : 0], zl + PF);
+        }
+    }
+    if (MODE == ZP_JAC_DOT3) {
+        s0 = gmg_block_sum(s0);
+        s1 = gmg_block_sum(s1);
+        s2 = gmg_block_sum(s2);
+        if (tid == 0) {
+            part[blockIdx.x] = s0;
+            part[GMG_MAXB + blockIdx.x] = s1;
+            part[2 * GMG_MAXB + blockIdx.x] = s2;
+        }
+    } else if (RED1) {
+        if (part) {
+            s0 = gmg_block_sum(s0);
+            if (tid == 0) part[blockIdx.x] = s0;
+        }
+    }
+}

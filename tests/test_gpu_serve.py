@@ -60,8 +60,8 @@ def test_served_results_meet_tolerance(ctx):
         assert r.t_done >= r.t_admit >= r.t_submit - 1e-9
 
 
-def test_alone_equals_in_mix(ctx):
-    target = ([128, 128], 2, 5)
+@pytest.mark.parametrize("target", [([128, 128], 2, 5), ([64, 64, 64], 9, 4)], ids=["128^2", "64^3"])
+def test_alone_equals_in_mix(ctx, target):
     alone, _ = _serve(ctx, [target])
     mixed, _ = _serve(ctx, MIX[:3], late=[target] + MIX[3:])
     k = 3  # first late request
