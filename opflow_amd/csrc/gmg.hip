@@ -1317,13 +1317,21 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
                                                       o, (const T*)cur, rhs, (T*)C.b, C.n[0], C.n[1], C.n[2], fc0,
                                                       fc1, bt));
     ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
-    if (H.dim == 3)
+    int p0 = 0;
+    if (zp && (H.post >= 2 || (H.post == 1 && !dot_q))) {
+        // the prolongation x += P x_c happens inside the first post-smoothing sweep
+        zp_launch<T, ZP_PJAC>(H, l, st, bt, (const T*)cur, (const T*)C.x, rhs, nullptr, tmp, (T*)nullptr, nullptr,
+                              C.N);
+        std::swap(cur, tmp);
+        p0 = 1;
+    } else if (H.dim == 3) {
         hipLaunchKernelGGL((k_prolong<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
                            C.n[2], o, bt);
-    else
+    } else {
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nblk(N), B), dim3(256), 0, st, cur, (const T*)C.x, C.n[0], C.n[1],
                            C.n[2], o, bt);
-    for (int s = 0; s < H.post; ++s) {
+    }
+    for (int s = p0; s < H.post; ++s) {
         if (dot_q && s == H.post - 1) {
             // last sweep also forms the partials of (r, z), (z, q), sum(z) (finest level only)
             const int G = red_count<T>(H);

@@ -26,6 +26,9 @@
 //   ZP_RESID    r = b - A x (optional store, precision TO), partial r^2
 //   ZP_RESTRICT coarse b = average over the 2 x 2 x 2 children of b - A x (even extents)
 //   ZP_APPLY    y = A x
+//   ZP_PJAC     prolongation fused into the first post-smoothing sweep: the ring holds
+//               x_f + P x_c (trilinear, coarse ghosts as the separate prolongation), and
+//               xo = that + (w / D)(b - A that); in1 is the coarse correction x_c
 
 template <typename T>
 struct ZpCfg;
@@ -45,7 +48,7 @@ struct ZpGrid {
     int blocks;  // tx * ty * nch
 };
 
-enum { ZP_JAC = 0, ZP_JAC_DOT3, ZP_JAC2Z, ZP_SPMV_P, ZP_RESID, ZP_RESTRICT, ZP_APPLY };
+enum { ZP_JAC = 0, ZP_JAC_DOT3, ZP_JAC2Z, ZP_SPMV_P, ZP_RESID, ZP_RESTRICT, ZP_APPLY, ZP_PJAC };
 
 template <typename T>
 __device__ __forceinline__ T zp_alpha(const GOp<T>& o, int d, int s) {
@@ -96,7 +99,9 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
     constexpr int RS = NT / TX;             // rows between a thread's output cells
     static_assert((TX * TY) % NT == 0 && NT % TX == 0 && TX % 2 == 0 && TY % 2 == 0, "tile shape");
     constexpr bool RING1 = MODE == ZP_SPMV_P;
-    constexpr bool SMOOTH = MODE == ZP_JAC || MODE == ZP_JAC_DOT3 || MODE == ZP_JAC2Z;
+    constexpr bool PJ = MODE == ZP_PJAC;
+    constexpr bool SMOOTH = MODE == ZP_JAC || MODE == ZP_JAC_DOT3 || MODE == ZP_JAC2Z || PJ;
+    static_assert(!PJ || PF % 2 == 0, "the coarse window shifts on odd planes: PF must be even");
     constexpr bool HASB = MODE != ZP_SPMV_P && MODE != ZP_APPLY;
     constexpr bool HASQ = MODE == ZP_JAC_DOT3;
     constexpr bool RED1 = MODE == ZP_SPMV_P || MODE == ZP_RESID;
@@ -106,6 +111,7 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
     BAT_SLOT(bt);
     in0 += vo;
     if (RING1) in1 += vo;
+    if (PJ) in1 += (long long)slot * cvs;
     if (HASB) cb += vo;
     if (HASQ) cq += vo;
     if (MODE == ZP_RESTRICT) out0 += (long long)slot * cvs;
@@ -144,6 +150,46 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
         if (MODE == ZP_JAC2Z) {
             dxe[m] = zp_dadj(o, 0, mx, o.n0) + zp_dadj(o, 1, my, o.n1);
             wde[m] = dxe[m] != (T)0 ? w / (o.dint - dxe[m]) : o.wdi;
+        }
+    }
+    // ZP_PJAC: coarse cells of each ring element's interpolation in x and y (parent, other
+    // neighbour; a non-periodic side's ghost is alpha * parent) and the weights of the others
+    const int cn0 = o.n0 >> 1, cn1 = o.n1 >> 1, cn2 = o.n2 >> 1;
+    uint32_t co[PJ ? NL : 1][4];
+    T gxe[PJ ? NL : 1], gye[PJ ? NL : 1];
+    if (PJ) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            int e = tid + m * NT;
+            if (e >= RP) e = 0;
+            T f = 1;
+            const int mx = zp_mirror(o, 0, x0 - 1 + e % RX, o.n0, f);
+            const int my = zp_mirror(o, 1, y0 - 1 + e / RX, o.n1, f);
+            int q[2][2];  // [dim][parent, other]
+            T gq[2];
+            const int mm[2] = {mx, my}, cn[2] = {cn0, cn1};
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int par = mm[d] >> 1;
+                int oth = (mm[d] & 1) ? par + 1 : par - 1;
+                T gf = (T)0.25;
+                if (oth < 0 || oth >= cn[d]) {
+                    if (o.per[d]) oth = oth < 0 ? oth + cn[d] : oth - cn[d];
+                    else {
+                        gf *= zp_alpha(o, d, oth < 0 ? 0 : 1);
+                        oth = par;
+                    }
+                }
+                q[d][0] = par;
+                q[d][1] = oth;
+                gq[d] = gf;
+            }
+            co[m][0] = (uint32_t)q[0][0] + (uint32_t)cn0 * (uint32_t)q[1][0];
+            co[m][1] = (uint32_t)q[0][1] + (uint32_t)cn0 * (uint32_t)q[1][0];
+            co[m][2] = (uint32_t)q[0][0] + (uint32_t)cn0 * (uint32_t)q[1][1];
+            co[m][3] = (uint32_t)q[0][1] + (uint32_t)cn0 * (uint32_t)q[1][1];
+            gxe[m] = gq[0];
+            gye[m] = gq[1];
         }
     }
     // output cells of this thread: (lx, ly0 + m * RS)
@@ -193,6 +239,38 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
             if (HASQ) rq[m] = cq[po + coff[m]];
         }
     };
+    // ZP_PJAC: x-y interpolated coarse planes Q(c) at the ring elements, for the pair
+    // (cA, cA + 1), cA = floor((zz - 1) / 2), that ring plane zz interpolates between
+    // in z; virtual coarse planes -1 and cn2 follow the coarse ghost rule. The raw
+    // values of the next plane are loaded two fine planes ahead.
+    T qa[PJ ? NL : 1], qb[PJ ? NL : 1], raw[PJ ? 4 : 1][PJ ? NL : 1];
+    T rawf = 1;
+    auto coarse_load = [&](int c) {
+        c = min(max(c, -1), cn2);
+        T f = 1;
+        const int mc = zp_mirror(o, 2, c, cn2, f);
+        rawf = f;
+        const T* p = in1 + (size_t)cn0 * (size_t)cn1 * (uint32_t)mc;
+#pragma unroll
+        for (int m = 0; m < NL; ++m)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) raw[k][m] = p[co[m][k]];
+    };
+    auto coarse_q = [&](T(&qo)[PJ ? NL : 1]) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const T ry0 = (T)0.75 * raw[0][m] + gxe[m] * raw[1][m];
+            const T ry1 = (T)0.75 * raw[2][m] + gxe[m] * raw[3][m];
+            qo[m] = rawf * ((T)0.75 * ry0 + gye[m] * ry1);
+        }
+    };
+    if (PJ) {
+        coarse_load(z0 / 2 - 1);
+        coarse_q(qa);
+        coarse_load(z0 / 2);
+        coarse_q(qb);
+        coarse_load(z0 / 2 + 1);
+    }
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         ring_load(pre[u], pre1[RING1 ? u : 0], fzp[u], mzp[u], min(zb + u, ze - 1));
@@ -222,6 +300,19 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
 #pragma unroll
                     for (int m = 0; m < NL; ++m)
                         rs[tid + m * NT] = (fac[m] * fz) * ((pre[u][m] - zmean) + beta * pre1[u][m]);
+                } else if (PJ) {
+                    if (u % 2 == 0 && zz > zb) {  // odd plane: the pair moves up by one coarse plane
+#pragma unroll
+                        for (int m = 0; m < NL; ++m) qa[m] = qb[m];
+                        coarse_q(qb);
+                        coarse_load((zz + 1) / 2 + 1);
+                    }
+                    // weights by the parity of the cell interpolated at (the mirror plane)
+                    const bool odd = mzp[u] & 1;
+                    const T wa = odd ? (T)0.75 : (T)0.25, wb = odd ? (T)0.25 : (T)0.75;
+#pragma unroll
+                    for (int m = 0; m < NL; ++m)
+                        rs[tid + m * NT] = (fac[m] * fz) * (pre[u][m] + (wa * qa[m] + wb * qb[m]));
                 } else {
 #pragma unroll
                     for (int m = 0; m < NL; ++m) rs[tid + m * NT] = (fac[m] * fz) * pre[u][m];
