@@ -244,7 +244,8 @@ typedef struct {
     int niter;        /* total Krylov iterations (inner iterations when mixed) */
     double relerr;    /* final ||b-Ax||_2 / ||b||_2 evaluated in FP64          */
     double abserr;    /* final ||b-Ax||_2                                      */
-    int nrefine;      /* outer iterative-refinement steps (mixed)              */
+    int nrefine;      /* FP64 true-residual evaluations (mixed: the start and
+                         each reliable update x += e, r = b - A x)           */
     int converged;
 } opf_solve_state;
 

@@ -741,6 +741,11 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int mode, doub
         case FIN_RR: s[S_RR] = v[0]; break;
         case FIN_BB: s[S_BB] = v[0]; break;
         case FIN_RR0: s[S_RR0] = v[0]; break;
+        case FIN_REP:
+            s[S_RR] = v[0];
+            s[S_RREF] = v[0];
+            break;
+        case FIN_MEAN: s[S_MEAN] = v[0]; break;
         case FIN_DOT3_FIRST:
             s[S_RZ] = v[0];
             s[S_ZMEAN] = singular ? v[2] * invN : 0.0;
@@ -859,6 +864,80 @@ __global__ void k_fill(long long N, T* __restrict__ x, T v, Bat bt) {
     x += vo;
     const uint32_t t = cell_id();
     if (t < (uint32_t)N) x[t] = v;
+}
+
+// reliable-update control (one thread per slot; see ph_rel_start / ph_rel_iter)
+__global__ void k_seti(int* __restrict__ p, int n, int v) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// slots in m_new, after ||b||^2 -> S_BB and the first true residual -> S_RR0
+__global__ void k_rel_init(int B, const int* __restrict__ m_new, int* __restrict__ m_it, int* __restrict__ m_start,
+                           double* __restrict__ scal, double tol2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    if (!m_new[k]) {
+        m_start[k] = 0;
+        return;
+    }
+    double* s = scal + (size_t)k * GMG_SSTRIDE;
+    const double bb = s[S_BB], rr = s[S_RR0];
+    s[S_RR] = rr;
+    s[S_RREF] = rr;
+    s[S_TGT] = tol2 * bb;
+    s[S_ITS] = 0.0;
+    s[S_NREP] = 1.0;
+    const int go = bb > 0.0 && rr > tol2 * bb && isfinite(rr);
+    s[S_DONE] = go ? 0.0 : 1.0;
+    m_it[k] = go;
+    m_start[k] = go;
+}
+
+// after an iteration's (r, r): count it; fold e into x and recompute the true residual
+// when the recurrence residual fell by delta2 since the last update, looks converged,
+// or the cap is reached (!(a >= b) also catches NaN)
+__global__ void k_rel_decide(int B, const int* __restrict__ m_it, int* __restrict__ m_rep, double* __restrict__ scal,
+                             double delta2, int maxit) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    if (!m_it[k]) {
+        m_rep[k] = 0;
+        return;
+    }
+    double* s = scal + (size_t)k * GMG_SSTRIDE;
+    const double its = s[S_ITS] + 1.0;
+    s[S_ITS] = its;
+    const double rr = s[S_RR];
+    const int rep = !(rr >= delta2 * s[S_RREF]) || rr <= s[S_TGT] || its >= (double)maxit;
+    m_rep[k] = rep;
+    if (rep) s[S_NREP] += 1.0;
+}
+
+// after the update's true residual: stop the slots that meet tol (or hit the cap / break down)
+__global__ void k_rel_done(int B, int* __restrict__ m_it, const int* __restrict__ m_rep, double* __restrict__ scal,
+                           int maxit) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B || !m_rep[k]) return;
+    double* s = scal + (size_t)k * GMG_SSTRIDE;
+    const double rr = s[S_RR];
+    if (rr <= s[S_TGT] || s[S_ITS] >= (double)maxit || !isfinite(rr)) {
+        m_it[k] = 0;
+        s[S_DONE] = 1.0;
+    }
+}
+
+// x (FP64) += e ; e = 0
+template <typename TE>
+__global__ void k_add_zero(long long N, double* __restrict__ x, TE* __restrict__ e, Bat bt) {
+    BAT_SLOT(bt);
+    x += vo;
+    e += vo;
+    const uint32_t t = cell_id();
+    if (t < (uint32_t)N) {
+        x[t] += (double)e[t];
+        e[t] = (TE)0;
+    }
 }
 
 // subtract a device scalar mean (singular problems): x -= s[slot] * scale
@@ -1199,6 +1278,8 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
     OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * GMG_SSTRIDE * H.B));
     OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * GMG_SSTRIDE * H.B));
     OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * GMG_SSTRIDE * H.B, hipHostMallocDefault));
+    OPF_HIP(hipMalloc((void**)&H.mk, sizeof(int) * MK_COUNT * H.B));
+    OPF_HIP(hipMemset(H.mk, 0, sizeof(int) * MK_COUNT * H.B));
     (void)ctx;
     return OPF_OK;
 }
@@ -1220,6 +1301,8 @@ void gmg_free(Hierarchy& H) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (H.hscal) hipHostFree(H.hscal);
+    if (H.mk) hipFree(H.mk);
+    H.mk = nullptr;
     H.cinv = nullptr;
     H.r = H.z = H.p[0] = H.p[1] = H.q = H.e = nullptr;
     H.r64 = H.part = H.scal = H.hscal = nullptr;
@@ -1441,21 +1524,77 @@ void ph_project(const Hierarchy& H, hipStream_t st, const int* mask, T* r) {
     const long long N = H.host[0].N;
     const Bat bt = bat(H, mask, N);
     hipLaunchKernelGGL(k_sum<T>, dim3(red_count<T>(H), (unsigned)H.B), dim3(256), 0, st, N, (const T*)r, H.part, bt);
-    ph_finish<T>(H, st, mask, FIN_RR0, 1);
-    hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, r, H.scal, (int)S_RR0,
+    ph_finish<T>(H, st, mask, FIN_MEAN, 1);
+    hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, r, H.scal, (int)S_MEAN,
                        1.0 / N, bt);
 }
 
-template <typename TE>
-void ph_add64(const Hierarchy& H, hipStream_t st, const int* mask, double* x, const TE* e) {
+template <typename T>
+static void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v) {
     const long long N = H.host[0].N;
-    hipLaunchKernelGGL(k_add_to64<TE>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, x, e, bat(H, mask, N));
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, v, (T)0, bat(H, mask, N));
+}
+
+static inline unsigned slot_blocks(const Hierarchy& H) { return (unsigned)((H.B + 63) / 64); }
+
+template <typename T>
+void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2) {
+    const int B = H.B;
+    const int* m_new = H.mk + MK_NEW * B;
+    const int* m_start = H.mk + MK_START * B;
+    ph_resid64<T>(H, st, m_new, nullptr, b64, (T*)nullptr, FIN_BB);
+    ph_resid64<T>(H, st, m_new, x64, b64, (T*)H.r, FIN_RR0);
+    if (H.singular) ph_project<T>(H, st, m_new, (T*)H.r);
+    hipLaunchKernelGGL(k_rel_init, dim3(slot_blocks(H)), dim3(64), 0, st, B, m_new, H.mk + MK_IT * B,
+                       H.mk + MK_START * B, H.scal, tol2);
+    ph_zero<T>(H, st, m_start, (T*)H.e);
+    ph_fcg_start<T>(H, st, m_start);
 }
 
 template <typename T>
-void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v) {
+void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, double delta2,
+                 int maxit) {
     const long long N = H.host[0].N;
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(N), (unsigned)H.B), dim3(256), 0, st, N, v, (T)0, bat(H, mask, N));
+    const int B = H.B;
+    int* m_it = H.mk + MK_IT * B;
+    int* m_rep = H.mk + MK_REP * B;
+    const int G = red_count<T>(H);
+    const Bat bt = bat(H, m_it, N);
+    GOp<T> o = make_op<T>(H, 0);
+    T* e = (T*)H.e;
+    T* z = (T*)H.z;
+    T* q = (T*)H.q;
+    T* r = (T*)H.r;
+    T* pold = (T*)H.p[pc];
+    T* pnew = (T*)H.p[1 - pc];
+    if (use_zp(H, 0))
+        zp_launch<T, ZP_SPMV_P>(H, 0, st, bt, (const T*)z, (const T*)pold, nullptr, nullptr, pnew, q, H.part);
+    else if (use_z(H, 0))
+        GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_spmv_p<T, D_>), dim3(G, B), dim3(256), 0, st, o, (const T*)z,
+                                                      (const T*)pold, pnew, q, H.scal, H.part, ztile(H.host[0]), bt));
+    else
+        GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o,
+                                                      (const T*)z, (const T*)pold, pnew, q, H.scal, H.part, bt));
+    ph_finish<T>(H, st, m_it, FIN_PQ, H.singular);
+    hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
+                       bt);
+    ph_finish<T>(H, st, m_it, FIN_RR, H.singular);
+    // reliable update: x += e, e = 0, r = b - A x in FP64 (the direction p is kept)
+    hipLaunchKernelGGL(k_rel_decide, dim3(slot_blocks(H)), dim3(64), 0, st, B, (const int*)m_it, m_rep, H.scal, delta2,
+                       maxit);
+    hipLaunchKernelGGL(k_add_zero<T>, dim3(nblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e, bat(H, m_rep, N));
+    ph_resid64<T>(H, st, m_rep, x64, b64, r, FIN_REP);
+    if (H.singular) ph_project<T>(H, st, m_rep, r);
+    hipLaunchKernelGGL(k_rel_done, dim3(slot_blocks(H)), dim3(64), 0, st, B, m_it, (const int*)m_rep, H.scal, maxit);
+    // next preconditioned residual for the slots still iterating
+    if (H.post > 0) {
+        ph_vcycle<T>(H, st, m_it, 0, (const T*)r, z, (const T*)q);
+    } else {
+        ph_vcycle<T>(H, st, m_it, 0, (const T*)r, z);
+        hipLaunchKernelGGL(k_dot3<T>, dim3(G, B), dim3(256), 0, st, N, (const T*)r, (const T*)z, (const T*)q, H.part,
+                           bt);
+    }
+    ph_finish<T>(H, st, m_it, FIN_DOT3, H.singular);
 }
 
 int read_scal(const Hierarchy& H, hipStream_t st) {
@@ -1512,32 +1651,29 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
         }
         nref = 1;
     } else {
-        // iterative refinement: FP64 residual / update, FP32 inner FCG
-        float* e = (float*)H.e;
-        bool first = true;
-        double target = 0, floor = 1e-5, rr_prev = 0, used = 0;
-        for (nref = 0; nref < 40; ++nref) {
-            ph_resid64<float>(H, s, nullptr, x, b, (float*)H.r, FIN_RR0);
-            OPF_TRY(read_scal(H, s));
-            if (first) {
-                bb = H.hscal[S_BB];
-                target = tol * tol * bb;
-                first = false;
+        // FP32 flexible CG with reliable updates: x (FP64) += e and r = b - A x (FP64)
+        // whenever the FP32 recurrence residual has dropped by 1e2 (or looks converged);
+        // the search direction continues across updates (no restart)
+        hipLaunchKernelGGL(k_seti, dim3(1), dim3(64), 0, s, H.mk + MK_NEW * H.B, H.B, 1);
+        ph_rel_start<float>(H, s, x, b, tol * tol);
+        OPF_TRY(read_scal(H, s));
+        bb = H.hscal[S_BB];
+        // stopped slots skip the remaining launches of an iteration, so big grids poll every iteration
+        const int check_every = H.host[0].N > (1 << 20) ? 1 : 2;
+        int pc = 0;
+        for (int it = 0; it < maxit && H.hscal[S_DONE] == 0.0; ++it) {
+            ph_rel_iter<float>(H, s, x, b, pc, rel_delta2<float>(), maxit);
+            pc = 1 - pc;
+            if ((it + 1) % check_every == 0 || it + 1 == maxit) {
+                OPF_TRY(read_scal(H, s));
+                if (verbose)
+                    fprintf(stderr, "[gmg mixed] it %d updates %d relres %.3e\n", (int)H.hscal[S_ITS],
+                            (int)H.hscal[S_NREP], std::sqrt(H.hscal[S_RR] / bb));
             }
-            double rr0 = H.hscal[S_RR0];
-            if (verbose) fprintf(stderr, "[gmg mixed] refine %d relres %.3e\n", nref, std::sqrt(rr0 / bb));
-            if (!(rr0 > target) || bb == 0 || total >= maxit) break;
-            if (nref > 0) floor = next_floor(floor, rr_prev, rr0, used);
-            rr_prev = rr0;
-            if (H.singular) ph_project<float>(H, s, nullptr, (float*)H.r);
-            ph_zero<float>(H, s, nullptr, e);
-            int its = 0;
-            double rrf = 0;
-            used = inner_target(target, rr0, floor);
-            OPF_TRY(fcg<float>(H, s, e, rr0, used, std::max(1, maxit - total), &its, &rrf));
-            total += its;
-            ph_add64<float>(H, s, nullptr, x, (const float*)e);
         }
+        OPF_TRY(read_scal(H, s));
+        total = (int)H.hscal[S_ITS];
+        nref = (int)H.hscal[S_NREP];
     }
     // final true residual in FP64
     ph_resid64<double>(H, s, nullptr, x, b, H.r64, FIN_RR);
@@ -1570,17 +1706,10 @@ int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y) {
 }
 
 // explicit instantiations used by the serving loop
-template void ph_fcg_start<float>(const Hierarchy&, hipStream_t, const int*);
-template void ph_fcg_start<double>(const Hierarchy&, hipStream_t, const int*);
-template void ph_fcg_iter<float>(const Hierarchy&, hipStream_t, const int*, float*, int);
-template void ph_fcg_iter<double>(const Hierarchy&, hipStream_t, const int*, double*, int);
-template void ph_resid64<float>(const Hierarchy&, hipStream_t, const int*, const double*, const double*, float*, int);
-template void ph_resid64<double>(const Hierarchy&, hipStream_t, const int*, const double*, const double*, double*,
-                                 int);
-template void ph_add64<float>(const Hierarchy&, hipStream_t, const int*, double*, const float*);
-template void ph_add64<double>(const Hierarchy&, hipStream_t, const int*, double*, const double*);
-template void ph_zero<float>(const Hierarchy&, hipStream_t, const int*, float*);
-template void ph_zero<double>(const Hierarchy&, hipStream_t, const int*, double*);
+template void ph_rel_start<float>(const Hierarchy&, hipStream_t, double*, const double*, double);
+template void ph_rel_start<double>(const Hierarchy&, hipStream_t, double*, const double*, double);
+template void ph_rel_iter<float>(const Hierarchy&, hipStream_t, double*, const double*, int, double, int);
+template void ph_rel_iter<double>(const Hierarchy&, hipStream_t, double*, const double*, int, double, int);
 
 }  // namespace opf
 

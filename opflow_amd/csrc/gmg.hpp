@@ -88,39 +88,48 @@ struct Hierarchy {
     double* part = nullptr; // reduction partials [4][GMG_MAXB] per slot
     double* scal = nullptr; // device scalars
     double* hscal = nullptr;  // pinned host mirror
+    int* mk = nullptr;        // device slot masks [MK_COUNT][B]
     size_t bytes = 0;
 };
 
 int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
               int precision, int slots = 1);
 int read_scal(const Hierarchy& H, hipStream_t st);
-// Inner (FP32) reduction target of a refinement: what is still needed, with a
-// margin, but not below `floor`, the reduction FP32 can deliver. floor starts
-// at 1e-5 and follows the true reductions the refinements achieve (an
-// ill-conditioned operator limits an FP32 correction to fewer digits).
-inline double inner_target(double target, double rr0, double floor) {
-    double v = std::sqrt(target / rr0) * 0.5;
-    return std::min(0.5, std::max(floor, v));
+// Krylov solve with reliable updates, batched over the slots of H (vectors slot-major).
+// The Krylov vectors and the V-cycle are in precision T; x64 / b64 are the FP64
+// solution and right-hand side. Start: for the slots in mk[MK_NEW], r = b - A x
+// (FP64, stored in T), ||b||^2, and the first preconditioned direction; slots
+// already within tolerance stop at once (S_DONE). Iteration: one flexible-CG step
+// for the slots in mk[MK_IT]; the device folds the correction e into x64 and
+// recomputes the true residual (keeping the search direction) whenever the
+// recurrence residual has fallen by delta^2 since the last such update, or looks
+// converged, or the iteration cap is reached; a slot whose true residual meets
+// tol stops (mk[MK_IT] cleared, S_DONE set) without a host round trip.
+template <typename T>
+void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2);
+template <typename T>
+void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, double delta2,
+                 int maxit);
+// reliable-update threshold (squared) for Krylov precision T
+template <typename T>
+inline double rel_delta2() {
+    return sizeof(T) == 4 ? 1e-4 : 0.0;
 }
-inline double next_floor(double floor, double rr0_prev, double rr0_new, double used) {
-    const double got = std::sqrt(rr0_new / rr0_prev);  // true reduction of the last refinement
-    if (got > 10.0 * used) floor = std::max(floor, std::min(0.1, 0.5 * got));
-    return floor;
-}
-// batched phases over all slots selected by a device mask (null: all slots)
-template <typename T>
-void ph_fcg_start(const Hierarchy& H, hipStream_t st, const int* mask);
-template <typename T>
-void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int pc);
-template <typename TO>
-void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const double* x, const double* b, TO* r,
-                int mode);
-template <typename TE>
-void ph_add64(const Hierarchy& H, hipStream_t st, const int* mask, double* x, const TE* e);
-template <typename T>
-void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v);
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_RR, S_BETA, S_ZQ, S_ZMEAN, S_BB, S_RR0, S_N };
-enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0 };
+// per-slot device scalars (GMG_SSTRIDE doubles per slot)
+enum {
+    S_RZ = 0, S_PQ, S_ALPHA, S_RR, S_BETA, S_ZQ, S_ZMEAN, S_BB, S_RR0,
+    S_RREF,  // true residual^2 at the last reliable update
+    S_TGT,   // tol^2 * ||b||^2
+    S_ITS,   // Krylov iterations done
+    S_NREP,  // true-residual evaluations (start + reliable updates)
+    S_DONE,  // 1 once the slot has stopped (converged, iteration cap or breakdown)
+    S_MEAN,  // scratch mean (singular projections)
+    S_N
+};
+static_assert(S_N <= GMG_SSTRIDE, "scalars per slot");
+enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0, FIN_REP, FIN_MEAN };
+// device masks per slot, rows of B ints (Hierarchy::mk)
+enum { MK_NEW = 0, MK_IT, MK_REP, MK_START, MK_COUNT };
 void gmg_free(Hierarchy& H);
 // solve A x = b on the stream: b, x FP64 device arrays (N each). Returns iterations etc.
 int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,

@@ -9,15 +9,18 @@
 // and are reused by later requests of the same shape; an empty group is
 // evicted only when a new shape needs its memory.
 //
-// Each slot runs FP32-inner / FP64-refinement: an FP64 residual r = b - A x,
-// an FP32 flexible-CG inner solve on A e = r, x += e. One scheduling round
+// Each slot runs FP32 flexible CG with FP64 reliable updates: the FP64
+// solution x and residual r = b - A x are refreshed on the device whenever the
+// FP32 recurrence residual has dropped by a fixed factor (the Krylov direction
+// continues), and the device stops a slot once its true FP64 residual meets
+// the tolerance. One scheduling round
 //   1. admits queued requests into free slots while the memory budget allows,
-//   2. runs the FP64 outer step for the slots that need it (batched) and, on
-//      the device, starts an inner solve for every slot still above tolerance,
-//   3. replays the group's captured hipGraph of two batched inner iterations
-//      (1-3 times by problem size) for iterating and just-started slots,
+//   2. starts the newly admitted slots of each group (batched: ||b||, the true
+//      residual, the first preconditioned direction),
+//   3. replays the group's captured hipGraph of two batched iterations (1-3
+//      times by problem size); stopped slots skip the work inside the launches,
 //   4. reads every slot's scalars with one copy per group, after one wait for
-//      all groups, and advances the state machines.
+//      all groups, and retires the slots the device has stopped.
 // Requests that arrive while others run are admitted in the next round, so
 // their work overlaps on the device with requests already in flight.
 #include <algorithm>
@@ -34,8 +37,7 @@ using namespace opf;
 
 namespace {
 
-enum Phase { FREE = 0, NEWREQ, OUTER, ITER };
-enum { M_NEW = 0, M_ADD, M_OUTER, M_ALLOW, M_START, M_ITER, M_COUNT };
+enum Phase { FREE = 0, NEWREQ, ITER };
 
 struct Pending {
     uint64_t id;
@@ -48,10 +50,6 @@ struct Slot {
     int phase = FREE;
     uint64_t id = 0;
     double t_submit = 0, t_admit = 0;
-    double bb = 0, target = 0, rr0 = 0;
-    double floor = 1e-5, used = 0;  // inner reduction floor and the last inner target
-    int its = 0, nref = 0, inner_its = 0;
-    bool pending_add = false;
     double* out_x = nullptr;
 };
 
@@ -62,8 +60,7 @@ struct Group {
     long long N = 0;
     double* x64 = nullptr;
     double* b64 = nullptr;
-    int* dmask = nullptr;   // [M_COUNT][B]
-    int* hmask = nullptr;   // pinned mirror
+    int* hnew = nullptr;    // pinned row of new-slot flags, copied to H.mk[MK_NEW]
     std::vector<Slot> slots;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -121,8 +118,7 @@ void free_group(Group* g) {
     gmg_free(g->H);
     if (g->x64) hipFree(g->x64);
     if (g->b64) hipFree(g->b64);
-    if (g->dmask) hipFree(g->dmask);
-    if (g->hmask) hipHostFree(g->hmask);
+    if (g->hnew) hipHostFree(g->hnew);
     if (g->st) hipStreamDestroy(g->st);
 }
 
@@ -170,8 +166,7 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     }
     const size_t BN = (size_t)g->B * g->N;
     if (hipMalloc(&g->x64, 8 * BN) != hipSuccess || hipMalloc(&g->b64, 8 * BN) != hipSuccess ||
-        hipMalloc(&g->dmask, sizeof(int) * M_COUNT * g->B) != hipSuccess ||
-        hipHostMalloc(&g->hmask, sizeof(int) * M_COUNT * g->B, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&g->hnew, sizeof(int) * g->B, hipHostMallocDefault) != hipSuccess) {
         free_group(g.get());
         set_error("serve: out of device memory");
         return OPF_ERR_NOMEM;
@@ -192,19 +187,24 @@ int make_group(opf_srv* s, const opf_poisson_desc& d, Group** out) {
     return OPF_OK;
 }
 
-// capture two batched inner iterations (the p ping-pong returns to its start)
+int max_iter(const opf_srv* s) { return s->cfg.max_iter > 0 ? s->cfg.max_iter : 200; }
+
+// two batched iterations (the p ping-pong returns to its start)
+void iterate2(opf_srv* s, Group* g, hipStream_t st) {
+    for (int pc = 0; pc < 2; ++pc) {
+        if (g->H.precision == OPF_PREC_MIXED)
+            ph_rel_iter<float>(g->H, st, g->x64, g->b64, pc, rel_delta2<float>(), max_iter(s));
+        else
+            ph_rel_iter<double>(g->H, st, g->x64, g->b64, pc, rel_delta2<double>(), max_iter(s));
+    }
+}
+
+// capture iterate2 as the group's graph
 int capture(opf_srv* s, Group* g) {
     if (!s->cfg.batched || g->exec) return OPF_OK;
     hipStream_t st = g->st;
-    const int* m = g->dmask + M_ITER * g->B;
     OPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    if (g->H.precision == OPF_PREC_MIXED) {
-        ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, 0);
-        ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, 1);
-    } else {
-        ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, 0);
-        ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, 1);
-    }
+    iterate2(s, g, st);
     OPF_HIP(hipStreamEndCapture(st, &g->graph));
     OPF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
     return OPF_OK;
@@ -274,157 +274,68 @@ int try_admit(opf_srv* s) {
     return OPF_OK;
 }
 
-void set_masks(Group* g, int maxit) {
-    for (int m = 0; m < M_COUNT; ++m)
-        for (int k = 0; k < g->B; ++k) g->hmask[m * g->B + k] = 0;
-    for (int k = 0; k < g->B; ++k) {
-        const Slot& sl = g->slots[k];
-        if (sl.phase == NEWREQ) {
-            g->hmask[M_NEW * g->B + k] = 1;
-            g->hmask[M_OUTER * g->B + k] = 1;
-        }
-        if (sl.phase == OUTER) {
-            g->hmask[M_OUTER * g->B + k] = 1;
-            if (sl.pending_add) g->hmask[M_ADD * g->B + k] = 1;
-        }
-        if (sl.phase == NEWREQ || sl.phase == OUTER) g->hmask[M_ALLOW * g->B + k] = sl.its < maxit && sl.nref < 40;
-        if (sl.phase == ITER) g->hmask[M_ITER * g->B + k] = 1;
-    }
-}
-
-// The outer step's decision for every slot of a group, on the device (so the
-// inner solve can start in the same round): a slot whose FP64 residual is still
-// above tol^2 * ||b||^2 (and that may iterate) starts an inner solve and joins
-// this round's iterations. The host repeats the same test on the same scalars.
-__global__ void k_decide(int B, const int* __restrict__ m_outer, const int* __restrict__ m_allow,
-                         int* __restrict__ m_start, int* __restrict__ m_iter, const double* __restrict__ scal,
-                         double tol2) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= B) return;
-    const double* sc = scal + (size_t)k * GMG_SSTRIDE;
-    int st = 0;
-    if (m_outer[k] && m_allow[k]) {
-        const double bb = sc[S_BB];
-        st = bb > 0 && sc[S_RR0] > tol2 * bb;
-    }
-    m_start[k] = st;
-    if (st) m_iter[k] = 1;
-}
-
 int round(opf_srv* s, int* nfinished) {
     *nfinished = 0;
     OPF_TRY(try_admit(s));
     const double tol = s->cfg.tol > 0 ? s->cfg.tol : 1e-8;
-    const double tol2 = tol * tol;
-    const int maxit = s->cfg.max_iter > 0 ? s->cfg.max_iter : 200;
     // ---- enqueue one round for every group with work, each on its own stream:
-    // FP64 outer step (NEW / OUTER slots) -> device decision -> inner starts ->
-    // graph replays for iterating and just-started slots -> scalars to the host
+    // start the new slots -> graph replays -> scalars to the host
     std::vector<Group*> active;
     for (auto& gp : s->groups) {
         Group* g = gp.get();
-        bool outer = false, iter = false;
-        for (auto& sl : g->slots) {
-            outer |= sl.phase == NEWREQ || sl.phase == OUTER;
-            iter |= sl.phase == ITER;
+        bool fresh = false, busy = false;
+        for (int k = 0; k < g->B; ++k) {
+            g->hnew[k] = g->slots[k].phase == NEWREQ;
+            fresh |= g->hnew[k] != 0;
+            busy |= g->slots[k].phase != FREE;
         }
-        if (!outer && !iter) continue;
+        if (!busy) continue;
         active.push_back(g);
         hipStream_t st = g->st;
-        const int B = g->B;
-        set_masks(g, maxit);
-        OPF_HIP(hipMemcpyAsync(g->dmask, g->hmask, sizeof(int) * M_COUNT * B, hipMemcpyHostToDevice, st));
-        const int* mk = g->dmask;
-        if (outer) {
-            ph_resid64<double>(g->H, st, mk + M_NEW * B, nullptr, g->b64, (double*)nullptr, FIN_BB);
-            if (g->H.precision == OPF_PREC_MIXED) {
-                ph_add64<float>(g->H, st, mk + M_ADD * B, g->x64, (const float*)g->H.e);
-                ph_resid64<float>(g->H, st, mk + M_OUTER * B, g->x64, g->b64, (float*)g->H.r, FIN_RR0);
-            } else {
-                ph_add64<double>(g->H, st, mk + M_ADD * B, g->x64, (const double*)g->H.e);
-                ph_resid64<double>(g->H, st, mk + M_OUTER * B, g->x64, g->b64, (double*)g->H.r, FIN_RR0);
-            }
-            hipLaunchKernelGGL(k_decide, dim3((B + 63) / 64), dim3(64), 0, st, B, mk + M_OUTER * B, mk + M_ALLOW * B,
-                               g->dmask + M_START * B, g->dmask + M_ITER * B, (const double*)g->H.scal, tol2);
-            const int* m = g->dmask + M_START * B;
-            if (g->H.precision == OPF_PREC_MIXED) {
-                ph_zero<float>(g->H, st, m, (float*)g->H.e);
-                ph_fcg_start<float>(g->H, st, m);
-            } else {
-                ph_zero<double>(g->H, st, m, (double*)g->H.e);
-                ph_fcg_start<double>(g->H, st, m);
-            }
+        if (fresh) {
+            OPF_HIP(hipMemcpyAsync(g->H.mk + MK_NEW * g->B, g->hnew, sizeof(int) * g->B, hipMemcpyHostToDevice, st));
+            if (g->H.precision == OPF_PREC_MIXED) ph_rel_start<float>(g->H, st, g->x64, g->b64, tol * tol);
+            else ph_rel_start<double>(g->H, st, g->x64, g->b64, tol * tol);
         }
         for (int r = 0; r < g->replays; ++r) {
-            if (s->cfg.batched) {
-                OPF_HIP(hipGraphLaunch(g->exec, st));
-            } else {
-                const int* m = g->dmask + M_ITER * B;
-                for (int pc = 0; pc < 2; ++pc) {
-                    if (g->H.precision == OPF_PREC_MIXED) ph_fcg_iter<float>(g->H, st, m, (float*)g->H.e, pc);
-                    else ph_fcg_iter<double>(g->H, st, m, (double*)g->H.e, pc);
-                }
-            }
+            if (s->cfg.batched) OPF_HIP(hipGraphLaunch(g->exec, st));
+            else iterate2(s, g, st);
         }
-        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * B, hipMemcpyDeviceToHost, st));
+        OPF_HIP(hipMemcpyAsync(g->H.hscal, g->H.scal, sizeof(double) * GMG_SSTRIDE * g->B, hipMemcpyDeviceToHost, st));
     }
     for (Group* g : active) OPF_HIP(hipStreamSynchronize(g->st));
-    // ---- advance every slot's state machine from the scalars
+    // ---- retire the slots the device has stopped
     for (Group* g : active) {
         hipStream_t st = g->st;
         for (int k = 0; k < g->B; ++k) {
             Slot& sl = g->slots[k];
             if (sl.phase == FREE) continue;
             const double* sc = g->H.hscal + (size_t)k * GMG_SSTRIDE;
-            if (sl.phase == NEWREQ || sl.phase == OUTER) {
-                if (sl.phase == NEWREQ) {
-                    sl.bb = sc[S_BB];
-                    sl.target = tol2 * sl.bb;
-                } else {
-                    sl.floor = next_floor(sl.floor, sl.rr0, sc[S_RR0], sl.used);
-                }
-                sl.rr0 = sc[S_RR0];
-                sl.pending_add = false;
-                const bool allow = sl.its < maxit && sl.nref < 40;
-                const bool start = allow && sl.bb > 0 && sl.rr0 > tol2 * sl.bb;  // k_decide's test
-                if (!start) {
-                    // finished: report, copy out (completes before the result is polled), free the slot
-                    opf_srv_result r{};
-                    r.id = sl.id;
-                    r.status = OPF_OK;
-                    r.state.niter = sl.its;
-                    r.state.nrefine = sl.nref;
-                    r.state.abserr = std::sqrt(sl.rr0);
-                    r.state.relerr = sl.bb > 0 ? std::sqrt(sl.rr0 / sl.bb) : 0.0;
-                    r.state.converged = r.state.relerr <= tol * (1 + 1e-12) || sl.bb == 0;
-                    r.t_submit = sl.t_submit;
-                    r.t_admit = sl.t_admit;
-                    if (sl.out_x)
-                        OPF_HIP(hipMemcpyAsync(sl.out_x, g->x64 + (size_t)k * g->N, 8 * g->N, hipMemcpyDeviceToHost,
-                                               st));
-                    r.t_done = s->now();
-                    s->done.push_back(r);
-                    s->done_streams.push_back(sl.out_x ? st : nullptr);
-                    sl.phase = FREE;
-                    --s->inflight;
-                    ++*nfinished;
-                    g->last_used = s->now();
-                    continue;
-                }
-                sl.phase = ITER;  // started on the device and iterated this round
-                sl.nref++;
-                sl.inner_its = 0;
-                sl.used = inner_target(sl.target, sl.rr0, sl.floor);
+            if (sc[S_DONE] == 0.0) {
+                sl.phase = ITER;
+                continue;
             }
-            // ITER (including slots started this round): the round's replays ran
-            sl.its += 2 * g->replays;
-            sl.inner_its += 2 * g->replays;
-            const double rr = sc[S_RR];
-            const double inner = sl.used;  // FP32 inner reduction target of this refinement
-            if (!(rr > inner * inner * sl.rr0) || !std::isfinite(rr) || sl.inner_its >= 40 || sl.its >= maxit) {
-                sl.phase = OUTER;
-                sl.pending_add = true;
-            }
+            // finished: report, copy out (completes before the result is polled), free the slot
+            const double bb = sc[S_BB], rr = sc[S_RR];
+            opf_srv_result r{};
+            r.id = sl.id;
+            r.status = OPF_OK;
+            r.state.niter = (int)sc[S_ITS];
+            r.state.nrefine = (int)sc[S_NREP];
+            r.state.abserr = std::sqrt(rr);
+            r.state.relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            r.state.converged = r.state.relerr <= tol * (1 + 1e-12) || bb == 0;
+            r.t_submit = sl.t_submit;
+            r.t_admit = sl.t_admit;
+            if (sl.out_x)
+                OPF_HIP(hipMemcpyAsync(sl.out_x, g->x64 + (size_t)k * g->N, 8 * g->N, hipMemcpyDeviceToHost, st));
+            r.t_done = s->now();
+            s->done.push_back(r);
+            s->done_streams.push_back(sl.out_x ? st : nullptr);
+            sl.phase = FREE;
+            --s->inflight;
+            ++*nfinished;
+            g->last_used = s->now();
         }
     }
     OPF_HIP(hipGetLastError());
