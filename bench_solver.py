@@ -172,12 +172,14 @@ def cfg5(of, ctx, ev, count=48):
     tol = 1e-8
     reqs = _requests(count, 0xC0FFEE + 5)
     shapes = sorted({tuple(n) for n, _, _ in reqs})
-    warm = [(list(n), 1, 1) for n in shapes]
-    # both servers keep per-shape state resident; warm each with one request per shape
-    srv = of.Server(ctx, 64 << 30, max_inflight=16, precision="mixed", batched=True, tol=tol)
-    _serve_run(of, srv, warm, tol)
+    # admission by memory footprint (64 GiB budget); the in-flight cap is above the request count
+    srv = of.Server(ctx, 64 << 30, max_inflight=64, precision="mixed", batched=True, tol=tol)
     base = _Fp64Server(of, ctx)
-    base.run(warm, tol)
+    # steady state: both servers are warmed with one pass of the same request list, so the
+    # state they keep resident (groups / per-shape solvers) exists before timing (nothing
+    # else carries over: every request regenerates its RHS and starts from x = 0)
+    _serve_run(of, srv, reqs, tol)
+    base.run(reqs, tol)
     t_srv, _ = _serve_run(of, srv, reqs, tol)
     t_64, _ = base.run(reqs, tol)
     # open loop: Poisson arrivals at a rate that keeps the FP64 one-at-a-time server at ~85% utilisation
@@ -191,6 +193,7 @@ def cfg5(of, ctx, ev, count=48):
     pct = lambda v, q: float(np.percentile(np.asarray(v), q))
     return {"served_poisson_mixed_sizes_relres1e-8": {
         "requests": count, "shapes": [list(n) for n in shapes], "arrival_rate_per_s": lam,
+        "server": {"mem_budget_GiB": 64, "max_inflight": 64, "warm_up": "one untimed pass of the same request list (both servers)"},
         "rhs": "seeded bumps generated on the device at admission (both servers)",
         "served_mixed_batched": {"solves_per_s": count / t_srv, "p50_s": pct(lat_srv, 50), "p99_s": pct(lat_srv, 99)},
         "fp64_one_at_a_time": {"solves_per_s": count / t_64, "p50_s": pct(lat_64, 50), "p99_s": pct(lat_64, 99)},
