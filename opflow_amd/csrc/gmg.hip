@@ -874,7 +874,7 @@ __global__ void k_seti(int* __restrict__ p, int n, int v) {
 
 // slots in m_new, after ||b||^2 -> S_BB and the first true residual -> S_RR0
 __global__ void k_rel_init(int B, const int* __restrict__ m_new, int* __restrict__ m_it, int* __restrict__ m_start,
-                           double* __restrict__ scal, double tol2) {
+                           double* __restrict__ scal, double tol2, double delta2, int x_zero) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= B) return;
     if (!m_new[k]) {
@@ -882,7 +882,10 @@ __global__ void k_rel_init(int B, const int* __restrict__ m_new, int* __restrict
         return;
     }
     double* s = scal + (size_t)k * GMG_SSTRIDE;
-    const double bb = s[S_BB], rr = s[S_RR0];
+    const double bb = s[S_BB];
+    if (x_zero) s[S_RR0] = bb;
+    const double rr = s[S_RR0];
+    s[S_DLT2] = delta2;
     s[S_RR] = rr;
     s[S_RREF] = rr;
     s[S_TGT] = tol2 * bb;
@@ -898,7 +901,7 @@ __global__ void k_rel_init(int B, const int* __restrict__ m_new, int* __restrict
 // when the recurrence residual fell by delta2 since the last update, looks converged,
 // or the cap is reached (!(a >= b) also catches NaN)
 __global__ void k_rel_decide(int B, const int* __restrict__ m_it, int* __restrict__ m_rep, double* __restrict__ scal,
-                             double delta2, int maxit) {
+                             int maxit) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= B) return;
     if (!m_it[k]) {
@@ -909,9 +912,12 @@ __global__ void k_rel_decide(int B, const int* __restrict__ m_it, int* __restric
     const double its = s[S_ITS] + 1.0;
     s[S_ITS] = its;
     const double rr = s[S_RR];
-    const int rep = !(rr >= delta2 * s[S_RREF]) || rr <= s[S_TGT] || its >= (double)maxit;
+    const int rep = !(rr >= s[S_DLT2] * s[S_RREF]) || rr <= s[S_TGT] || its >= (double)maxit;
     m_rep[k] = rep;
-    if (rep) s[S_NREP] += 1.0;
+    if (rep) {
+        s[S_NREP] += 1.0;
+        s[S_RREC] = rr;
+    }
 }
 
 // after the update's true residual: stop the slots that meet tol (or hit the cap / break down)
@@ -924,6 +930,14 @@ __global__ void k_rel_done(int B, int* __restrict__ m_it, const int* __restrict_
     if (rr <= s[S_TGT] || s[S_ITS] >= (double)maxit || !isfinite(rr)) {
         m_it[k] = 0;
         s[S_DONE] = 1.0;
+        return;
+    }
+    // drift of the recurrence residual since the last update -> next update threshold
+    const double d2 = s[S_DLT2];
+    if (d2 > 0.0) {
+        const double drift = rr / s[S_RREC];
+        if (drift < 4.0) s[S_DLT2] = fmax(d2 * 1e-2, 1e-12);
+        else if (drift > 100.0) s[S_DLT2] = fmin(d2 * 1e2, 1e-2);
     }
 }
 
@@ -1538,22 +1552,26 @@ static void ph_zero(const Hierarchy& H, hipStream_t st, const int* mask, T* v) {
 static inline unsigned slot_blocks(const Hierarchy& H) { return (unsigned)((H.B + 63) / 64); }
 
 template <typename T>
-void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2) {
+void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2, double delta2,
+                  bool x_zero) {
     const int B = H.B;
     const int* m_new = H.mk + MK_NEW * B;
     const int* m_start = H.mk + MK_START * B;
-    ph_resid64<T>(H, st, m_new, nullptr, b64, (T*)nullptr, FIN_BB);
-    ph_resid64<T>(H, st, m_new, x64, b64, (T*)H.r, FIN_RR0);
+    if (x_zero) {
+        ph_resid64<T>(H, st, m_new, nullptr, b64, (T*)H.r, FIN_BB);  // r = b, ||b||^2
+    } else {
+        ph_resid64<T>(H, st, m_new, nullptr, b64, (T*)nullptr, FIN_BB);
+        ph_resid64<T>(H, st, m_new, x64, b64, (T*)H.r, FIN_RR0);
+    }
     if (H.singular) ph_project<T>(H, st, m_new, (T*)H.r);
     hipLaunchKernelGGL(k_rel_init, dim3(slot_blocks(H)), dim3(64), 0, st, B, m_new, H.mk + MK_IT * B,
-                       H.mk + MK_START * B, H.scal, tol2);
+                       H.mk + MK_START * B, H.scal, tol2, delta2, (int)x_zero);
     ph_zero<T>(H, st, m_start, (T*)H.e);
     ph_fcg_start<T>(H, st, m_start);
 }
 
 template <typename T>
-void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, double delta2,
-                 int maxit) {
+void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, int maxit) {
     const long long N = H.host[0].N;
     const int B = H.B;
     int* m_it = H.mk + MK_IT * B;
@@ -1580,8 +1598,7 @@ void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* 
                        bt);
     ph_finish<T>(H, st, m_it, FIN_RR, H.singular);
     // reliable update: x += e, e = 0, r = b - A x in FP64 (the direction p is kept)
-    hipLaunchKernelGGL(k_rel_decide, dim3(slot_blocks(H)), dim3(64), 0, st, B, (const int*)m_it, m_rep, H.scal, delta2,
-                       maxit);
+    hipLaunchKernelGGL(k_rel_decide, dim3(slot_blocks(H)), dim3(64), 0, st, B, (const int*)m_it, m_rep, H.scal, maxit);
     hipLaunchKernelGGL(k_add_zero<T>, dim3(nblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e, bat(H, m_rep, N));
     ph_resid64<T>(H, st, m_rep, x64, b64, r, FIN_REP);
     if (H.singular) ph_project<T>(H, st, m_rep, r);
@@ -1655,14 +1672,14 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
         // whenever the FP32 recurrence residual has dropped by 1e2 (or looks converged);
         // the search direction continues across updates (no restart)
         hipLaunchKernelGGL(k_seti, dim3(1), dim3(64), 0, s, H.mk + MK_NEW * H.B, H.B, 1);
-        ph_rel_start<float>(H, s, x, b, tol * tol);
+        ph_rel_start<float>(H, s, x, b, tol * tol, rel_delta2<float>(), false);
         OPF_TRY(read_scal(H, s));
         bb = H.hscal[S_BB];
         // stopped slots skip the remaining launches of an iteration, so big grids poll every iteration
         const int check_every = H.host[0].N > (1 << 20) ? 1 : 2;
         int pc = 0;
         for (int it = 0; it < maxit && H.hscal[S_DONE] == 0.0; ++it) {
-            ph_rel_iter<float>(H, s, x, b, pc, rel_delta2<float>(), maxit);
+            ph_rel_iter<float>(H, s, x, b, pc, maxit);
             pc = 1 - pc;
             if ((it + 1) % check_every == 0 || it + 1 == maxit) {
                 OPF_TRY(read_scal(H, s));
@@ -1706,10 +1723,10 @@ int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y) {
 }
 
 // explicit instantiations used by the serving loop
-template void ph_rel_start<float>(const Hierarchy&, hipStream_t, double*, const double*, double);
-template void ph_rel_start<double>(const Hierarchy&, hipStream_t, double*, const double*, double);
-template void ph_rel_iter<float>(const Hierarchy&, hipStream_t, double*, const double*, int, double, int);
-template void ph_rel_iter<double>(const Hierarchy&, hipStream_t, double*, const double*, int, double, int);
+template void ph_rel_start<float>(const Hierarchy&, hipStream_t, double*, const double*, double, double, bool);
+template void ph_rel_start<double>(const Hierarchy&, hipStream_t, double*, const double*, double, double, bool);
+template void ph_rel_iter<float>(const Hierarchy&, hipStream_t, double*, const double*, int, int);
+template void ph_rel_iter<double>(const Hierarchy&, hipStream_t, double*, const double*, int, int);
 
 }  // namespace opf
 

@@ -20,7 +20,7 @@ namespace opf {
 
 constexpr int GMG_MAXB = 16384;  // max blocks of a reduction kernel (partials per value)
 constexpr int GMG_RBLK = 2048;   // blocks of a one-dimensional (streaming) reduction launch
-constexpr int GMG_SSTRIDE = 16;            // scalars per slot
+constexpr int GMG_SSTRIDE = 24;            // scalars per slot
 constexpr int GMG_PSTRIDE = 4 * GMG_MAXB;  // reduction partials per slot
 
 // Operator of one level as seen by the kernels (T = float or double).
@@ -99,18 +99,23 @@ int read_scal(const Hierarchy& H, hipStream_t st);
 // The Krylov vectors and the V-cycle are in precision T; x64 / b64 are the FP64
 // solution and right-hand side. Start: for the slots in mk[MK_NEW], r = b - A x
 // (FP64, stored in T), ||b||^2, and the first preconditioned direction; slots
-// already within tolerance stop at once (S_DONE). Iteration: one flexible-CG step
-// for the slots in mk[MK_IT]; the device folds the correction e into x64 and
-// recomputes the true residual (keeping the search direction) whenever the
-// recurrence residual has fallen by delta^2 since the last such update, or looks
-// converged, or the iteration cap is reached; a slot whose true residual meets
-// tol stops (mk[MK_IT] cleared, S_DONE set) without a host round trip.
+// already within tolerance stop at once (S_DONE); x_zero: x64 is known to be zero
+// (r = b, one pass fewer). Iteration: one flexible-CG step for the slots in
+// mk[MK_IT]; the device folds the correction e into x64 and recomputes the true
+// residual (keeping the search direction) whenever the recurrence residual has
+// fallen by the slot's threshold since the last such update, or looks converged,
+// or the iteration cap is reached; a slot whose true residual meets tol stops
+// (mk[MK_IT] cleared, S_DONE set) without a host round trip. The threshold starts
+// at delta2 and follows the drift the updates measure between the recurrence and
+// the true residual: 1e-2 lower while they agree within a factor 2 (in norm),
+// 1e2 higher when they differ by more than 10.
 template <typename T>
-void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2);
+void ph_rel_start(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, double tol2, double delta2,
+                  bool x_zero);
 template <typename T>
-void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, double delta2,
-                 int maxit);
-// reliable-update threshold (squared) for Krylov precision T
+void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* b64, int pc, int maxit);
+// initial reliable-update threshold (squared) for Krylov precision T (FP64: only
+// the convergence test triggers an update)
 template <typename T>
 inline double rel_delta2() {
     return sizeof(T) == 4 ? 1e-4 : 0.0;
@@ -124,6 +129,8 @@ enum {
     S_NREP,  // true-residual evaluations (start + reliable updates)
     S_DONE,  // 1 once the slot has stopped (converged, iteration cap or breakdown)
     S_MEAN,  // scratch mean (singular projections)
+    S_RREC,  // recurrence residual^2 when the last update was decided
+    S_DLT2,  // current update threshold: update when rr < S_DLT2 * S_RREF
     S_N
 };
 static_assert(S_N <= GMG_SSTRIDE, "scalars per slot");

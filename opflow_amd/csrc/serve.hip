@@ -192,10 +192,8 @@ int max_iter(const opf_srv* s) { return s->cfg.max_iter > 0 ? s->cfg.max_iter : 
 // two batched iterations (the p ping-pong returns to its start)
 void iterate2(opf_srv* s, Group* g, hipStream_t st) {
     for (int pc = 0; pc < 2; ++pc) {
-        if (g->H.precision == OPF_PREC_MIXED)
-            ph_rel_iter<float>(g->H, st, g->x64, g->b64, pc, rel_delta2<float>(), max_iter(s));
-        else
-            ph_rel_iter<double>(g->H, st, g->x64, g->b64, pc, rel_delta2<double>(), max_iter(s));
+        if (g->H.precision == OPF_PREC_MIXED) ph_rel_iter<float>(g->H, st, g->x64, g->b64, pc, max_iter(s));
+        else ph_rel_iter<double>(g->H, st, g->x64, g->b64, pc, max_iter(s));
     }
 }
 
@@ -294,8 +292,11 @@ int round(opf_srv* s, int* nfinished) {
         hipStream_t st = g->st;
         if (fresh) {
             OPF_HIP(hipMemcpyAsync(g->H.mk + MK_NEW * g->B, g->hnew, sizeof(int) * g->B, hipMemcpyHostToDevice, st));
-            if (g->H.precision == OPF_PREC_MIXED) ph_rel_start<float>(g->H, st, g->x64, g->b64, tol * tol);
-            else ph_rel_start<double>(g->H, st, g->x64, g->b64, tol * tol);
+            // admitted slots start from x = 0 (zeroed at admission)
+            if (g->H.precision == OPF_PREC_MIXED)
+                ph_rel_start<float>(g->H, st, g->x64, g->b64, tol * tol, rel_delta2<float>(), true);
+            else
+                ph_rel_start<double>(g->H, st, g->x64, g->b64, tol * tol, rel_delta2<double>(), true);
         }
         for (int r = 0; r < g->replays; ++r) {
             if (s->cfg.batched) OPF_HIP(hipGraphLaunch(g->exec, st));
