@@ -214,37 +214,32 @@ __global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T
     }
 }
 
-// 3-D temporal blocking for fields periodic in every dimension. A workgroup
-// owns a TX x TY column of the x-y plane and the z chunk [z0, z0 + zc). It
-// streams planes of u^n, each with a K-cell wrapped halo in x and y, through
-// LDS and advances them K steps in a z pipeline: once plane zz of level 0 has
-// landed, level L (1..K) computes its plane zz - L from level L-1's planes
-// zz-L-1 .. zz-L+1 (a ring of 3 planes per level) over an x-y window that
-// shrinks by one cell per level; level K is exactly the tile and goes straight
-// to global memory. Level-0 planes are loaded PF planes ahead into registers,
-// so their HBM latency overlaps the compute of the planes before them. Reads
-// wrap to interior cells, so the input's ghosts are never read; the output's
-// ghosts are left stale and refilled by the host loop before anything reads
-// them (a branch-free body keeps the load waits counted). Per cell and step the arithmetic is that of
-// ftcs3d_kernel, so the result has the same bits as K single steps.
-template <int TX, int TY, int K>
-struct Ftcs3dTile {
-    static constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
-};
-
-template <typename T, bool POW2, int TX, int TY, int K, int NT, int PF, bool XCD>
+// 3-D temporal blocking, two steps per launch, for fields periodic in every
+// dimension. A workgroup owns a TX x TY column of the x-y plane and the z chunk
+// [z0, z0 + zc) and streams the planes of u^n with a wrapped 2-cell halo in x
+// and y. Every thread owns fixed positions of the (TX+4) x (TY+4) region and
+// keeps, for each, the level-0 (u^n) values of the last three planes and the
+// level-1 (u^{n+1}) values of the last three planes in registers, so the z
+// neighbours and the centre never go through LDS; LDS holds only the current
+// plane of each level (double-buffered) for the x and y neighbours. Step zz
+// stores level-0 plane zz, computes level 1 at plane zz - 1 (on the region
+// shrunk by one cell) and level 2 (u^{n+2}, exactly the tile, straight to
+// global memory) at plane zz - 2, with one barrier per step. Level-0 planes are
+// loaded three planes ahead; the unroll by 3 keeps the history slots static.
+// Reads wrap to interior cells, so the input's ghosts are never read; the
+// output's ghosts are left stale and refilled by the host loop before anything
+// reads them (a branch-free load path keeps the load waits counted). Per cell
+// and step the arithmetic is that of ftcs3d_kernel, so the result has the same
+// bits as two single steps.
+template <typename T, bool POW2, int TX, int TY, int NT, bool XCD>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
-                                                       int zc) {
-    static_assert(K >= 2, "level K must not share a barrier interval with the level-0 store");
-    using G = Ftcs3dTile<TX, TY, K>;
-    constexpr int RX = G::RX, RY = G::RY, RP = G::RP;
+                                                        int zc) {
+    constexpr int RX = TX + 4, RY = TY + 4, RP = RX * RY;
     constexpr int NL = (RP + NT - 1) / NT;
-    // level 0 rows are padded to NL * NT so every thread loads and stores
-    // unconditionally (no branches around the loads: their waits stay counted)
-    __shared__ T ring0[3][NL * NT];
-    __shared__ T ringl[K - 1][3][RP];
+    constexpr int PF = 3;
+    __shared__ T l0[2][NL * NT];
+    __shared__ T l1[2][NL * NT];
     const int tilesx = (a.n[0] + TX - 1) / TX, tilesy = (a.n[1] + TY - 1) / TY;
-    // XCD: neighbouring tiles share an XCD's L2 (halo reuse)
     int b = XCD ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int x0 = (b % tilesx) * TX;
     b /= tilesx;
@@ -252,25 +247,27 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     const int z0 = (b / tilesy) * zc;
     const int z1 = min(z0 + zc, a.n[2]);
     const long long sz = (long long)a.e0 * a.e1;
-    // wrapped in-plane storage offsets of this thread's level-0 elements (padding
-    // elements load the region's first cell)
+    const int tid = threadIdx.x;
     int off[NL];
+    bool in1[NL], in2[NL];
+    long long oo[NL];  // output offset in a plane (level-2 positions)
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
-        int e = threadIdx.x + m * NT;
-        if (e >= RP) e = 0;
-        const int lx = e % RX, ly = e / RX;
-        const int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
-        const int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
+        const int e = tid + m * NT;
+        const bool valid = e < RP;
+        const int ee = valid ? e : 0;
+        const int lx = ee % RX, ly = ee / RX;
+        const int gx = ((x0 - 2 + lx) % a.n[0] + a.n[0]) % a.n[0];
+        const int gy = ((y0 - 2 + ly) % a.n[1] + a.n[1]) % a.n[1];
         off[m] = (gx + 1) + a.e0 * (gy + 1);
+        in1[m] = valid && lx >= 1 && lx <= RX - 2 && ly >= 1 && ly <= RY - 2;
+        const int i = x0 + lx - 2, j = y0 + ly - 2;
+        in2[m] = valid && lx >= 2 && lx <= RX - 3 && ly >= 2 && ly <= RY - 3 && i < a.n[0] && j < a.n[1];
+        oo[m] = (long long)(i + 1) + (long long)a.e0 * (j + 1);
     }
-    auto slot = [](int z) { return (z + 3 * K) % 3; };  // z >= -K
-    auto level = [&](int L, int s) -> T* { return L == 0 ? ring0[s] : ringl[L - 1][s]; };
-    // level-0 planes zb .. ze-1 stream through PF register buffers (loads run
-    // PF planes ahead of the plane being stored to LDS)
-    const int zb = z0 - K, ze = z1 + K;
-    T pre[PF][NL];
-    auto load = [&](T (&r)[NL], int z) {
+    const int zb = z0 - 2, ze = z1 + 2;  // level-0 planes zb .. ze - 1
+    T pre[PF][NL], h0[NL][3], h1[NL][3];
+    auto load = [&](T(&r)[NL], int z) {
         z = (z % a.n[2] + a.n[2]) % a.n[2];
         const T* p = in + sz * (z + 1);
 #pragma unroll
@@ -283,38 +280,45 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
         for (int u = 0; u < PF; ++u) {
             const int zz = zz0 + u;
             if (zz >= ze) break;  // uniform over the block
-            T* r0 = ring0[slot(zz)];
-#pragma unroll
-            for (int m = 0; m < NL; ++m) r0[threadIdx.x + m * NT] = pre[u][m];
-            load(pre[u], min(zz + PF, ze - 1));
+            // history slot of plane p is (p - zb) % 3: plane zz -> u, zz-1 -> u+2, zz-2 -> u+1
+            const int s0 = u, s1 = (u + 2) % 3, s2 = (u + 1) % 3;
             __syncthreads();
+            T* w0 = l0[zz & 1];
 #pragma unroll
-            for (int L = 1; L <= K; ++L) {
-                const int zl = zz - L;
-                if (zl < z0 - K + L) break;  // pipeline still filling (uniform over the block)
-                const T* pm = level(L - 1, slot(zl - 1));
-                const T* pc = level(L - 1, slot(zl));
-                const T* pp = level(L - 1, slot(zl + 1));
-                const int w = RX - 2 * L, h = RY - 2 * L;
-                for (int t = threadIdx.x; t < w * h; t += NT) {
-                    const int lx = L + t % w, ly = L + t / w;
-                    const int e = lx + RX * ly;
-                    const T c = pc[e];
-                    T s = d2<T, POW2>(pc[e - 1], c, pc[e + 1], a.hh[0]) +
-                          d2<T, POW2>(pc[e - RX], c, pc[e + RX], a.hh[1]);
-                    s = s + d2<T, POW2>(pm[e], c, pp[e], a.hh[2]);
+            for (int m = 0; m < NL; ++m) {
+                h0[m][s0] = pre[u][m];
+                w0[tid + m * NT] = pre[u][m];
+            }
+            load(pre[u], min(zz + PF, ze - 1));
+            if (zz >= z0) {
+                // level 1 at plane zz - 1 from level-0 planes zz - 2 .. zz
+                const T* P = l0[(zz - 1) & 1];
+                T* w1 = l1[(zz - 1) & 1];
+#pragma unroll
+                for (int m = 0; m < NL; ++m) {
+                    if (!in1[m]) continue;
+                    const int e = tid + m * NT;
+                    const T c = h0[m][s1];
+                    T s = d2<T, POW2>(P[e - 1], c, P[e + 1], a.hh[0]) + d2<T, POW2>(P[e - RX], c, P[e + RX], a.hh[1]);
+                    s = s + d2<T, POW2>(h0[m][s2], c, h0[m][s0], a.hh[2]);
                     const T v = c + a.coef * s;
-                    if (L < K) {
-                        ringl[L - 1][slot(zl)][e] = v;
-                    } else {
-                        const int i = x0 + lx - K, j = y0 + ly - K;
-                        if (i < a.n[0] && j < a.n[1]) {
-                            const int p[3] = {i, j, zl};
-                            out[ftcs_index<3>(a.e0, a.e1, p)] = v;
-                        }
-                    }
+                    h1[m][s1] = v;
+                    w1[e] = v;
                 }
-                if (L < K) __syncthreads();
+            }
+            if (zz >= z0 + 2) {
+                // level 2 (the output) at plane zz - 2 from level-1 planes zz - 3 .. zz - 1
+                const T* Q = l1[(zz - 2) & 1];
+                T* po = out + sz * (zz - 2 + 1);
+#pragma unroll
+                for (int m = 0; m < NL; ++m) {
+                    if (!in2[m]) continue;
+                    const int e = tid + m * NT;
+                    const T c = h1[m][s2];
+                    T s = d2<T, POW2>(Q[e - 1], c, Q[e + 1], a.hh[0]) + d2<T, POW2>(Q[e - RX], c, Q[e + RX], a.hh[1]);
+                    s = s + d2<T, POW2>(h1[m][s0], c, h1[m][s1], a.hh[2]);
+                    po[oo[m]] = c + a.coef * s;
+                }
             }
         }
     }
@@ -324,19 +328,19 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
 // 3-D single steps march FT_ZC planes per block
 constexpr int FT_ZC = 16, FT_TS2 = 32, FT_TB2 = 8;
 
-// 3-D blocked launch shape per precision (measured at 512^3 on MI355X):
-// TX x TY columns, K steps per launch, NT threads, PF planes of prefetch, z
-// chunks of ZC planes
+// 3-D blocked launch shape per precision (swept at 512^3 on MI355X): TX x TY
+// columns, NT threads, z chunks of ZC planes; two steps per launch
+constexpr int FT_K3 = 2;
 template <typename T>
 struct Ftcs3dCfg;
 template <>
 struct Ftcs3dCfg<double> {
-    static constexpr int TX = 32, TY = 16, K = 2, NT = 256, PF = 2, ZC = 64;
+    static constexpr int TX = 64, TY = 16, NT = 512, ZC = 128;
     static constexpr bool XCD = true;
 };
 template <>
 struct Ftcs3dCfg<float> {
-    static constexpr int TX = 64, TY = 8, K = 2, NT = 256, PF = 4, ZC = 64;
+    static constexpr int TX = 128, TY = 16, NT = 1024, ZC = 64;
     static constexpr bool XCD = true;
 };
 
@@ -345,8 +349,8 @@ struct Ftcs3dCfg<float> {
 int launch_steps(int dim, int dtype, const int n[3], const int per[3], long long plane, int remaining, int tblock) {
     if (tblock == 1) return 1;
     if (dim == 2 && remaining >= FT_TB2 && n[0] >= FT_TB2 && n[1] >= FT_TB2) return FT_TB2;
-    const int k3 = dtype == OPF_F64 ? Ftcs3dCfg<double>::K : Ftcs3dCfg<float>::K;
-    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= k3) return k3;
+    (void)dtype;
+    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= FT_K3) return FT_K3;
     return 1;
 }
 
@@ -434,8 +438,8 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         } else if (d.dim == 3 && k > 1) {
             using C3 = Ftcs3dCfg<T>;
             int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::K, C3::NT, C3::PF, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::K, C3::NT, C3::PF, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
             ghosts_stale = true;
         } else if (d.dim == 3) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + FT_ZC - 1) / FT_ZC);
