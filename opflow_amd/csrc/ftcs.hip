@@ -214,33 +214,34 @@ __global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T
     }
 }
 
-// 3-D temporal blocking, two steps per launch, for fields periodic in every
+// 3-D temporal blocking, K steps per launch, for fields periodic in every
 // dimension. A workgroup owns a TX x TY column of the x-y plane and the z chunk
-// [z0, z0 + zc) and streams the planes of u^n with a wrapped 2-cell halo in x
-// and y. Every thread owns fixed positions of the (TX+4) x (TY+4) region and
-// keeps, for each, the level-0 (u^n) values of the last three planes and the
-// level-1 (u^{n+1}) values of the last three planes in registers, so the z
-// neighbours and the centre never go through LDS; LDS holds only the current
-// plane of each level (double-buffered) for the x and y neighbours. Step zz
-// stores level-0 plane zz, computes level 1 at plane zz - 1 (on the region
-// shrunk by one cell) and level 2 (u^{n+2}, exactly the tile, straight to
-// global memory) at plane zz - 2, with one barrier per step. Level-0 planes are
-// loaded three planes ahead; the unroll by 3 keeps the history slots static.
-// Reads wrap to interior cells, so the input's ghosts are never read; the
-// output's ghosts are left stale and refilled by the host loop before anything
-// reads them (a branch-free load path keeps the load waits counted). Per cell
-// and step the arithmetic is that of ftcs3d_kernel, so the result has the same
-// bits as two single steps.
-template <typename T, bool POW2, int TX, int TY, int NT, bool XCD>
+// [z0, z0 + zc) and streams the planes of u^n with a wrapped K-cell halo in x
+// and y. Every thread owns fixed positions of the (TX+2K) x (TY+2K) region and
+// keeps, for each, the last three planes of levels 0 .. K-1 (u^n .. u^{n+K-1})
+// in registers, so the centre and the z neighbours never go through LDS; LDS
+// holds only the current plane of each level (double-buffered), read for the x
+// and y neighbours. Step zz stores level-0 plane zz and computes level L at
+// plane zz - L (L = 1 .. K) on the region shrunk by L cells; level K is exactly
+// the tile and goes straight to global memory. One barrier per step separates
+// a step's LDS writes from the next step's reads (levels read the plane the
+// level below wrote one step earlier). Level-0 planes are loaded three planes
+// ahead; the unroll by 3 keeps the history slots static. Reads wrap to interior
+// cells, so the input's ghosts are never read; the output's ghosts are left
+// stale and refilled by the host loop before anything reads them (a
+// branch-free load path keeps the load waits counted). Per cell and step the
+// arithmetic is that of ftcs3d_kernel, so the result has the same bits as K
+// single steps.
+template <typename T, bool POW2, int TX, int TY, int NT, int K, bool XCD>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
-                                                        int zc) {
-    constexpr int RX = TX + 4, RY = TY + 4, RP = RX * RY;
+                                                       int zc) {
+    static_assert(K >= 2, "one step per launch is ftcs3d_kernel");
+    constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
     constexpr int NL = (RP + NT - 1) / NT;
     constexpr int PF = 3;
-    __shared__ T l0[2][NL * NT];
-    __shared__ T l1[2][NL * NT];
+    __shared__ T lds[K][2][NL * NT];  // current plane of levels 0 .. K-1
     const int tilesx = (a.n[0] + TX - 1) / TX, tilesy = (a.n[1] + TY - 1) / TY;
-    int b = XCD ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    int b = XCD ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;  // neighbouring tiles share an XCD's L2
     const int x0 = (b % tilesx) * TX;
     b /= tilesx;
     const int y0 = (b % tilesy) * TY;
@@ -248,25 +249,31 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     const int z1 = min(z0 + zc, a.n[2]);
     const long long sz = (long long)a.e0 * a.e1;
     const int tid = threadIdx.x;
-    int off[NL];
-    bool in1[NL], in2[NL];
-    long long oo[NL];  // output offset in a plane (level-2 positions)
+    // this thread's region positions: wrapped input offset, region coordinates, output offset
+    int off[NL], lxm[NL], lym[NL];
+    bool valid[NL], inr[NL];
+    long long oo[NL];
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
         const int e = tid + m * NT;
-        const bool valid = e < RP;
-        const int ee = valid ? e : 0;
+        valid[m] = e < RP;
+        const int ee = valid[m] ? e : 0;
         const int lx = ee % RX, ly = ee / RX;
-        const int gx = ((x0 - 2 + lx) % a.n[0] + a.n[0]) % a.n[0];
-        const int gy = ((y0 - 2 + ly) % a.n[1] + a.n[1]) % a.n[1];
+        lxm[m] = lx;
+        lym[m] = ly;
+        const int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
+        const int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
         off[m] = (gx + 1) + a.e0 * (gy + 1);
-        in1[m] = valid && lx >= 1 && lx <= RX - 2 && ly >= 1 && ly <= RY - 2;
-        const int i = x0 + lx - 2, j = y0 + ly - 2;
-        in2[m] = valid && lx >= 2 && lx <= RX - 3 && ly >= 2 && ly <= RY - 3 && i < a.n[0] && j < a.n[1];
+        const int i = x0 + lx - K, j = y0 + ly - K;
+        inr[m] = i < a.n[0] && j < a.n[1];
         oo[m] = (long long)(i + 1) + (long long)a.e0 * (j + 1);
     }
-    const int zb = z0 - 2, ze = z1 + 2;  // level-0 planes zb .. ze - 1
-    T pre[PF][NL], h0[NL][3], h1[NL][3];
+    // position m takes part in level L (region shrunk by L cells)
+    auto at_level = [&](int m, int L) {
+        return valid[m] && lxm[m] >= L && lxm[m] <= RX - 1 - L && lym[m] >= L && lym[m] <= RY - 1 - L;
+    };
+    const int zb = z0 - K, ze = z1 + K;  // level-0 planes zb .. ze - 1
+    T pre[PF][NL], hist[K][NL][3];       // history slot of plane p: (p - zb) % 3
     auto load = [&](T(&r)[NL], int z) {
         z = (z % a.n[2] + a.n[2]) % a.n[2];
         const T* p = in + sz * (z + 1);
@@ -278,46 +285,36 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     for (int zz0 = zb; zz0 < ze; zz0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int zz = zz0 + u;
-            if (zz >= ze) break;  // uniform over the block
-            // history slot of plane p is (p - zb) % 3: plane zz -> u, zz-1 -> u+2, zz-2 -> u+1
-            const int s0 = u, s1 = (u + 2) % 3, s2 = (u + 1) % 3;
+            const int zz = zz0 + u;  // (zz - zb) % 3 == u
+            if (zz >= ze) break;     // uniform over the block
             __syncthreads();
-            T* w0 = l0[zz & 1];
+            T* w0 = lds[0][zz & 1];
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
-                h0[m][s0] = pre[u][m];
+                hist[0][m][u] = pre[u][m];
                 w0[tid + m * NT] = pre[u][m];
             }
             load(pre[u], min(zz + PF, ze - 1));
-            if (zz >= z0) {
-                // level 1 at plane zz - 1 from level-0 planes zz - 2 .. zz
-                const T* P = l0[(zz - 1) & 1];
-                T* w1 = l1[(zz - 1) & 1];
+#pragma unroll
+            for (int L = 1; L <= K; ++L) {
+                if (zz < z0 - K + 2 * L) break;  // plane zz - L of level L not computable yet (uniform)
+                // level L-1 planes zz-L+1, zz-L, zz-L-1 sit in slots sp, sc, sm
+                const int sp = ((u - L + 1) % 3 + 3) % 3, sc = ((u - L) % 3 + 3) % 3, sm = ((u - L - 1) % 3 + 3) % 3;
+                const T* P = lds[L - 1][(zz - L) & 1];
 #pragma unroll
                 for (int m = 0; m < NL; ++m) {
-                    if (!in1[m]) continue;
+                    if (L < K ? !at_level(m, L) : !(at_level(m, K) && inr[m])) continue;
                     const int e = tid + m * NT;
-                    const T c = h0[m][s1];
+                    const T c = hist[L - 1][m][sc];
                     T s = d2<T, POW2>(P[e - 1], c, P[e + 1], a.hh[0]) + d2<T, POW2>(P[e - RX], c, P[e + RX], a.hh[1]);
-                    s = s + d2<T, POW2>(h0[m][s2], c, h0[m][s0], a.hh[2]);
+                    s = s + d2<T, POW2>(hist[L - 1][m][sm], c, hist[L - 1][m][sp], a.hh[2]);
                     const T v = c + a.coef * s;
-                    h1[m][s1] = v;
-                    w1[e] = v;
-                }
-            }
-            if (zz >= z0 + 2) {
-                // level 2 (the output) at plane zz - 2 from level-1 planes zz - 3 .. zz - 1
-                const T* Q = l1[(zz - 2) & 1];
-                T* po = out + sz * (zz - 2 + 1);
-#pragma unroll
-                for (int m = 0; m < NL; ++m) {
-                    if (!in2[m]) continue;
-                    const int e = tid + m * NT;
-                    const T c = h1[m][s2];
-                    T s = d2<T, POW2>(Q[e - 1], c, Q[e + 1], a.hh[0]) + d2<T, POW2>(Q[e - RX], c, Q[e + RX], a.hh[1]);
-                    s = s + d2<T, POW2>(h1[m][s0], c, h1[m][s1], a.hh[2]);
-                    po[oo[m]] = c + a.coef * s;
+                    if (L < K) {
+                        hist[L][m][sc] = v;
+                        lds[L][(zz - L) & 1][e] = v;
+                    } else {
+                        out[sz * (zz - K + 1) + oo[m]] = v;
+                    }
                 }
             }
         }
@@ -329,18 +326,18 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
 constexpr int FT_ZC = 16, FT_TS2 = 32, FT_TB2 = 8;
 
 // 3-D blocked launch shape per precision (swept at 512^3 on MI355X): TX x TY
-// columns, NT threads, z chunks of ZC planes; two steps per launch
-constexpr int FT_K3 = 2;
+// columns, NT threads, z chunks of ZC planes, K steps per launch (a remainder of
+// two steps takes one K = 2 launch of the same tile, one step the single-step kernel)
 template <typename T>
 struct Ftcs3dCfg;
 template <>
 struct Ftcs3dCfg<double> {
-    static constexpr int TX = 64, TY = 16, NT = 512, ZC = 128;
+    static constexpr int TX = 64, TY = 32, NT = 1024, ZC = 64, K = 3;
     static constexpr bool XCD = true;
 };
 template <>
 struct Ftcs3dCfg<float> {
-    static constexpr int TX = 128, TY = 16, NT = 1024, ZC = 64;
+    static constexpr int TX = 128, TY = 32, NT = 1024, ZC = 64, K = 3;
     static constexpr bool XCD = true;
 };
 
@@ -349,8 +346,8 @@ struct Ftcs3dCfg<float> {
 int launch_steps(int dim, int dtype, const int n[3], const int per[3], long long plane, int remaining, int tblock) {
     if (tblock == 1) return 1;
     if (dim == 2 && remaining >= FT_TB2 && n[0] >= FT_TB2 && n[1] >= FT_TB2) return FT_TB2;
-    (void)dtype;
-    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= FT_K3) return FT_K3;
+    const int k3 = dtype == OPF_F64 ? Ftcs3dCfg<double>::K : Ftcs3dCfg<float>::K;
+    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= 2) return std::min(k3, remaining);
     return 1;
 }
 
@@ -438,8 +435,14 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         } else if (d.dim == 3 && k > 1) {
             using C3 = Ftcs3dCfg<T>;
             int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-            else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            if (k == C3::K) {
+                if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            } else {
+                static_assert(C3::K == 3, "remainder launches assume K = 3");
+                if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, 2, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, 2, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            }
             ghosts_stale = true;
         } else if (d.dim == 3) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + FT_ZC - 1) / FT_ZC);

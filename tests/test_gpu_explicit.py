@@ -94,12 +94,12 @@ def test_ftcs3d_periodic(ctx, dtype):
     u.close()
 
 
-def test_ftcs3d_periodic_ragged_blocked(ctx):
-    """Temporal blocking on a grid that no tile divides, odd step count (blocked
-    launches plus a trailing single step): equal bits to the oracle and to one
-    launch per step."""
+@pytest.mark.parametrize("steps", [13, 14], ids=["remainder-1", "remainder-2"])
+def test_ftcs3d_periodic_ragged_blocked(ctx, steps):
+    """Temporal blocking on a grid that no tile divides, with a remainder of one
+    step (a trailing single step) or two (a trailing two-step launch): equal bits
+    to the oracle and to one launch per step."""
     n = [40, 36, 20]
-    steps = 13
     mesh, om = _mesh(ctx, [m + 1 for m in n], [(0.0, m / 32.0) for m in n])
     fields = []
     for tblock in (0, 1):
@@ -110,7 +110,7 @@ def test_ftcs3d_periodic_ragged_blocked(ctx):
         if tblock == 1:
             assert (nl, k) == (steps, 1)
         else:
-            assert k > 1 and nl == steps // k + steps % k
+            assert k > 1 and nl == -(-steps // k)  # K-step launches, then one launch for the rest
         u.ftcs_blocked(0.2 / 32**2, steps, tblock)
         fields.append(u.numpy())
         u.close()
