@@ -2,7 +2,7 @@
 
 Headline workload (one GPU): 3-D FTCS heat sweep, 512^3 cells FP64,
 periodic, u0 ~ U[-1,1] i.i.d. from a seed, dt = 0.2 h^2, alpha = 1 -- one
-"step" is one FTCS time step over the whole grid (one fused launch).
+"step" is one FTCS time step over the whole grid (a launch fuses up to three).
 value = cell-updates/s over the timed steps (inputs resident in HBM).
 
 Also reported (extra): the FP32 variant of the same sweep, the 2-D 256^2
@@ -241,6 +241,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "step_equivalent_GBps": alg_bytes * args.steps / (ms / 1e3) / 1e9,
                      "step_equivalent_frac": alg_bytes * args.steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic_GBps": traffic / per_launch_s / 1e9 if traffic else None,
+                     "traffic_frac": traffic / per_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "kernel": "ftcs3d_tb_kernel" if steps_per_launch > 1 else "ftcs3d_kernel",
                      "steps_per_launch": steps_per_launch, "launches": launches},
         "cpu_baseline": cpu,
