@@ -37,9 +37,10 @@ def _locs(nodes, fields, dim):
     return locs
 
 
-def _weno(v, h, up):
+def _weno_parts(v, h, up):
     """D1WENO53Upwind::kernel / D1WENO53Downwind::kernel (D1WENO53Upwind.hpp:133-151,
-    D1WENO53Downwind.hpp:136-154); v = six values in index order."""
+    D1WENO53Downwind.hpp:136-154); v = six values in index order. Returns the three
+    candidate derivatives, the three smoothness indicators and the three weights."""
     if up:
         pm2, pm1, p0, pp1, pp2, pp3 = v
         d1, d2, d3, d4, d5 = (pp3 - pp2) / h, (pp2 - pp1) / h, (pp1 - p0) / h, (p0 - pm1) / h, (pm1 - pm2) / h
@@ -62,7 +63,13 @@ def _weno(v, h, up):
     w1 = a1 / (a1 + a2 + a3)
     w2 = a2 / (a1 + a2 + a3)
     w3 = a3 / (a1 + a2 + a3)
-    return w1 * ddx1 + w2 * ddx2 + w3 * ddx3
+    return (ddx1, ddx2, ddx3), (s1, s2, s3), (w1, w2, w3)
+
+
+def _weno(v, h, up):
+    """The WENO5 derivative: the weighted sum of the candidates (same operation order)."""
+    (c1, c2, c3), _, (w1, w2, w3) = _weno_parts(v, h, up)
+    return w1 * c1 + w2 * c2 + w3 * c3
 
 
 def evaluate(nodes, root, fields, mesh, lo, hi):
