@@ -205,8 +205,14 @@ struct opf_eqn {
     opf_ctx* ctx = nullptr;
     opf_field* target = nullptr;
     std::vector<opf_field*> fields;
+    std::vector<opf_expr_node> all;  // lhs - rhs
     LoweredProgram P;
     void* prog = nullptr;
+    // right-hand side only (static operators after the first solve): the tree with the
+    // unknown read from a zero field whose ghosts hold the boundary constants
+    LoweredProgram Pb;
+    void* progb = nullptr;
+    void* zbuf = nullptr;
     opf_solve_params params{};
     int s[3], n[3];
     long long N = 0;
@@ -264,6 +270,52 @@ static int eqn_eval_pads(opf_eqn* e) {
               "equation reads the unknown off the 5/7-point axis stencil (not a Laplacian-type operator)");
     OPF_CHECK(!(herr & 2), OPF_ERR_UNSUPPORTED, "equation is nonlinear in the unknown");
     return OPF_OK;
+}
+
+// generateb (HYPREEqnSolveHandler.hpp:168-179): the bias of every row without the
+// coefficients. The pad of lhs - rhs is linear in the unknown, so its constant
+// part is the tree evaluated with the unknown replaced by a field that is zero
+// inside and whose ghosts follow the target's boundary rules (the affine
+// constants the pad evaluation folds in). One sweep, no host round trip.
+static int eqn_eval_bias(opf_eqn* e) {
+    opf_ctx* ctx = e->ctx;
+    hipStream_t st = ctx->stream;
+    opf_field* t = e->target;
+    const int nf = (int)e->fields.size();
+    if (!e->progb) {
+        OPF_CHECK(nf + 1 <= SWEEP_MAX_FIELDS, OPF_ERR_UNSUPPORTED, "equation uses too many fields");
+        OPF_HIP(hipMalloc(&e->zbuf, t->nbytes));
+        OPF_HIP(hipMemsetAsync(e->zbuf, 0, t->nbytes, st));
+        std::vector<opf_field*> fl(e->fields.begin(), e->fields.end());
+        fl.push_back(t);  // slot nf: the zero field (the target's layout)
+        OPF_TRY(lower_tree(e->all.data(), (int)e->all.size(), (int)e->all.size() - 1, fl.data(), nf + 1, t->mesh, nf,
+                           &e->Pb, true));
+        OPF_TRY(check_bounds(e->Pb, fl.data(), nf + 1, t->mesh, e->s, e->n));
+        OPF_TRY(upload_program(ctx, e->Pb.code.data(), e->Pb.code.size() * sizeof(Insn), &e->progb));
+    }
+    OPF_TRY(launch_ghost_fill(t, e->zbuf));  // boundary constants (values may change between solves)
+    SweepArgs a{};
+    for (int f = 0; f < nf; ++f) {
+        const opf_field_desc& fd = e->fields[f]->d;
+        a.f[f] = {e->fields[f]->data, fd.offset[0], fd.offset[1], fd.offset[2], fd.extent[0], fd.extent[1]};
+    }
+    const opf_field_desc& td = t->d;
+    a.f[nf] = {e->zbuf, td.offset[0], td.offset[1], td.offset[2], td.extent[0], td.extent[1]};
+    OPF_TRY(mesh_arrays(t->mesh, OPF_F64, a.x, a.dx));
+    for (int q = 0; q < 3; ++q) {
+        a.morg[q] = t->mesh->start[q] - t->mesh->pad;
+        a.s[q] = e->s[q];
+        a.n[q] = e->n[q];
+        a.oo[q] = e->s[q];  // dense rows, x-fastest over the row box
+    }
+    a.oe0 = e->n[0];
+    a.oe1 = e->n[1];
+    a.op = OPF_OP_EQ;
+    a.out = e->bias;
+    a.old = e->bias;
+    a.code = e->progb;
+    a.ncode = (int)e->Pb.code.size();
+    return launch_sweep(st, OPF_F64, a);
 }
 
 // rows -> multigrid level (face coefficients, sigma); checks symmetry
@@ -393,9 +445,10 @@ extern "C" int opf_eqn_create(const opf_expr_node* nodes, int nnodes, int lhs_ro
     e->fields.assign(fields, fields + nfields);
     e->params = params ? *params : opf_solve_params{};
     // lhs - rhs as one tree: append a SUB node
-    std::vector<opf_expr_node> all(nodes, nodes + nnodes);
-    all.push_back({OPF_N_SUB, lhs_root, rhs_root, 0, 0.0});
-    int rc = lower_tree(all.data(), (int)all.size(), (int)all.size() - 1, fields, nfields, target->mesh, tslot, &e->P);
+    e->all.assign(nodes, nodes + nnodes);
+    e->all.push_back({OPF_N_SUB, lhs_root, rhs_root, 0, 0.0});
+    int rc = lower_tree(e->all.data(), (int)e->all.size(), (int)e->all.size() - 1, fields, nfields, target->mesh, tslot,
+                        &e->P);
     if (rc != OPF_OK) {
         delete e;
         return rc;
@@ -459,9 +512,13 @@ extern "C" int opf_eqn_solve(opf_eqn* e, opf_solve_state* st) {
     OPF_CHECK(e && st, OPF_ERR_ARG, "opf_eqn_solve: null arg");
     opf_ctx* ctx = e->ctx;
     hipStream_t s = ctx->stream;
-    // generateAb / generateb (HYPREEqnSolveHandler.hpp:125-179)
-    OPF_TRY(eqn_eval_pads(e));
-    if (e->first || !e->params.static_mat) OPF_TRY(eqn_build_operator(e));
+    // generateAb / generateb (HYPREEqnSolveHandler.hpp:125-179, solve :190-209)
+    if (e->first || !e->params.static_mat) {
+        OPF_TRY(eqn_eval_pads(e));
+        OPF_TRY(eqn_build_operator(e));
+    } else {
+        OPF_TRY(eqn_eval_bias(e));
+    }
     e->first = false;
     const long long N = e->N;
     const unsigned nb = (unsigned)((N + 255) / 256);
@@ -564,7 +621,7 @@ extern "C" int opf_eqn_destroy(opf_eqn* e) {
     if (!e) return OPF_OK;
     hipStreamSynchronize(e->ctx->stream);
     if (e->built) gmg_free(e->H);
-    void* p[] = {e->coef, e->bias, e->b, e->x, e->shift, e->err};
+    void* p[] = {e->coef, e->bias, e->b, e->x, e->shift, e->err, e->zbuf};
     for (void* q : p)
         if (q) hipFree(q);
     delete e;

@@ -25,6 +25,7 @@ struct Lowerer {
     int nfields;
     const opf_mesh* mesh;
     int unknown_slot = -1;  // field slot standing for OPF_N_UNKNOWN (pad mode)
+    bool unknown_as_field = false;  // read OPF_N_UNKNOWN as an ordinary field (bias evaluation)
     std::vector<Insn> code;
     int depth = 0, maxdepth = 0;
     // per field, per dim min/max read offset relative to the evaluation index
@@ -116,12 +117,13 @@ struct Lowerer {
                 int f = e.kind == OPF_N_FIELD ? e.a : unknown_slot;
                 const opf_field_desc& fd = fields[f]->d;
                 long long lin = sh[0] + (long long)fd.extent[0] * (sh[1] + (long long)fd.extent[1] * sh[2]);
-                code.push_back({e.kind == OPF_N_FIELD ? I_LD : I_LDU, f, (int)lin, 0, 0.0});
+                const bool ldu = e.kind == OPF_N_UNKNOWN && !unknown_as_field;
+                code.push_back({ldu ? I_LDU : I_LD, f, (int)lin, 0, 0.0});
                 for (int d = 0; d < 3; ++d) {
                     omin[f][d] = std::min(omin[f][d], sh[d]);
                     omax[f][d] = std::max(omax[f][d], sh[d]);
                 }
-                if (e.kind == OPF_N_UNKNOWN) {
+                if (ldu) {
                     // remember the offset in the instruction for pad mode slot lookup
                     code.back().c = (sh[0] + 8) | ((sh[1] + 8) << 5) | ((sh[2] + 8) << 10);
                 }
@@ -212,7 +214,7 @@ struct Lowerer {
 };
 
 int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
-               const opf_mesh* mesh, int unknown_slot, LoweredProgram* out) {
+               const opf_mesh* mesh, int unknown_slot, LoweredProgram* out, bool unknown_as_field) {
     OPF_CHECK(nodes && nnodes > 0 && root >= 0 && root < nnodes, OPF_ERR_ARG, "lowering: bad tree");
     OPF_CHECK(nfields >= 0 && nfields <= SWEEP_MAX_FIELDS, OPF_ERR_UNSUPPORTED, "at most %d fields per sweep",
               SWEEP_MAX_FIELDS);
@@ -223,6 +225,7 @@ int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* cons
     L.nfields = nfields;
     L.mesh = mesh;
     L.unknown_slot = unknown_slot;
+    L.unknown_as_field = unknown_as_field;
     L.init();
     int rc = L.compute_locs();
     OPF_CHECK(rc == OPF_OK, rc, "lowering: malformed tree (children must precede parents, valid field slots)");
@@ -375,6 +378,14 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
 
 namespace opf {
 
+int launch_sweep(hipStream_t st, int dtype, const SweepArgs& a) {
+    dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, a.n[2]);
+    if (dtype == OPF_F64) hipLaunchKernelGGL(sweep_kernel<double>, grid, dim3(64, 4), 0, st, a);
+    else hipLaunchKernelGGL(sweep_kernel<float>, grid, dim3(64, 4), 0, st, a);
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
+}
+
 int mesh_arrays(const opf_mesh* mc, int dtype, const void* x[3], const void* dx[3]) {
     opf_mesh* m = const_cast<opf_mesh*>(mc);
     for (int d = 0; d < 3; ++d) {
@@ -457,10 +468,7 @@ extern "C" int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_f
         OPF_TRY(upload_program(ctx, P.code.data(), P.code.size() * sizeof(Insn), &dev));
         a.code = dev;
         a.ncode = (int)P.code.size();
-        dim3 grid((n[0] + 63) / 64, (n[1] + 3) / 4, n[2]);
-        if (d.dtype == OPF_F64) hipLaunchKernelGGL(sweep_kernel<double>, grid, dim3(64, 4), 0, st, a);
-        else hipLaunchKernelGGL(sweep_kernel<float>, grid, dim3(64, 4), 0, st, a);
-        OPF_HIP(hipGetLastError());
+        OPF_TRY(launch_sweep(st, d.dtype, a));
         if (inplace) std::swap(out->data, out->spare);
     }
     return launch_ghost_fill(out, out->data);

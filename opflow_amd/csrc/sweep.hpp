@@ -59,8 +59,12 @@ struct SweepArgs {
     int ncode;
 };
 
+// unknown_slot: the field slot OPF_N_UNKNOWN reads; pad mode (unknown_as_field = false)
+// emits I_LDU for it, an ordinary sweep (true) reads that slot like any field
 int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
-               const opf_mesh* mesh, int unknown_slot, LoweredProgram* out);
+               const opf_mesh* mesh, int unknown_slot, LoweredProgram* out, bool unknown_as_field = false);
+// the stack-program sweep over the box [a.s, a.s + a.n) on stream st
+int launch_sweep(hipStream_t st, int dtype, const SweepArgs& a);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
                  const int n[3]);
 int mesh_arrays(const opf_mesh* m, int dtype, const void* x[3], const void* dx[3]);
