@@ -120,6 +120,13 @@ int opf_ctx_create(int device, opf_ctx** out);
 int opf_ctx_destroy(opf_ctx* ctx);
 int opf_ctx_sync(opf_ctx* ctx);
 void* opf_ctx_stream(opf_ctx* ctx); /* hipStream_t as void* */
+/* Engine of expression sweeps (opf_sweep and equation right-hand sides). No
+ * counterpart in the reference, whose expressions are C++ templates compiled
+ * ahead of time: 0 = the stack-program interpreter, 1 = (default) a kernel
+ * generated from the expression and compiled at run time (hipRTC, cached in the
+ * process and under $OPF_JIT_CACHE or ~/.cache/opflow_amd/jit) for sweeps of at
+ * least 2^18 cells, 2 = that kernel for every sweep. Both give the same bits. */
+int opf_ctx_set_jit(opf_ctx* ctx, int mode);
 const char* opf_last_error(void);
 const char* opf_version(void);
 

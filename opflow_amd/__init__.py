@@ -81,6 +81,11 @@ class Context:
     def sync(self):
         L.check(self.lib.opf_ctx_sync(self.h))
 
+    def set_jit(self, mode):
+        """Engine of expression sweeps: 0 interpreter, 1 compiled kernels for >= 2^18 cells
+        (default), 2 compiled kernels always (same bits either way)."""
+        L.check(self.lib.opf_ctx_set_jit(self.h, int(mode)))
+
     @property
     def stream(self):
         return self.lib.opf_ctx_stream(self.h)

@@ -96,6 +96,7 @@ SIGNATURES = {
     "opf_ctx_destroy": (_I, [_P]),
     "opf_ctx_sync": (_I, [_P]),
     "opf_ctx_stream": (_P, [_P]),
+    "opf_ctx_set_jit": (_I, [_P, _I]),
     "opf_last_error": (C.c_char_p, []),
     "opf_version": (C.c_char_p, []),
     "opf_mesh_uniform_arrays": (_I, [_I, _D, _D, _I, _DP, _DP]),

@@ -33,7 +33,7 @@ def build(jobs=8):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda so: _run([HIPCC] + FLAGS + ["-c", so[0], "-o", so[1]]), zip(srcs, objs)))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lhiprtc"])
     os.replace(tmp, LIB)
     return LIB
 

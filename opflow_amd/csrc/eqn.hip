@@ -315,7 +315,8 @@ static int eqn_eval_bias(opf_eqn* e) {
     a.old = e->bias;
     a.code = e->progb;
     a.ncode = (int)e->Pb.code.size();
-    return launch_sweep(st, OPF_F64, a);
+    (void)st;
+    return launch_sweep(ctx, OPF_F64, a, e->Pb);
 }
 
 // rows -> multigrid level (face coefficients, sigma); checks symmetry

@@ -56,6 +56,8 @@ struct opf_ctx {
         size_t bytes;
     };
     std::vector<ProgEntry> progs;
+    // expression sweeps: 0 = interpreter, 1 = compiled kernel for >= 2^18 cells, 2 = compiled kernel always
+    int jit_mode = 1;
 };
 
 struct opf_mesh {

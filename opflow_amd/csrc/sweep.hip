@@ -273,42 +273,9 @@ int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields,
 }
 
 // -------------------------------------------------------------- device side
-template <typename T>
-__device__ __forceinline__ T weno_kernel(T v0, T v1, T v2, T v3, T v4, T v5, T h, bool up) {
-    // D1WENO53Upwind::kernel (D1WENO53Upwind.hpp:133-151) takes
-    // (pm2, pm1, p0, pp1, pp2, pp3); D1WENO53Downwind::kernel
-    // (D1WENO53Downwind.hpp:136-154) takes (pm3, pm2, pm1, p0, pp1, pp2).
-    T d1, d2, d3, d4, d5;
-    if (up) {
-        d1 = (v5 - v4) / h; d2 = (v4 - v3) / h; d3 = (v3 - v2) / h; d4 = (v2 - v1) / h; d5 = (v1 - v0) / h;
-    } else {
-        d1 = (v1 - v0) / h; d2 = (v2 - v1) / h; d3 = (v3 - v2) / h; d4 = (v4 - v3) / h; d5 = (v5 - v4) / h;
-    }
-    const double c1312 = 13. / 12.;
-    T ddx1 = d1 / (T)3. - (T)7. * d2 / (T)6. + (T)11. * d3 / (T)6.;
-    T ddx2 = -d2 / (T)6. + (T)5 * d3 / (T)6 + d4 / (T)3;
-    T ddx3 = d3 / (T)3 + (T)5 * d4 / (T)6 - d5 / (T)6;
-    double t1 = (double)(d1 - (T)2 * d2 + d3), t2 = (double)(d1 - (T)4 * d2 + (T)3 * d3);
-    double s1 = c1312 * (t1 * t1) + (t2 * t2) / 4;
-    double t3 = (double)(d2 - (T)2 * d3 + d4), t4 = (double)(d2 - d4);
-    double s2 = c1312 * (t3 * t3) + (t4 * t4) / 4.;
-    double t5 = (double)(d3 - (T)2 * d4 + d5), t6 = (double)((T)3 * d3 - (T)4 * d4 + d5);
-    double s3 = c1312 * (t5 * t5) + (t6 * t6) / 4.;
-    T m = d1 * d1;
-    T q = d2 * d2; m = m < q ? q : m;
-    q = d3 * d3; m = m < q ? q : m;
-    q = d4 * d4; m = m < q ? q : m;
-    q = d5 * d5; m = m < q ? q : m;
-    const double eps = 1e-6 * (double)m + 1e-99;
-    double e1 = s1 + eps, e2 = s2 + eps, e3 = s3 + eps;
-    double a1 = .1 / (e1 * e1);
-    double a2 = .6 / (e2 * e2);
-    double a3 = .3 / (e3 * e3);
-    double w1 = a1 / (a1 + a2 + a3);
-    double w2 = a2 / (a1 + a2 + a3);
-    double w3 = a3 / (a1 + a2 + a3);
-    return (T)(w1 * (double)ddx1 + w2 * (double)ddx2 + w3 * (double)ddx3);
-}
+#define OPF_SHARED_SRC(...) __VA_ARGS__
+#include "weno_device.inc"
+#undef OPF_SHARED_SRC
 
 template <typename T>
 __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
@@ -378,7 +345,10 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
 
 namespace opf {
 
-int launch_sweep(hipStream_t st, int dtype, const SweepArgs& a) {
+int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P) {
+    hipStream_t st = ctx->stream;
+    const long long cells = (long long)a.n[0] * a.n[1] * a.n[2];
+    if (ctx->jit_mode == 2 || (ctx->jit_mode == 1 && cells >= (1LL << 18))) return jit_launch_sweep(st, P, dtype, a);
     dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, a.n[2]);
     if (dtype == OPF_F64) hipLaunchKernelGGL(sweep_kernel<double>, grid, dim3(64, 4), 0, st, a);
     else hipLaunchKernelGGL(sweep_kernel<float>, grid, dim3(64, 4), 0, st, a);
@@ -468,7 +438,7 @@ extern "C" int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_f
         OPF_TRY(upload_program(ctx, P.code.data(), P.code.size() * sizeof(Insn), &dev));
         a.code = dev;
         a.ncode = (int)P.code.size();
-        OPF_TRY(launch_sweep(st, d.dtype, a));
+        OPF_TRY(launch_sweep(ctx, d.dtype, a, P));
         if (inplace) std::swap(out->data, out->spare);
     }
     return launch_ghost_fill(out, out->data);

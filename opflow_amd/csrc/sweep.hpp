@@ -38,33 +38,21 @@ struct LoweredProgram {
     int loc[3];
 };
 
-struct FieldArg {
-    const void* ptr;
-    int off0, off1, off2;
-    int e0, e1;
-};
-
-struct SweepArgs {
-    FieldArg f[SWEEP_MAX_FIELDS];
-    const void* x[3];
-    const void* dx[3];
-    int morg[3];
-    int s[3], n[3];
-    void* out;
-    const void* old;
-    int oo[3];
-    int oe0, oe1;
-    int op;
-    const void* code;
-    int ncode;
-};
+#define OPF_SHARED_SRC(...) __VA_ARGS__
+#include "sweep_args.inc"
+#undef OPF_SHARED_SRC
+static_assert(sizeof(((SweepArgs*)nullptr)->f) / sizeof(FieldArg) == SWEEP_MAX_FIELDS, "field slots");
 
 // unknown_slot: the field slot OPF_N_UNKNOWN reads; pad mode (unknown_as_field = false)
 // emits I_LDU for it, an ordinary sweep (true) reads that slot like any field
 int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
                const opf_mesh* mesh, int unknown_slot, LoweredProgram* out, bool unknown_as_field = false);
-// the stack-program sweep over the box [a.s, a.s + a.n) on stream st
-int launch_sweep(hipStream_t st, int dtype, const SweepArgs& a);
+// the sweep of program P over the box [a.s, a.s + a.n) on the context's stream, by the
+// engine the context's jit mode selects (interpreter or the compiled kernel of P)
+int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P);
+// the compiled kernel of P (hipRTC, cached) and its launch
+int jit_sweep_function(const LoweredProgram& P, int dtype, int op, hipFunction_t* fn);
+int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
                  const int n[3]);
 int mesh_arrays(const opf_mesh* m, int dtype, const void* x[3], const void* dx[3]);

@@ -70,8 +70,15 @@ int main(int argc, char** argv) {
 
     long long iters = 0;
     double worst = 0;
+    // the first step also builds the pressure hierarchy and compiles the expression kernels
+    // (once per process / machine): it is timed on its own, the rest as the steady rate
     auto t0 = std::chrono::steady_clock::now();
+    auto t_first = t0;
     for (int i = 0; i < steps; ++i) {
+        if (i == 1) {
+            opf_ctx_sync(globalContext()->h);
+            t_first = std::chrono::steady_clock::now();
+        }
         du = dt * (nu * (d2x<D2SecondOrderCentered>(u) + d2y<D2SecondOrderCentered>(u)) - (conv_xx(u, u) + conv_xy(u, v)));
         dv = dt * (nu * (d2x<D2SecondOrderCentered>(v) + d2y<D2SecondOrderCentered>(v)) - (conv_yx(u, v) + conv_yy(v, v)));
         u = u + du;
@@ -85,16 +92,18 @@ int main(int argc, char** argv) {
     const auto& hu = u.hostData();
     const auto& hv = v.hostData();
     auto t1 = std::chrono::steady_clock::now();
-    double sec = std::chrono::duration<double>(t1 - t0).count();
+    const double first = std::chrono::duration<double>(t_first - t0).count();
+    const double rest = std::chrono::duration<double>(t1 - t_first).count();
+    const double per_step = steps > 1 ? rest / (steps - 1) : first;
     auto div = ExprBuilder<Field>().setMesh(mesh).setName("div").setLoc(LocOnMesh::Center).setExt(1)
                        .setBC(0, DimPos::start, BCType::Neum, 0.).setBC(0, DimPos::end, BCType::Neum, 0.)
                        .setBC(1, DimPos::start, BCType::Neum, 0.).setBC(1, DimPos::end, BCType::Neum, 0.).build();
     div = dx<D1FirstOrderCentered>(u) + dy<D1FirstOrderCentered>(v);
     double divmax = std::max(std::fabs(rangeReduce(div.assignableRange, MaxOp{}, div)),
                              std::fabs(rangeReduce(div.assignableRange, MinOp{}, div)));
-    std::printf("{\"n\": %d, \"steps\": %d, \"mixed\": %d, \"dt\": %.6e, \"s_per_step\": %.6e, \"poisson_iters\": %lld, "
-                "\"worst_relres\": %.3e, \"max_div\": %.3e, \"u_center\": %.17g}\n",
-                n, steps, (int)mixed, dt, sec / steps, iters, worst, divmax, u.evalAt(DS::MDIndex<2>{n / 2, n / 2}));
+    std::printf("{\"n\": %d, \"steps\": %d, \"mixed\": %d, \"dt\": %.6e, \"s_per_step\": %.6e, \"s_first_step\": %.6e, "
+                "\"poisson_iters\": %lld, \"worst_relres\": %.3e, \"max_div\": %.3e, \"u_center\": %.17g}\n",
+                n, steps, (int)mixed, dt, per_step, first, iters, worst, divmax, u.evalAt(DS::MDIndex<2>{n / 2, n / 2}));
     if (out) {
         FILE* f = std::fopen(out, "wb");
         std::fwrite(hu.data(), sizeof(double), hu.size(), f);

@@ -225,22 +225,31 @@ OPS = {
 }
 
 
+@pytest.fixture(params=["interp", "jit"])
+def engine(ctx, request):
+    """Run a sweep test with the stack-program interpreter and with the kernel compiled
+    from the expression at run time; both must equal the oracle bit for bit."""
+    ctx.set_jit(0 if request.param == "interp" else 2)
+    yield request.param
+    ctx.set_jit(1)
+
+
 @pytest.mark.parametrize("name", list(OPS))
 @pytest.mark.parametrize("spec", [C_DIRC2, C_NEUM2, N_DIRC2], ids=["cdirc", "cneum", "ndirc"])
-def test_sweep_operators_bitexact(ctx, name, spec):
+def test_sweep_operators_bitexact(ctx, engine, name, spec):
     got, ref = _sweep_case(ctx, OPS[name], [spec], [17, 13], None, SEED + 5, ranges=[(0.0, 1.0), (0.0, 2.5)])
     np.testing.assert_array_equal(got, ref)
 
 
 @pytest.mark.parametrize("kind", ["up", "down"])
-def test_sweep_weno_periodic(ctx, kind):
+def test_sweep_weno_periodic(ctx, engine, kind):
     k = of.D1WENO53Upwind if kind == "up" else of.D1WENO53Downwind
     got, ref = _sweep_case(ctx, lambda u: u - 0.001 * (of.dx(k, u) + of.dy(k, u)), [C_PER3], [33, 21], None,
                            SEED + 6)
     np.testing.assert_array_equal(got, ref)
 
 
-def test_sweep_lid_driven_terms(ctx):
+def test_sweep_lid_driven_terms(ctx, engine):
     """Convective terms of examples/LidDriven/LidDriven2D.cpp:30-41 on a MAC grid."""
     nodes = [25, 25]
     u_spec = ([0, 1], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 1.0)], (1, 1))
@@ -261,12 +270,12 @@ def test_sweep_lid_driven_terms(ctx):
 
 
 @pytest.mark.parametrize("op", [1, 2, 3, 4])
-def test_sweep_compound_ops(ctx, op):
+def test_sweep_compound_ops(ctx, engine, op):
     got, ref = _sweep_case(ctx, lambda u, w: w * 0.5 + u, [C_DIRC2, C_NEUM2], [13, 13], C_DIRC2, SEED + 8, op=op)
     np.testing.assert_array_equal(got, ref)
 
 
-def test_sweep_3d_f64_and_range_error(ctx):
+def test_sweep_3d_f64_and_range_error(ctx, engine):
     spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
     lap3 = (lambda u: u + 0.01 * (of.d2x(of.D2SecondOrderCentered, u) + of.d2y(of.D2SecondOrderCentered, u) +
                                   of.d2z(of.D2SecondOrderCentered, u)))
@@ -277,6 +286,27 @@ def test_sweep_3d_f64_and_range_error(ctx):
     u = build_field(ctx, mesh, 2, 1, [(DIRC, DIRC), (DIRC, DIRC)])
     with pytest.raises(of._lib.OpfError):
         u.assign(of.dx(of.D1WENO53Upwind, u))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_sweep_engines_agree_at_size(ctx, dtype):
+    """A 3-D sweep of 96^3 cells (above the size from which the default engine compiles
+    the expression): the compiled kernel and the interpreter give the same bits."""
+    n = 96
+    mesh, om = _mesh(ctx, [n + 1] * 3)
+    spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
+    outs = []
+    for mode in (1, 0):
+        ctx.set_jit(mode)
+        u = build_field(ctx, mesh, 3, spec[0], spec[1], spec[2], ext=spec[3], dtype=dtype)
+        a = oracle.update_padding(u.desc, om, random_storage(u, SEED + 11, 0.5, 1.5)).astype(u.dtype)
+        u.upload(a)
+        D2, D1 = of.D2SecondOrderCentered, of.D1FirstOrderCentered
+        u.assign(u + 1e-5 * (of.d2x(D2, u) + of.d2y(D2, u) + of.d2z(D2, u)) - 1e-3 * u * of.dz(of.D1FirstOrderBiasedUpwind, u))
+        outs.append(u.numpy())
+        u.close()
+    ctx.set_jit(1)
+    np.testing.assert_array_equal(outs[0], outs[1])
 
 
 def test_reduce_matches_oracle(ctx):
