@@ -123,6 +123,55 @@ def cpu_baseline_ftcs3d(n, steps):
     return n ** 3 * steps / dt / 1e6, dt, oracle.num_threads()
 
 
+def generic_sweeps(of, ctx, ev, n):
+    """The generic `u = expr` path (opf_sweep) on the headline shape: the FTCS update
+    written as an expression (3-D 7-point, FP64 and FP32) and a WENO53 advection step
+    (2-D 2048^2, 3 ghost layers), with the compiled-expression engine (default) and the
+    stack-program interpreter. One step per launch; the first (compiling) sweep is untimed."""
+    import numpy as np
+    D2, W = of.D2SecondOrderCentered, of.D1WENO53Upwind
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, ctx.stream)
+        for _ in range(reps):
+            fn()
+        ev.record(e1, ctx.stream)
+        return ev.elapsed_ms(e0, e1) / reps
+
+    out = {}
+    for dtype in ("f64", "f32"):
+        _, u = ftcs_field(of, ctx, n, 3, dtype, of.BCType.Periodic, 0xC0FFEE + 2)
+        expr = u + (0.2 / n / n) * (of.d2x(D2, u) + of.d2y(D2, u) + of.d2z(D2, u))
+        for mode, tag in ((1, "compiled"), (0, "interpreter")):
+            ctx.set_jit(mode)
+            ms = timed(lambda: u.assign(expr), 20 if mode else 3)
+            s = 16.0 if dtype == "f64" else 8.0
+            out[f"sweep_expr_ftcs3d_{n}_{dtype}_{tag}"] = {
+                "Mcell_updates_per_s": n ** 3 / (ms / 1e3) / 1e6, "ms_per_step": ms,
+                "alg_GBps": s * n ** 3 / (ms / 1e3) / 1e9}
+        u.close()
+    ctx.set_jit(1)
+    m = 2048
+    mesh = of.MeshBuilder(ctx).newMesh(m + 1, m + 1).setMeshOfDim(0, 0., 1.).setMeshOfDim(1, 0., 1.).build()
+    b = of.ExprBuilder().setName("u").setMesh(mesh).setLoc(of.LocOnMesh.Center).setExt(3)
+    for d in range(2):
+        b.setBC(d, of.DimPos.start, of.BCType.Periodic).setBC(d, of.DimPos.end, of.BCType.Periodic)
+    u = b.build()
+    u.upload(of.gen_uniform(0xC0FFEE + 7, int(np.prod(u.shape)), -1.0, 1.0).reshape(u.shape))
+    u.updatePadding()
+    v = u.clone("v")
+    expr = u - 1e-4 * (of.dx(W, u) + of.dy(W, u))
+    ms = timed(lambda: v.assign(expr), 20)
+    out[f"sweep_expr_weno53_2d_{m}_f64_compiled"] = {"Mcell_updates_per_s": m * m / (ms / 1e3) / 1e6,
+                                                      "ms_per_step": ms, "alg_GBps": 16.0 * m * m / (ms / 1e3) / 1e9}
+    u.close()
+    v.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +258,7 @@ def main():
         extra["ftcs2d_256_fp64_1000steps"] = {"Mcell_updates_per_s": n2 * n2 * 1000 / (ms2 / 1e3) / 1e6,
                                               "ms_total": ms2}
         u2.close()
+        extra.update(generic_sweeps(of, ctx, ev, n))
         try:
             from bench_solver import solver_extra
             extra.update(solver_extra(of, ctx, ev, cpu=not args.no_cpu))
