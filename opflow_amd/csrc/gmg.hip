@@ -708,6 +708,166 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
 }
 
 
+// ---- coarse tail: every level of at most GMG_TAIL_N cells in one workgroup per slot.
+// A V-cycle's coarse levels are tiny and launch-bound (a handful of cells per
+// CU); here one 1024-thread workgroup runs their smoothing sweeps, residual
+// restrictions, the dense coarsest solve, prolongations and post-smoothing in
+// sequence with barriers instead of launches, on copies of the levels' vectors
+// in LDS (the first tail level's b comes in from HBM, its x goes back). Per cell
+// the arithmetic is that of the per-level kernels' general path (apply_op), and
+// the buffers alternate as in ph_vcycle, so the last sweep of every level lands
+// in its x.
+constexpr int GMG_TAIL_N = 1024;      // cells of the first tail level at most
+constexpr int GMG_TAIL_CELLS = 1400;  // cells of all tail levels together at most (LDS: 3 vectors each)
+constexpr int GMG_TAIL_MAXL = 12;
+constexpr int GMG_TAIL_NT = 1024;
+
+template <typename T>
+struct TailLevel {
+    GOp<T> o;
+    int N;
+    int ofs;  // LDS offset of x (b follows at ofs + N, t at ofs + 2 N)
+};
+
+template <typename T>
+struct TailArgs {
+    TailLevel<T> lv[GMG_TAIL_MAXL];
+    int nl;  // levels in the tail; the last is the coarsest (dense inverse)
+    const T* b0;  // the first tail level's b and x in HBM (slot stride lv[0].N)
+    T* x0;
+    const double* cinv;
+    int cN;
+    T w;
+    int pre, post;
+};
+
+// one weighted Jacobi sweep over a level (zero: from x = 0)
+template <typename T, bool VAR, int DIM>
+__device__ void tail_sweep(const GOp<T>& o, int N, const T* xi, const T* b, T* xo, T w, bool zero) {
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+        int i, j, k;
+        cell_of((uint32_t)t, o.f0, o.f1, i, j, k);
+        T D;
+        if (zero) {
+            auto zv = [&](uint32_t) { return (T)0; };
+            apply_op<T, VAR, DIM>(o, i, j, k, (uint32_t)t, (T)0, zv, D);
+            xo[t] = w * b[t] / D;
+        } else {
+            auto xv = [&](uint32_t l) { return xi[l]; };
+            const T xc = xi[t];
+            const T ax = apply_op<T, VAR, DIM>(o, i, j, k, (uint32_t)t, xc, xv, D);
+            xo[t] = xc + w * (b[t] - ax) / D;
+        }
+    }
+    __syncthreads();
+}
+
+// coarse b = average over the 2^d children of the fine residual b - A x
+template <typename T, bool VAR, int DIM>
+__device__ void tail_restrict(const GOp<T>& o, const T* x, const T* b, T* bc, const GOp<T>& oc) {
+    const int NC = oc.n0 * oc.n1 * oc.n2;
+    auto xv = [&](uint32_t l) { return x[l]; };
+    for (int t = threadIdx.x; t < NC; t += blockDim.x) {
+        int I, J, K;
+        cell_of((uint32_t)t, oc.f0, oc.f1, I, J, K);
+        T s = 0;
+        const int nz = DIM == 3 ? 2 : 1;
+        for (int dk = 0; dk < nz; ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di, j = 2 * J + dj, k = DIM == 3 ? 2 * K + dk : 0;
+                    const uint32_t f = (uint32_t)i + o.s1 * j + o.s2 * k;
+                    T D;
+                    s += b[f] - apply_op<T, VAR, DIM>(o, i, j, k, f, x[f], xv, D);
+                }
+        bc[t] = s * (T)(DIM == 3 ? 0.125 : 0.25);
+    }
+    __syncthreads();
+}
+
+// x_f += P x_c (the general path of k_prolong)
+template <typename T, int DIM>
+__device__ void tail_prolong(const GOp<T>& o, T* xf, const T* xc, const GOp<T>& oc) {
+    const int N = o.n0 * o.n1 * o.n2;
+    const int cn[3] = {oc.n0, oc.n1, oc.n2};
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+        int i, j, k;
+        cell_of((uint32_t)t, o.f0, o.f1, i, j, k);
+        const int f[3] = {i, j, k};
+        int par[3], oth[3];
+        for (int d = 0; d < 3; ++d) {
+            par[d] = f[d] >> 1;
+            oth[d] = (f[d] & 1) ? par[d] + 1 : par[d] - 1;
+        }
+        T s = 0;
+        for (int m = 0; m < (1 << DIM); ++m) {
+            T wgt = 1, fac = 1;
+            int q[3] = {0, 0, 0};
+            for (int d = 0; d < DIM; ++d) {
+                const bool use_oth = (m >> d) & 1;
+                wgt *= use_oth ? (T)0.25 : (T)0.75;
+                int c = use_oth ? oth[d] : par[d];
+                if (c < 0 || c >= cn[d]) {
+                    if (o.per[d]) c = c < 0 ? c + cn[d] : c - cn[d];
+                    else {
+                        fac *= (T)1 - o.bfac[d][c < 0 ? 0 : 1];
+                        c = par[d];
+                    }
+                }
+                q[d] = c;
+            }
+            s += wgt * fac * xc[(uint32_t)q[0] + (uint32_t)cn[0] * ((uint32_t)q[1] + (uint32_t)cn[1] * q[2])];
+        }
+        xf[t] += s;
+    }
+    __syncthreads();
+}
+
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(GMG_TAIL_NT) void k_vtail(TailArgs<T> ta, Bat bt) {
+    __shared__ T lds[3 * GMG_TAIL_CELLS];
+    BAT_SLOT(bt);
+    const int nsw = ta.pre + ta.post;  // sweeps per level; sweep m writes x when (nsw - 1 - m) is even
+    auto X = [&](const TailLevel<T>& L) { return lds + L.ofs; };
+    auto Bv = [&](const TailLevel<T>& L) { return lds + L.ofs + L.N; };
+    auto buf = [&](const TailLevel<T>& L, int m) -> T* { return lds + L.ofs + (((nsw - 1 - m) & 1) ? 2 * L.N : 0); };
+    {
+        const TailLevel<T>& L = ta.lv[0];
+        const T* b0 = ta.b0 + (size_t)slot * L.N;
+        for (int t = threadIdx.x; t < L.N; t += blockDim.x) Bv(L)[t] = b0[t];
+        __syncthreads();
+    }
+    for (int l = 0; l + 1 < ta.nl; ++l) {  // down: pre-smoothing from zero, restriction
+        const TailLevel<T>& L = ta.lv[l];
+        for (int m = 0; m < ta.pre; ++m)
+            tail_sweep<T, VAR, DIM>(L.o, L.N, m ? buf(L, m - 1) : nullptr, Bv(L), buf(L, m), ta.w, m == 0);
+        const TailLevel<T>& C = ta.lv[l + 1];
+        tail_restrict<T, VAR, DIM>(L.o, buf(L, ta.pre - 1), Bv(L), Bv(C), C.o);
+    }
+    {  // coarsest: x = Cinv b in FP64
+        const TailLevel<T>& L = ta.lv[ta.nl - 1];
+        for (int row = threadIdx.x; row < ta.cN; row += blockDim.x) {
+            double s = 0;
+            const double* a = ta.cinv + (long long)row * ta.cN;
+            for (int c = 0; c < ta.cN; ++c) s += a[c] * (double)Bv(L)[c];
+            X(L)[row] = (T)s;
+        }
+        __syncthreads();
+    }
+    for (int l = ta.nl - 2; l >= 0; --l) {  // up: prolongation, post-smoothing
+        const TailLevel<T>& L = ta.lv[l];
+        const TailLevel<T>& C = ta.lv[l + 1];
+        tail_prolong<T, DIM>(L.o, buf(L, ta.pre - 1), X(C), C.o);
+        for (int m = ta.pre; m < nsw; ++m)
+            tail_sweep<T, VAR, DIM>(L.o, L.N, buf(L, m - 1), Bv(L), buf(L, m), ta.w, false);
+    }
+    {
+        const TailLevel<T>& L = ta.lv[0];
+        T* x0 = ta.x0 + (size_t)slot * L.N;
+        for (int t = threadIdx.x; t < L.N; t += blockDim.x) x0[t] = X(L)[t];
+    }
+}
+
 __global__ void k_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ s, double invN,
                          int singular, Bat bt) {
     BAT_SLOT(bt);
@@ -1348,9 +1508,51 @@ static void zp_launch(const Hierarchy& H, int l, hipStream_t st, const Bat& bt, 
                        cb, cq, out0, out1, (const double*)H.scal, part, (T)H.omega, g, cvs, bt);
 }
 
+// first level (>= 1) handled by the coarse tail, or -1: levels of at most GMG_TAIL_N cells,
+// at least two of them (the coarsest included)
+static int tail_level(const Hierarchy& H) {
+    const int nl = (int)H.dev.size();
+    if (H.pre < 1 || H.cN > GMG_TAIL_CELLS) return -1;
+    for (int l = 1; l < nl; ++l) {
+        if (H.host[l].N > GMG_TAIL_N) continue;
+        long long cells = 0;
+        for (int q = l; q < nl; ++q) cells += H.host[q].N;
+        return (nl - l >= 2 && nl - l <= GMG_TAIL_MAXL && cells <= GMG_TAIL_CELLS) ? l : -1;
+    }
+    return -1;
+}
+
+template <typename T>
+static void ph_vtail(const Hierarchy& H, hipStream_t st, const int* mask, int l0) {
+    TailArgs<T> ta{};
+    ta.nl = (int)H.dev.size() - l0;
+    int ofs = 0;
+    for (int q = 0; q < ta.nl; ++q) {
+        const GLevelDev& D = H.dev[l0 + q];
+        ta.lv[q].o = make_op<T>(H, l0 + q);
+        ta.lv[q].N = (int)D.N;
+        ta.lv[q].ofs = ofs;
+        ofs += 3 * (int)D.N;
+    }
+    ta.b0 = (const T*)H.dev[l0].b;
+    ta.x0 = (T*)H.dev[l0].x;
+    ta.cinv = H.cinv;
+    ta.cN = H.cN;
+    ta.w = (T)H.omega;
+    ta.pre = H.pre;
+    ta.post = H.post;
+    const Bat bt = bat(H, mask, H.dev[l0].N);
+    GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_vtail<T, V_, D_>), dim3(1, (unsigned)H.B), dim3(GMG_TAIL_NT), 0,
+                                                  st, ta, bt));
+}
+
 template <typename T>
 void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out,
                const T* dot_q = nullptr) {
+    if (l >= 1 && l == tail_level(H)) {  // rhs / out are this level's b / x
+        ph_vtail<T>(H, st, mask, l);
+        return;
+    }
     const GLevelDev& D = H.dev[l];
     const GLevelHost& L = H.host[l];
     const long long N = L.N;
