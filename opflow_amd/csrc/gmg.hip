@@ -1476,6 +1476,10 @@ void gmg_free(Hierarchy& H) {
         if (p) hipFree(p);
     if (H.hscal) hipHostFree(H.hscal);
     if (H.mk) hipFree(H.mk);
+    if (H.iter_exec) hipGraphExecDestroy(H.iter_exec);
+    if (H.iter_graph) hipGraphDestroy(H.iter_graph);
+    H.iter_exec = nullptr;
+    H.iter_graph = nullptr;
     H.mk = nullptr;
     H.cinv = nullptr;
     H.r = H.z = H.p[0] = H.p[1] = H.q = H.e = nullptr;
@@ -1822,28 +1826,39 @@ int read_scal(const Hierarchy& H, hipStream_t st) {
     return OPF_OK;
 }
 
-// Flexible PCG in precision T on A e = r (slot 0; r is modified, e accumulates).
+// A single solve (slot 0) in Krylov precision T: flexible CG with reliable updates
+// (FP32: FP64 update whenever the recurrence residual fell by the slot's threshold;
+// FP64: at the convergence test only), stopped on the device. Two iterations are
+// replayed as a graph per host poll; a stopped slot skips the work inside them.
 template <typename T>
-static int fcg(const Hierarchy& H, hipStream_t st, T* e, double rr0, double rtol, int maxit, int* iters,
-               double* rr_out) {
-    const long long N = H.host[0].N;
-    ph_fcg_start<T>(H, st, nullptr);
-    int it = 0;
-    double rr = rr0;
-    const int check_every = N > (1 << 20) ? 1 : 2;
-    int pc = 0;
-    for (it = 1; it <= maxit; ++it) {
-        ph_fcg_iter<T>(H, st, nullptr, e, pc);
-        pc = 1 - pc;
-        if (it % check_every == 0 || it == maxit) {
-            OPF_TRY(read_scal(H, st));
-            rr = H.hscal[S_RR];
-            if (!(rr > rtol * rtol * rr0) || !std::isfinite(rr)) break;
-        }
+static int rel_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, int verbose) {
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_seti, dim3(1), dim3(64), 0, s, H.mk + MK_NEW * H.B, H.B, 1);
+    ph_rel_start<T>(H, s, x, b, tol * tol, rel_delta2<T>(), false);
+    OPF_TRY(read_scal(H, s));
+    if (H.hscal[S_DONE] != 0.0) return OPF_OK;
+    if (!H.iter_exec || H.iter_x != x || H.iter_b != b || H.iter_maxit != maxit) {
+        if (H.iter_exec) OPF_HIP(hipGraphExecDestroy(H.iter_exec));
+        if (H.iter_graph) OPF_HIP(hipGraphDestroy(H.iter_graph));
+        H.iter_exec = nullptr;
+        H.iter_graph = nullptr;
+        OPF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        ph_rel_iter<T>(H, s, x, b, 0, maxit);
+        ph_rel_iter<T>(H, s, x, b, 1, maxit);
+        OPF_HIP(hipStreamEndCapture(s, &H.iter_graph));
+        OPF_HIP(hipGraphInstantiate(&H.iter_exec, H.iter_graph, nullptr, nullptr, 0));
+        H.iter_x = x;
+        H.iter_b = b;
+        H.iter_maxit = maxit;
     }
-    OPF_HIP(hipGetLastError());
-    *iters = std::min(it, maxit);
-    *rr_out = rr;
+    for (int it = 0; it < maxit; it += 2) {
+        OPF_HIP(hipGraphLaunch(H.iter_exec, s));
+        OPF_TRY(read_scal(H, s));
+        if (verbose)
+            fprintf(stderr, "[gmg] it %d updates %d relres %.3e\n", (int)H.hscal[S_ITS], (int)H.hscal[S_NREP],
+                    std::sqrt(H.hscal[S_RR] / H.hscal[S_BB]));
+        if (H.hscal[S_DONE] != 0.0) break;
+    }
     return OPF_OK;
 }
 
@@ -1851,49 +1866,10 @@ int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol
               int verbose) {
     OPF_CHECK(H.B == 1, OPF_ERR_ARG, "gmg_solve drives single-slot hierarchies");
     hipStream_t s = ctx->stream;
-    ph_resid64<double>(H, s, nullptr, nullptr, b, (double*)nullptr, FIN_BB);
-    int total = 0, nref = 0;
+    if (H.precision == OPF_PREC_MIXED) OPF_TRY(rel_solve<float>(ctx, H, b, x, tol, maxit, verbose));
+    else OPF_TRY(rel_solve<double>(ctx, H, b, x, tol, maxit, verbose));
+    const int total = (int)H.hscal[S_ITS], nref = (int)H.hscal[S_NREP];
     double rr = 0, bb = 0;
-    if (H.precision == OPF_PREC_FP64) {
-        double* r = (double*)H.r;
-        ph_resid64<double>(H, s, nullptr, x, b, r, FIN_RR0);
-        OPF_TRY(read_scal(H, s));
-        bb = H.hscal[S_BB];
-        double rr0 = H.hscal[S_RR0];
-        if (H.singular) ph_project<double>(H, s, nullptr, r);
-        double target = tol * tol * bb;
-        if (rr0 > target && bb > 0) {
-            int its = 0;
-            double rrf = 0;
-            OPF_TRY(fcg<double>(H, s, x, rr0, std::sqrt(target / rr0), maxit, &its, &rrf));
-            total = its;
-        }
-        nref = 1;
-    } else {
-        // FP32 flexible CG with reliable updates: x (FP64) += e and r = b - A x (FP64)
-        // whenever the FP32 recurrence residual has dropped by 1e2 (or looks converged);
-        // the search direction continues across updates (no restart)
-        hipLaunchKernelGGL(k_seti, dim3(1), dim3(64), 0, s, H.mk + MK_NEW * H.B, H.B, 1);
-        ph_rel_start<float>(H, s, x, b, tol * tol, rel_delta2<float>(), false);
-        OPF_TRY(read_scal(H, s));
-        bb = H.hscal[S_BB];
-        // stopped slots skip the remaining launches of an iteration, so big grids poll every iteration
-        const int check_every = H.host[0].N > (1 << 20) ? 1 : 2;
-        int pc = 0;
-        for (int it = 0; it < maxit && H.hscal[S_DONE] == 0.0; ++it) {
-            ph_rel_iter<float>(H, s, x, b, pc, maxit);
-            pc = 1 - pc;
-            if ((it + 1) % check_every == 0 || it + 1 == maxit) {
-                OPF_TRY(read_scal(H, s));
-                if (verbose)
-                    fprintf(stderr, "[gmg mixed] it %d updates %d relres %.3e\n", (int)H.hscal[S_ITS],
-                            (int)H.hscal[S_NREP], std::sqrt(H.hscal[S_RR] / bb));
-            }
-        }
-        OPF_TRY(read_scal(H, s));
-        total = (int)H.hscal[S_ITS];
-        nref = (int)H.hscal[S_NREP];
-    }
     // final true residual in FP64
     ph_resid64<double>(H, s, nullptr, x, b, H.r64, FIN_RR);
     OPF_TRY(read_scal(H, s));

@@ -89,6 +89,13 @@ struct Hierarchy {
     double* scal = nullptr; // device scalars
     double* hscal = nullptr;  // pinned host mirror
     int* mk = nullptr;        // device slot masks [MK_COUNT][B]
+    // single solves: two device-controlled iterations captured as a graph for the
+    // (x, b, maxit) they were captured with
+    hipGraph_t iter_graph = nullptr;
+    hipGraphExec_t iter_exec = nullptr;
+    const void* iter_x = nullptr;
+    const void* iter_b = nullptr;
+    int iter_maxit = 0;
     size_t bytes = 0;
 };
 
