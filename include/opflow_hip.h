@@ -104,7 +104,8 @@ enum {
     OPF_N_WENO_DOWN = 12, /* D1WENO53Downwind                 D1WENO53Downwind.hpp:75-154 */
     OPF_N_INTP_C2N = 13,  /* D1Linear<dim, Cen2Cor>           Interpolator/D1Linear.hpp:34-42 */
     OPF_N_INTP_N2C = 14,  /* D1Linear<dim, Cor2Cen>           Interpolator/D1Linear.hpp:43-45 */
-    OPF_N_UNKNOWN = 15    /* the StencilField of the solve target (StencilField.hpp:123-350) */
+    OPF_N_UNKNOWN = 15    /* the StencilField of the solve target (StencilField.hpp:123-350);
+                             a = the target's index in an equation set (0 otherwise) */
 };
 typedef struct {
     int kind;
@@ -280,6 +281,40 @@ int opf_eqn_solve(opf_eqn* e, opf_solve_state* state);
 int opf_eqn_csr(opf_eqn* e, long long* nrows, long long* nnz, long long* ptr, long long* col, double* val,
                 double* rhs);
 int opf_eqn_destroy(opf_eqn* e);
+
+/* One equation of a set: `tree(lhs_root) == tree(rhs_root)` over post-order
+ * nodes; OPF_N_FIELD slots index fields[]; OPF_N_UNKNOWN's `a` is the index of
+ * the target (in the set's targets[]) whose StencilField it stands for, so an
+ * equation may read several unknowns (EqnHolder's getters take every target's
+ * StencilField, Equation/EquationHolder.hpp:31-99).                        */
+typedef struct {
+    const opf_expr_node* nodes;
+    int nnodes;
+    int lhs_root, rhs_root;
+    opf_field* const* fields;
+    int nfields;
+} opf_eqn_desc;
+/* Replaces SolveEqns<Solver>(getters..., targets..., mapper, params)
+ * (Equation/UnifiedSolve.hpp:69-100): equation i is solved for targets[i]
+ * (FP64, centre-located, one shared layout, at most 16). The system is the one
+ * CSRMatrixGenerator builds (opf_eqns_csr), solved from x = 0 to params[0].tol
+ * (default 1e-8) on the whole system's relative residual, then every target is
+ * written back and its padding updated. params[i].pin_value pins block i's
+ * last unknown at 0. Blocks that read only their own unknown are solved by
+ * multigrid, blocks with equal operators batched as slots of one hierarchy;
+ * coupled sets by BiCGStab preconditioned with one V-cycle per diagonal block.
+ * state: Krylov iterations (the most of any block when block-diagonal) and the
+ * whole system's residual.                                                  */
+int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, const opf_solve_params* params,
+                   opf_solve_state* state);
+/* Replaces CSRMatrixGenerator::generate(StencilHolder, ColoredMDRangeMapper,
+ * pin) (Equation/CSRMatrixGenerator.hpp:26-41, 55-193) for a set: block i
+ * holds targets[i]'s cells x-fastest (ColoredMDRangeMapper.hpp:20-41); a row
+ * holds the (unknown, cell) entries the equation's pad touches, columns
+ * ascending (block offset + cell rank), rhs = -bias; pin[i] makes block i's
+ * LAST row x = 0. pin may be null. Null arrays return nrows / nnz only.      */
+int opf_eqns_csr(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, const int* pin, long long* nrows,
+                 long long* nnz, long long* ptr, long long* col, double* val, double* rhs);
 
 /* Direct multigrid API on device vectors (used by the serving loop and the
  * benchmarks): constant-coefficient operator

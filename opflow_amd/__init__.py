@@ -22,7 +22,8 @@ __all__ = [
     "D1FirstOrderBiasedUpwind", "D1FirstOrderBiasedDownwind", "D1WENO53Upwind", "D1WENO53Downwind",
     "d2x", "d2y", "d2z", "dx", "dy", "dz", "d1", "d2", "d1IntpCenterToCorner", "d1IntpCornerToCenter",
     "rangeReduce", "StructSolverParams", "EqnSolveState", "Solve", "makeEqnSolveHandler", "PrecondStructSolver",
-    "PoissonGMG", "Server",
+    "PoissonGMG", "Server", "SolveEqns", "makeEqnHolder", "makeStencilHolder", "CSRMatrixGenerator",
+    "ColoredMDRangeMapper", "IJSolverParams", "StencilUnknown",
 ]
 
 
@@ -208,7 +209,9 @@ class Expr:
         def visit(e):
             if id(e) in memo:
                 return memo[id(e)]
-            if e.kind == _N_FIELD and unknown is not None and e.field is unknown:
+            if e.kind == _N_UNKNOWN:  # a set's StencilField placeholder: a = the target's index
+                node = (_N_UNKNOWN, int(e.a), 0, 0, 0.0)
+            elif e.kind == _N_FIELD and unknown is not None and e.field is unknown:
                 node = (_N_UNKNOWN, 0, 0, 0, 0.0)
             elif e.kind == _N_FIELD:
                 key = id(e.field)
@@ -228,6 +231,14 @@ class Expr:
 
         root = visit(self)
         return nodes, fields, root
+
+
+class StencilUnknown(Expr):
+    """The StencilField of target `index` of an equation set, as EqnHolder hands it to
+    every getter (Equation/EquationHolder.hpp:31-99)."""
+
+    def __init__(self, index, target):
+        super().__init__(_N_UNKNOWN, a=int(index), field=target)
 
 
 class Equation:
@@ -520,4 +531,5 @@ def gen_uniform(seed, n, lo=0.0, hi=1.0):
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
                     PrecondStructSolver, PoissonGMG, Server, DeviceBuffer, poisson_desc, gen_bumps_grid,
-                    gen_bumps_grid_dev)
+                    gen_bumps_grid_dev, SolveEqns, makeEqnHolder, makeStencilHolder, CSRMatrixGenerator,
+                    ColoredMDRangeMapper, IJSolverParams)

@@ -58,6 +58,11 @@ class SolveParams(C.Structure):
     ]
 
 
+class EqnDesc(C.Structure):
+    _fields_ = [("nodes", C.POINTER(ExprNode)), ("nnodes", C.c_int), ("lhs_root", C.c_int), ("rhs_root", C.c_int),
+                ("fields", C.POINTER(C.c_void_p)), ("nfields", C.c_int)]
+
+
 class SolveState(C.Structure):
     _fields_ = [("niter", C.c_int), ("relerr", C.c_double), ("abserr", C.c_double),
                 ("nrefine", C.c_int), ("converged", C.c_int)]
@@ -131,6 +136,9 @@ SIGNATURES = {
     "opf_eqn_solve": (_I, [_P, C.POINTER(SolveState)]),
     "opf_eqn_csr": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _P, _P, _P, _P]),
     "opf_eqn_destroy": (_I, [_P]),
+    "opf_eqns_solve": (_I, [_I, C.POINTER(EqnDesc), C.POINTER(_P), C.POINTER(SolveParams), C.POINTER(SolveState)]),
+    "opf_eqns_csr": (_I, [_I, C.POINTER(EqnDesc), C.POINTER(_P), _P, C.POINTER(C.c_longlong),
+                          C.POINTER(C.c_longlong), _P, _P, _P, _P]),
     "opf_gmg_create": (_I, [_P, C.POINTER(PoissonDesc), _I, C.POINTER(_P)]),
     "opf_gmg_solve": (_I, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveState)]),
     "opf_gmg_apply": (_I, [_P, _P, _P]),

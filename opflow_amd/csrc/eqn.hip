@@ -23,6 +23,13 @@ namespace opf {
 constexpr int PAD_K = 7;       // centre + 2 per dim
 constexpr int PAD_W = PAD_K + 2;  // coefficients, bias, linear flag
 constexpr int PAD_BLOCK = 32;  // 16-deep stack of 9 doubles per lane: 36 KB of LDS
+constexpr int EQS_MAX = 16;    // unknowns of an equation set
+
+// boundary handling of one unknown: periodic dims and the ghost rule's affine constant
+struct PadBC {
+    int per[3];
+    double beta[3][2];
+};
 
 struct PadArgs {
     FieldArg f[SWEEP_MAX_FIELDS];
@@ -30,9 +37,9 @@ struct PadArgs {
     const double* dx[3];
     int morg[3];
     int s[3], n[3];      // row box (target interior)
-    int tl[3], te[3];    // target local range [tl, te)
-    int per[3];
-    double beta[3][2];   // ghost rule affine constants
+    int tl[3], te[3];    // target local range [tl, te) (every unknown has this layout)
+    PadBC bc[EQS_MAX];   // per unknown
+    int live;            // unknown whose coefficients are kept; -1: every unknown, in shared slots
     const void* code;
     int ncode;
     double* coef;        // [PAD_K][N]
@@ -78,17 +85,20 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
                 t[PAD_K] = ((const double*)F.ptr)[base + in.b];
             } break;
             case I_LDU: {
+                // another unknown than the live one still counts as linear (so products of
+                // unknowns are caught) and still brings its boundary constant to the bias
+                const int u = in.a;
                 int ox = (in.c & 31) - 8, oy = ((in.c >> 5) & 31) - 8, oz = ((in.c >> 10) & 31) - 8;
                 push();
                 int sl = slot_of(ox, oy, oz);
                 if (sl < 0) { bad |= 1; break; }
-                t[sl] = 1.0;
+                if (a.live < 0 || u == a.live) t[sl] = 1.0;
                 t[PAD_K + 1] = 1.0;
                 int p[3] = {ci + ox, cj + oy, ck + oz};
                 for (int d = 0; d < 3; ++d) {
                     if (p[d] >= a.tl[d] && p[d] < a.te[d]) continue;
                     int side = p[d] < a.tl[d] ? 0 : 1;
-                    if (!a.per[d]) t[PAD_K] = a.beta[d][side];
+                    if (!a.bc[u].per[d]) t[PAD_K] = a.bc[u].beta[d][side];
                 }
             } break;
             case I_CONST: push(); t[PAD_K] = in.imm; break;
@@ -197,6 +207,35 @@ __global__ void k_sum_rows(long long N, const double* __restrict__ b, double* __
 }
 __global__ void k_copy1(const double* __restrict__ x, long long i, double* __restrict__ out) { out[0] = x[i]; }
 
+// x_ghost = alpha * x_mirror + beta per face of a centre-located target (the
+// folding StencilField::evalAtImpl_final applies, StencilField.hpp:123-350)
+static int target_bc(const opf_field* t, int per[3], double alpha[3][2], double beta[3][2]) {
+    GhostInfo gi;
+    OPF_TRY(build_ghost_info(t, &gi));
+    for (int d = 0; d < 3; ++d) {
+        per[d] = 0;
+        for (int s = 0; s < 2; ++s) {
+            alpha[d][s] = 1.0;
+            beta[d][s] = 0.0;
+        }
+    }
+    for (int d = 0; d < t->d.dim; ++d) {
+        per[d] = gi.per[d];
+        for (int s = 0; s < 2; ++s) {
+            const GhostRule& r = gi.rule[d][s][0];
+            switch (r.type) {
+                case 1: alpha[d][s] = r.c1 / r.c3; beta[d][s] = (0.0 * r.c1 - r.c2) / r.c3; break;
+                case 2: alpha[d][s] = 1.0; beta[d][s] = r.c1; break;
+                case 3: alpha[d][s] = 1.0; beta[d][s] = 0.0; break;
+                case 4: alpha[d][s] = -1.0; beta[d][s] = 0.0; break;
+                case 5: break;
+                default: set_error("implicit solve: every boundary needs a BC"); return OPF_ERR_UNSUPPORTED;
+            }
+        }
+    }
+    return OPF_OK;
+}
+
 }  // namespace opf
 
 using namespace opf;
@@ -250,10 +289,11 @@ static int eqn_eval_pads(opf_eqn* e) {
         a.n[q] = e->n[q];
         a.tl[q] = td.local_start[q];
         a.te[q] = td.local_end[q];
-        a.per[q] = e->per[q];
-        a.beta[q][0] = e->beta[q][0];
-        a.beta[q][1] = e->beta[q][1];
+        a.bc[0].per[q] = e->per[q];
+        a.bc[0].beta[q][0] = e->beta[q][0];
+        a.bc[0].beta[q][1] = e->beta[q][1];
     }
+    a.live = -1;
     a.code = e->prog;
     a.ncode = (int)e->P.code.size();
     a.coef = e->coef;
@@ -319,34 +359,33 @@ static int eqn_eval_bias(opf_eqn* e) {
     return launch_sweep(ctx, OPF_F64, a, e->Pb);
 }
 
-// rows -> multigrid level (face coefficients, sigma); checks symmetry
-static int eqn_build_operator(opf_eqn* e) {
-    const long long N = e->N;
-    const int dim = e->target->d.dim;
-    std::vector<double> c((size_t)PAD_K * N);
-    OPF_HIP(hipMemcpyAsync(c.data(), e->coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, e->ctx->stream));
-    OPF_HIP(hipStreamSynchronize(e->ctx->stream));
+// The rows of one diagonal block (PAD_K coefficient arrays of N rows, host copy) as
+// a multigrid level: face coefficients from the neighbour slots (checked
+// symmetric), the rest of the diagonal as sigma. *sign orients the diagonal
+// positive; a uniform operator becomes the matrix-free constant mode.
+static int rows_to_level(const std::vector<double>& c, long long N, const int nn[3], int dim, const int per[3],
+                         double* sign, GLevelHost& L) {
     // orientation: diagonal must be positive after scaling by sign
     double dsum = 0;
     for (long long r = 0; r < N; ++r) dsum += c[r];
     OPF_CHECK(dsum != 0.0, OPF_ERR_UNSUPPORTED, "equation has no diagonal");
-    e->sign = dsum > 0 ? 1.0 : -1.0;
-    const double s = e->sign;
-    GLevelHost L;
+    *sign = dsum > 0 ? 1.0 : -1.0;
+    const double s = *sign;
+    L = GLevelHost{};
     L.var = true;
-    for (int q = 0; q < 3; ++q) L.n[q] = e->n[q];
+    for (int q = 0; q < 3; ++q) L.n[q] = nn[q];
     L.N = N;
-    auto cidx = [&](int i, int j, int k) { return i + (long long)e->n[0] * (j + (long long)e->n[1] * k); };
+    auto cidx = [&](int i, int j, int k) { return i + (long long)nn[0] * (j + (long long)nn[1] * k); };
     double maxa = 0;
     for (int d = 0; d < dim; ++d) {
-        int fn[3] = {e->n[0] + (d == 0), e->n[1] + (d == 1), e->n[2] + (d == 2)};
+        int fn[3] = {nn[0] + (d == 0), nn[1] + (d == 1), nn[2] + (d == 2)};
         L.af[d].assign((size_t)fn[0] * fn[1] * fn[2], 0.0);
         for (int k = 0; k < fn[2]; ++k)
             for (int j = 0; j < fn[1]; ++j)
                 for (int i = 0; i < fn[0]; ++i) {
                     int p[3] = {i, j, k};
                     double alo = NAN, ahi = NAN;  // from the row above (its lo face) and below (its hi face)
-                    if (p[d] < e->n[d]) alo = -s * c[(size_t)(1 + 2 * d) * N + cidx(i, j, k)];
+                    if (p[d] < nn[d]) alo = -s * c[(size_t)(1 + 2 * d) * N + cidx(i, j, k)];
                     if (p[d] > 0) {
                         int q[3] = {i, j, k};
                         q[d] -= 1;
@@ -361,14 +400,14 @@ static int eqn_build_operator(opf_eqn* e) {
                     L.af[d][(size_t)i + (size_t)fn[0] * (j + (size_t)fn[1] * k)] = a;
                     maxa = std::max(maxa, std::fabs(a));
                 }
-        if (e->per[d]) {  // face 0 and face n are the same physical face
+        if (per[d]) {  // face 0 and face n are the same physical face
             for (int k = 0; k < fn[2]; ++k)
                 for (int j = 0; j < fn[1]; ++j)
                     for (int i = 0; i < fn[0]; ++i) {
                         int p[3] = {i, j, k};
                         if (p[d] != 0) continue;
                         int q[3] = {i, j, k};
-                        q[d] = e->n[d];
+                        q[d] = nn[d];
                         double& f0 = L.af[d][(size_t)p[0] + (size_t)fn[0] * (p[1] + (size_t)fn[1] * p[2])];
                         double fnv = L.af[d][(size_t)q[0] + (size_t)fn[0] * (q[1] + (size_t)fn[1] * q[2])];
                         OPF_CHECK(std::fabs(f0 - fnv) <= 1e-12 * std::max(std::fabs(f0), std::fabs(fnv)),
@@ -377,14 +416,14 @@ static int eqn_build_operator(opf_eqn* e) {
         }
     }
     L.sg.assign(N, 0.0);
-    for (int k = 0; k < e->n[2]; ++k)
-        for (int j = 0; j < e->n[1]; ++j)
-            for (int i = 0; i < e->n[0]; ++i) {
+    for (int k = 0; k < nn[2]; ++k)
+        for (int j = 0; j < nn[1]; ++j)
+            for (int i = 0; i < nn[0]; ++i) {
                 long long r = cidx(i, j, k);
                 double sum = 0;
                 int p[3] = {i, j, k};
                 for (int d = 0; d < dim; ++d) {
-                    int fn0 = e->n[0] + (d == 0), fn1 = e->n[1] + (d == 1);
+                    int fn0 = nn[0] + (d == 0), fn1 = nn[1] + (d == 1);
                     int lo[3] = {i, j, k}, hi[3] = {i, j, k};
                     hi[d] += 1;
                     sum += L.af[d][(size_t)lo[0] + (size_t)fn0 * (lo[1] + (size_t)fn1 * lo[2])];
@@ -414,6 +453,17 @@ static int eqn_build_operator(opf_eqn* e) {
         C.sigma = L.sg[0];
         L = C;
     }
+    return OPF_OK;
+}
+
+// rows -> multigrid hierarchy of a single equation
+static int eqn_build_operator(opf_eqn* e) {
+    const long long N = e->N;
+    std::vector<double> c((size_t)PAD_K * N);
+    OPF_HIP(hipMemcpyAsync(c.data(), e->coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, e->ctx->stream));
+    OPF_HIP(hipStreamSynchronize(e->ctx->stream));
+    GLevelHost L;
+    OPF_TRY(rows_to_level(c, N, e->n, e->target->d.dim, e->per, &e->sign, L));
     if (e->built) gmg_free(e->H);
     e->H = Hierarchy{};
     if (e->params.pre_smooth > 0) e->H.pre = e->params.pre_smooth;
@@ -464,28 +514,10 @@ extern "C" int opf_eqn_create(const opf_expr_node* nodes, int nnodes, int lhs_ro
         delete e;
         return rc;
     }
-    GhostInfo gi;
-    rc = build_ghost_info(target, &gi);
+    rc = target_bc(target, e->per, e->alpha, e->beta);
     if (rc != OPF_OK) {
         delete e;
         return rc;
-    }
-    for (int d = 0; d < td.dim; ++d) {
-        e->per[d] = gi.per[d];
-        for (int s = 0; s < 2; ++s) {
-            const GhostRule& r = gi.rule[d][s][0];
-            switch (r.type) {
-                case 1: e->alpha[d][s] = r.c1 / r.c3; e->beta[d][s] = (0.0 * r.c1 - r.c2) / r.c3; break;
-                case 2: e->alpha[d][s] = 1.0; e->beta[d][s] = r.c1; break;
-                case 3: e->alpha[d][s] = 1.0; e->beta[d][s] = 0.0; break;
-                case 4: e->alpha[d][s] = -1.0; e->beta[d][s] = 0.0; break;
-                case 5: break;
-                default:
-                    delete e;
-                    set_error("implicit solve: every boundary needs a BC");
-                    return OPF_ERR_UNSUPPORTED;
-            }
-        }
     }
     opf_ctx* ctx = e->ctx;
     rc = upload_program(ctx, e->P.code.data(), e->P.code.size() * sizeof(Insn), &e->prog);
@@ -627,4 +659,629 @@ extern "C" int opf_eqn_destroy(opf_eqn* e) {
         if (q) hipFree(q);
     delete e;
     return OPF_OK;
+}
+
+// ------------------------------------------------------------------ equation sets
+// SolveEqns (UnifiedSolve.hpp:69-100) with EqnHolder / StencilHolder and the
+// multi-equation CSRMatrixGenerator::generate (CSRMatrixGenerator.hpp:26-41,
+// 55-193): equation i is lhs_i - rhs_i over target i's rows; OPF_N_UNKNOWN's
+// `a` names the target whose StencilField it stands for, so an equation may read
+// several unknowns. Rows and columns follow ColoredMDRangeMapper
+// (ColoredMDRangeMapper.hpp:20-41): block i holds target i's cells x-fastest,
+// column = block offset of the unknown + its cell rank.
+namespace opf {
+
+struct EqsBC {
+    int per[3];
+    double alpha[3][2];
+};
+
+// y (+)= s * (block row of coefficients c applied to x): slot q of row r multiplies
+// the unknown at the slot's neighbour; a neighbour outside the box is the ghost,
+// alpha * (this cell) (its beta is in the bias), or the periodic image
+__global__ void k_pad_apply(long long N, int n0, int n1, int n2, int dim, const double* __restrict__ c, EqsBC bc,
+                            const double* __restrict__ x, double* __restrict__ y, double s, int acc) {
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int n[3] = {n0, n1, n2};
+    int p[3] = {(int)(r % n0), (int)((r / n0) % n1), (int)(r / ((long long)n0 * n1))};
+    const long long st[3] = {1, n0, (long long)n0 * n1};
+    const double xr = x[r];
+    double v = c[r] * xr;
+    for (int d = 0; d < dim; ++d)
+        for (int sd = 0; sd < 2; ++sd) {
+            const double a = c[(long long)(1 + 2 * d + sd) * N + r];
+            const bool edge = sd == 0 ? p[d] == 0 : p[d] == n[d] - 1;
+            double xn;
+            if (!edge) xn = x[r + (sd == 0 ? -st[d] : st[d])];
+            else if (bc.per[d]) xn = x[r + (sd == 0 ? 1 : -1) * (long long)(n[d] - 1) * st[d]];
+            else xn = bc.alpha[d][sd] * xr;
+            v += a * xn;
+        }
+    y[r] = (acc ? y[r] : 0.0) + s * v;
+}
+__global__ void k_pin_row(double* __restrict__ y, const double* __restrict__ x, long long i) { y[i] = x[i]; }
+__global__ void k_sub_scalar(long long N, double* __restrict__ x, const double* __restrict__ s) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] -= s[0];
+}
+// partials of (a, b) and, when c is given, (c, d): one per block, fixed order
+__global__ void k_dot2(long long M, const double* __restrict__ a, const double* __restrict__ b,
+                       const double* __restrict__ c, const double* __restrict__ d, double* __restrict__ part) {
+    __shared__ double s1[256], s2[256];
+    double u = 0, w = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < M; t += (long long)gridDim.x * blockDim.x) {
+        u += a[t] * b[t];
+        if (c) w += c[t] * d[t];
+    }
+    s1[threadIdx.x] = u;
+    s2[threadIdx.x] = w;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s1[0];
+        part[gridDim.x + blockIdx.x] = s2[0];
+    }
+}
+// p = r + beta (p - omega v)
+__global__ void k_bicg_p(long long M, double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ v,
+                         double beta, double omega) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < M) p[t] = r[t] + beta * (p[t] - omega * v[t]);
+}
+// y = x + a z
+__global__ void k_axpy(long long M, double* __restrict__ y, const double* __restrict__ x, double a,
+                       const double* __restrict__ z) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < M) y[t] = x[t] + a * z[t];
+}
+// x += a p + w s
+__global__ void k_bicg_x(long long M, double* __restrict__ x, double a, const double* __restrict__ p, double w,
+                         const double* __restrict__ s) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < M) x[t] += a * p[t] + w * s[t];
+}
+
+struct EqsSystem {
+    opf_ctx* ctx = nullptr;
+    int neq = 0, dim = 2;
+    std::vector<opf_field*> tg;
+    int s[3], n[3];
+    long long N = 0;
+    int per[EQS_MAX][3];
+    double alpha[EQS_MAX][3][2], beta[EQS_MAX][3][2];
+    std::vector<std::vector<opf_field*>> fl;  // per equation: its fields, then the unknowns' layout
+    std::vector<LoweredProgram> P;
+    std::vector<void*> prog;
+    std::vector<std::vector<int>> refs;  // unknowns each equation reads, ascending
+    std::vector<int> blk;                // i * neq + j -> coefficient block, -1 if eq i does not read unknown j
+    std::vector<unsigned> touched;       // per block: pad slots the equation's program reads
+    int nblk = 0;
+    double* coef = nullptr;  // [nblk][PAD_K][N]
+    double* bias = nullptr;  // [neq][N]
+    int* err = nullptr;
+    ~EqsSystem() {
+        if (ctx) hipStreamSynchronize(ctx->stream);
+        if (coef) hipFree(coef);
+        if (bias) hipFree(bias);
+        if (err) hipFree(err);
+    }
+};
+
+static int eqs_setup(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, EqsSystem& S) {
+    OPF_CHECK(neq >= 1 && neq <= EQS_MAX && eqs && targets, OPF_ERR_ARG, "equation set: 1..%d equations", EQS_MAX);
+    const opf_field* t0 = targets[0];
+    OPF_CHECK(t0, OPF_ERR_ARG, "equation set: null target");
+    const opf_field_desc& d0 = t0->d;
+    for (int i = 0; i < neq; ++i) {
+        const opf_field* t = targets[i];
+        OPF_CHECK(t && t->mesh == t0->mesh && t->d.dtype == OPF_F64, OPF_ERR_ARG,
+                  "equation set: targets must be FP64 fields on one mesh");
+        const opf_field_desc& d = t->d;
+        bool same = d.dim == d0.dim;
+        for (int q = 0; q < 3; ++q)
+            same &= d.extent[q] == d0.extent[q] && d.offset[q] == d0.offset[q] &&
+                    d.local_start[q] == d0.local_start[q] && d.local_end[q] == d0.local_end[q] &&
+                    d.assign_start[q] == d0.assign_start[q] && d.assign_end[q] == d0.assign_end[q];
+        OPF_CHECK(same, OPF_ERR_UNSUPPORTED, "equation set: targets must share one layout and range");
+        for (int q = 0; q < d.dim; ++q)
+            OPF_CHECK(d.loc[q] == OPF_LOC_CENTER, OPF_ERR_UNSUPPORTED, "implicit solve supports centre-located targets");
+        OPF_TRY(target_bc(t, S.per[i], S.alpha[i], S.beta[i]));
+    }
+    S.ctx = t0->ctx;
+    S.neq = neq;
+    S.dim = d0.dim;
+    S.tg.assign(targets, targets + neq);
+    for (int q = 0; q < 3; ++q) {
+        S.s[q] = std::max(d0.assign_start[q], d0.local_start[q]);
+        S.n[q] = std::min(d0.assign_end[q], d0.local_end[q]) - S.s[q];
+    }
+    S.N = (long long)S.n[0] * S.n[1] * S.n[2];
+    OPF_CHECK(S.N > 0, OPF_ERR_ARG, "equation set: empty range");
+    S.fl.resize(neq);
+    S.P.resize(neq);
+    S.prog.assign(neq, nullptr);
+    S.refs.resize(neq);
+    S.blk.assign((size_t)neq * neq, -1);
+    for (int i = 0; i < neq; ++i) {
+        const opf_eqn_desc& E = eqs[i];
+        OPF_CHECK(E.nodes && E.nnodes > 0 && E.nfields >= 0 && (E.fields || E.nfields == 0), OPF_ERR_ARG,
+                  "equation set: equation %d is empty", i);
+        OPF_CHECK(E.nfields + 1 <= SWEEP_MAX_FIELDS, OPF_ERR_UNSUPPORTED, "equation %d reads too many fields", i);
+        for (int f = 0; f < E.nfields; ++f) {
+            OPF_CHECK(E.fields[f] && E.fields[f]->mesh == t0->mesh && E.fields[f]->d.dtype == OPF_F64, OPF_ERR_ARG,
+                      "equation fields must share the targets' mesh and be FP64");
+            S.fl[i].push_back(E.fields[f]);
+        }
+        S.fl[i].push_back(targets[0]);  // the layout every unknown has
+        std::vector<opf_expr_node> all(E.nodes, E.nodes + E.nnodes);
+        all.push_back({OPF_N_SUB, E.lhs_root, E.rhs_root, 0, 0.0});
+        std::vector<int> uslot(neq, E.nfields);
+        OPF_TRY(lower_tree_set(all.data(), (int)all.size(), (int)all.size() - 1, S.fl[i].data(), (int)S.fl[i].size(),
+                               t0->mesh, uslot.data(), neq, &S.P[i]));
+        OPF_TRY(check_bounds(S.P[i], S.fl[i].data(), (int)S.fl[i].size(), t0->mesh, S.s, S.n));
+        OPF_TRY(upload_program(S.ctx, S.P[i].code.data(), S.P[i].code.size() * sizeof(Insn), &S.prog[i]));
+        std::vector<unsigned> mask(neq, 0u);
+        for (const Insn& in : S.P[i].code) {
+            if (in.op != I_LDU) continue;
+            int ox = (in.c & 31) - 8, oy = ((in.c >> 5) & 31) - 8, oz = ((in.c >> 10) & 31) - 8;
+            int sl = -1;
+            if (ox == 0 && oy == 0 && oz == 0) sl = 0;
+            else if (oy == 0 && oz == 0 && (ox == -1 || ox == 1)) sl = ox == -1 ? 1 : 2;
+            else if (ox == 0 && oz == 0 && (oy == -1 || oy == 1)) sl = oy == -1 ? 3 : 4;
+            else if (ox == 0 && oy == 0 && (oz == -1 || oz == 1)) sl = oz == -1 ? 5 : 6;
+            OPF_CHECK(sl >= 0, OPF_ERR_UNSUPPORTED,
+                      "equation %d reads an unknown off the 5/7-point axis stencil (not a Laplacian-type operator)", i);
+            mask[in.a] |= 1u << sl;
+        }
+        for (int j = 0; j < neq; ++j)
+            if (mask[j]) {
+                S.refs[i].push_back(j);
+                S.blk[(size_t)i * neq + j] = S.nblk++;
+                S.touched.push_back(mask[j]);
+            }
+        OPF_CHECK(!S.refs[i].empty(), OPF_ERR_ARG, "equation %d reads no unknown", i);
+    }
+    OPF_HIP(hipMalloc(&S.coef, sizeof(double) * PAD_K * S.N * S.nblk));
+    OPF_HIP(hipMalloc(&S.bias, sizeof(double) * S.N * neq));
+    OPF_HIP(hipMalloc(&S.err, sizeof(int)));
+    return OPF_OK;
+}
+
+// every block's coefficients and every row's bias, on the device
+static int eqs_eval(EqsSystem& S) {
+    hipStream_t st = S.ctx->stream;
+    OPF_HIP(hipMemsetAsync(S.err, 0, sizeof(int), st));
+    const opf_field_desc& td = S.tg[0]->d;
+    const void* xs[3];
+    const void* dxs[3];
+    OPF_TRY(mesh_arrays(S.tg[0]->mesh, OPF_F64, xs, dxs));
+    for (int i = 0; i < S.neq; ++i) {
+        PadArgs a{};
+        for (size_t f = 0; f < S.fl[i].size(); ++f) {
+            const opf_field_desc& fd = S.fl[i][f]->d;
+            a.f[f] = {S.fl[i][f]->data, fd.offset[0], fd.offset[1], fd.offset[2], fd.extent[0], fd.extent[1]};
+        }
+        for (int q = 0; q < 3; ++q) {
+            a.x[q] = (const double*)xs[q];
+            a.dx[q] = (const double*)dxs[q];
+            a.morg[q] = S.tg[0]->mesh->start[q] - S.tg[0]->mesh->pad;
+            a.s[q] = S.s[q];
+            a.n[q] = S.n[q];
+            a.tl[q] = td.local_start[q];
+            a.te[q] = td.local_end[q];
+        }
+        for (int u = 0; u < S.neq; ++u)
+            for (int q = 0; q < 3; ++q) {
+                a.bc[u].per[q] = S.per[u][q];
+                a.bc[u].beta[q][0] = S.beta[u][q][0];
+                a.bc[u].beta[q][1] = S.beta[u][q][1];
+            }
+        a.code = S.prog[i];
+        a.ncode = (int)S.P[i].code.size();
+        a.bias = S.bias + (size_t)i * S.N;  // every pass writes the same bias
+        a.err = S.err;
+        for (int j : S.refs[i]) {
+            a.live = j;
+            a.coef = S.coef + (size_t)S.blk[(size_t)i * S.neq + j] * PAD_K * S.N;
+            hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((S.N + PAD_BLOCK - 1) / PAD_BLOCK)), dim3(PAD_BLOCK), 0, st,
+                               a);
+        }
+    }
+    OPF_HIP(hipGetLastError());
+    int herr = 0;
+    OPF_HIP(hipMemcpyAsync(&herr, S.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    OPF_HIP(hipStreamSynchronize(st));
+    OPF_CHECK(!(herr & 1), OPF_ERR_UNSUPPORTED,
+              "equation reads an unknown off the 5/7-point axis stencil (not a Laplacian-type operator)");
+    OPF_CHECK(!(herr & 2), OPF_ERR_UNSUPPORTED, "equation set is nonlinear in the unknowns");
+    return OPF_OK;
+}
+
+static bool same_level(const GLevelHost& a, const GLevelHost& b) {
+    if (a.var != b.var || a.N != b.N) return false;
+    for (int q = 0; q < 3; ++q)
+        if (a.n[q] != b.n[q]) return false;
+    if (!a.var) return a.a[0] == b.a[0] && a.a[1] == b.a[1] && a.a[2] == b.a[2] && a.sigma == b.sigma;
+    return a.af[0] == b.af[0] && a.af[1] == b.af[1] && a.af[2] == b.af[2] && a.sg == b.sg;
+}
+
+}  // namespace opf
+
+extern "C" int opf_eqns_csr(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, const int* pin,
+                            long long* nrows, long long* nnz, long long* ptr, long long* col, double* val,
+                            double* rhs) {
+    // CSRMatrixGenerator::generate_s per equation, appended (CSRMatrixGenerator.hpp:26-41, 143-193):
+    // block i's rows are target i's cells; a row lists the (unknown, cell) entries its
+    // pad holds in ascending global column; with pin[i] the block's LAST row is x = 0
+    OPF_CHECK(nrows && nnz, OPF_ERR_ARG, "opf_eqns_csr: null arg");
+    EqsSystem S;
+    OPF_TRY(eqs_setup(neq, eqs, targets, S));
+    OPF_TRY(eqs_eval(S));
+    const long long N = S.N;
+    std::vector<double> c((size_t)PAD_K * N * S.nblk), bias((size_t)N * neq);
+    OPF_HIP(hipMemcpyAsync(c.data(), S.coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, S.ctx->stream));
+    OPF_HIP(hipMemcpyAsync(bias.data(), S.bias, sizeof(double) * bias.size(), hipMemcpyDeviceToHost, S.ctx->stream));
+    OPF_HIP(hipStreamSynchronize(S.ctx->stream));
+    const long long st[3] = {1, S.n[0], (long long)S.n[0] * S.n[1]};
+    long long cnt = 0;
+    if (ptr) ptr[0] = 0;
+    std::vector<std::pair<long long, double>> ent;
+    for (int i = 0; i < neq; ++i)
+        for (long long r = 0; r < N; ++r) {
+            const long long row = (long long)i * N + r;
+            ent.clear();
+            const bool pinned = pin && pin[i] && r == N - 1;
+            if (pinned) {
+                ent.push_back({row, 1.0});
+            } else {
+                int p[3];
+                long long rem = r;
+                for (int d = 0; d < 3; ++d) {
+                    p[d] = (int)(rem % S.n[d]);
+                    rem /= S.n[d];
+                }
+                auto add = [&](long long q, double v) {
+                    for (auto& e : ent)
+                        if (e.first == q) {
+                            e.second += v;
+                            return;
+                        }
+                    ent.push_back({q, v});
+                };
+                for (int j : S.refs[i]) {
+                    const int b = S.blk[(size_t)i * neq + j];
+                    const double* cb = c.data() + (size_t)b * PAD_K * N;
+                    const unsigned tm = S.touched[b];
+                    const long long base = (long long)j * N;
+                    if (tm & 1u) add(base + r, cb[r]);
+                    for (int d = 0; d < S.dim; ++d)
+                        for (int sd = 0; sd < 2; ++sd) {
+                            const int q = 1 + 2 * d + sd;
+                            if (!(tm & (1u << q))) continue;
+                            const double a = cb[(size_t)q * N + r];
+                            const bool edge = sd == 0 ? p[d] == 0 : p[d] == S.n[d] - 1;
+                            if (edge && !S.per[j][d]) {
+                                add(base + r, S.alpha[j][d][sd] * a);  // the ghost, folded onto the mirror
+                                continue;
+                            }
+                            long long nb = r + (sd == 0 ? -st[d] : st[d]);
+                            if (edge) nb = r + (sd == 0 ? 1 : -1) * (long long)(S.n[d] - 1) * st[d];
+                            add(base + nb, a);
+                        }
+                }
+                std::sort(ent.begin(), ent.end());
+            }
+            for (const auto& e : ent) {
+                if (col) col[cnt] = e.first;
+                if (val) val[cnt] = e.second;
+                ++cnt;
+            }
+            if (ptr) ptr[row + 1] = cnt;
+            if (rhs) rhs[row] = pinned ? 0.0 : -bias[(size_t)i * N + r];
+        }
+    *nrows = N * neq;
+    *nnz = cnt;
+    return OPF_OK;
+}
+
+namespace opf {
+
+static double host_sum(const std::vector<double>& v, size_t from, size_t n) {
+    double s = 0;
+    for (size_t q = 0; q < n; ++q) s += v[from + q];
+    return s;
+}
+
+// (a, b) and (c, d) over M entries; deterministic (fixed grid, fixed order)
+static int dots(hipStream_t st, long long M, const double* a, const double* b, const double* c, const double* d,
+                double* part, std::vector<double>& hp, double* o1, double* o2) {
+    const int G = (int)std::min<long long>(512, (M + 255) / 256);
+    hipLaunchKernelGGL(k_dot2, dim3(G), dim3(256), 0, st, M, a, b, c, d, part);
+    OPF_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, st));
+    OPF_HIP(hipStreamSynchronize(st));
+    *o1 = host_sum(hp, 0, G);
+    if (o2) *o2 = host_sum(hp, G, G);
+    return OPF_OK;
+}
+
+// Coupled sets: BiCGStab (the solver of SolveEqns' AMGCL backend, AMGCLBackend.hpp:36-74)
+// on the whole system, right-preconditioned by one multigrid V-cycle per diagonal
+// block. Row block i is scaled by sign[i] (the orientation of its diagonal block,
+// which its hierarchy approximates); pinned rows are x = 0.
+struct EqsCoupled {
+    EqsSystem& S;
+    std::vector<Hierarchy>& H;
+    const double* sign;
+    const std::vector<int>& pin;
+    void apply(hipStream_t st, const double* x, double* y) const {
+        const long long N = S.N;
+        const unsigned nb = (unsigned)((N + 255) / 256);
+        for (int i = 0; i < S.neq; ++i) {
+            bool first = true;
+            for (int j : S.refs[i]) {
+                EqsBC bc;
+                for (int d = 0; d < 3; ++d) {
+                    bc.per[d] = S.per[j][d];
+                    bc.alpha[d][0] = S.alpha[j][d][0];
+                    bc.alpha[d][1] = S.alpha[j][d][1];
+                }
+                const double* cb = S.coef + (size_t)S.blk[(size_t)i * S.neq + j] * PAD_K * N;
+                hipLaunchKernelGGL(k_pad_apply, dim3(nb), dim3(256), 0, st, N, S.n[0], S.n[1], S.n[2], S.dim, cb, bc,
+                                   x + (size_t)j * N, y + (size_t)i * N, sign[i], first ? 0 : 1);
+                first = false;
+            }
+            if (pin[i]) hipLaunchKernelGGL(k_pin_row, dim3(1), dim3(1), 0, st, y + (size_t)i * N, x + (size_t)i * N, N - 1);
+        }
+    }
+    int precond(opf_ctx* ctx, const double* r, double* z) const {
+        for (int i = 0; i < S.neq; ++i) OPF_TRY(gmg_vcycle64(ctx, H[i], r + (size_t)i * S.N, z + (size_t)i * S.N));
+        return OPF_OK;
+    }
+};
+
+static int eqs_bicgstab(const EqsCoupled& A, double* x, const double* b, double tol, int maxit, opf_solve_state* out,
+                        int verbose) {
+    opf_ctx* ctx = A.S.ctx;
+    hipStream_t st = ctx->stream;
+    const long long M = A.S.N * A.S.neq;
+    const unsigned nb = (unsigned)((M + 255) / 256);
+    double* w = nullptr;
+    OPF_HIP(hipMalloc(&w, sizeof(double) * (8 * M + 1024)));
+    double *r = w, *rh = w + M, *p = w + 2 * M, *v = w + 3 * M, *sv = w + 4 * M, *t = w + 5 * M, *ph = w + 6 * M,
+           *sh = w + 7 * M, *part = w + 8 * M;
+    std::vector<double> hp(1024);
+    int rc = OPF_OK;
+    double bb = 0, rr = 0, rho = 1, alpha = 1, omega = 1;
+    int it = 0;
+    auto fail = [&](int code) {
+        rc = code;
+        return code;
+    };
+    do {
+        // r = b - A x; r_hat = r; p = v = 0
+        A.apply(st, x, t);
+        hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, b, -1.0, (const double*)t);
+        if (fail(hipMemcpyAsync(rh, r, sizeof(double) * M, hipMemcpyDeviceToDevice, st) ? OPF_ERR_HIP : OPF_OK)) break;
+        if (fail(hipMemsetAsync(p, 0, sizeof(double) * 2 * M, st) ? OPF_ERR_HIP : OPF_OK)) break;  // p, v
+        if (fail(dots(st, M, b, b, r, r, part, hp, &bb, &rr))) break;
+        const double tgt = tol * tol * bb;
+        while (rr > tgt && it < maxit) {
+            double rho1 = 0;
+            if (fail(dots(st, M, rh, r, nullptr, nullptr, part, hp, &rho1, nullptr))) break;
+            if (rho1 == 0.0) break;  // breakdown
+            const double beta = (rho1 / rho) * (alpha / omega);
+            rho = rho1;
+            hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(256), 0, st, M, p, (const double*)r, (const double*)v, beta,
+                               omega);
+            if (fail(A.precond(ctx, p, ph))) break;
+            A.apply(st, ph, v);
+            double rv = 0;
+            if (fail(dots(st, M, rh, v, nullptr, nullptr, part, hp, &rv, nullptr))) break;
+            if (rv == 0.0) break;
+            alpha = rho / rv;
+            hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, sv, (const double*)r, -alpha, (const double*)v);
+            if (fail(A.precond(ctx, sv, sh))) break;
+            A.apply(st, sh, t);
+            double ts = 0, tt = 0;
+            if (fail(dots(st, M, t, sv, t, t, part, hp, &ts, &tt))) break;
+            omega = tt > 0 ? ts / tt : 0.0;
+            hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(256), 0, st, M, x, alpha, (const double*)ph, omega,
+                               (const double*)sh);
+            hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, (const double*)sv, -omega, (const double*)t);
+            if (fail(dots(st, M, r, r, nullptr, nullptr, part, hp, &rr, nullptr))) break;
+            ++it;
+            if (verbose) fprintf(stderr, "[eqns] bicgstab it %d relres %.3e\n", it, bb > 0 ? std::sqrt(rr / bb) : 0.0);
+            if (omega == 0.0) break;
+        }
+        if (rc) break;
+        // the true residual of the final x
+        A.apply(st, x, t);
+        hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, b, -1.0, (const double*)t);
+        if (fail(dots(st, M, r, r, nullptr, nullptr, part, hp, &rr, nullptr))) break;
+    } while (false);
+    hipStreamSynchronize(st);
+    hipFree(w);
+    if (rc) return rc;
+    out->niter = it;
+    out->nrefine = 0;
+    out->abserr = std::sqrt(rr);
+    out->relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    out->converged = out->relerr <= tol * (1 + 1e-12) || bb == 0;
+    return OPF_OK;
+}
+
+}  // namespace opf
+
+extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const* targets,
+                              const opf_solve_params* params, opf_solve_state* state) {
+    // SolveEqns (UnifiedSolve.hpp:69-100): assemble, solve from x = 0 with params[0]'s
+    // tolerance (relative residual of the whole system), write every target back and
+    // updatePadding
+    OPF_CHECK(state && params, OPF_ERR_ARG, "opf_eqns_solve: null arg");
+    EqsSystem S;
+    OPF_TRY(eqs_setup(neq, eqs, targets, S));
+    OPF_TRY(eqs_eval(S));
+    opf_ctx* ctx = S.ctx;
+    hipStream_t st = ctx->stream;
+    const long long N = S.N;
+    const unsigned nb = (unsigned)((N + 255) / 256);
+    const double tol = params[0].tol > 0 ? params[0].tol : 1e-8;  // amgcl solver defaults
+    const int maxit = params[0].max_iter > 0 ? params[0].max_iter : 100;
+    bool coupled = false;
+    for (int i = 0; i < neq; ++i) coupled |= !(S.refs[i].size() == 1 && S.refs[i][0] == i);
+    std::vector<int> pin(neq);
+    for (int i = 0; i < neq; ++i) pin[i] = params[i].pin_value != 0;
+    // diagonal blocks -> multigrid levels
+    std::vector<GLevelHost> L(neq);
+    std::vector<double> sign(neq, 1.0);
+    {
+        std::vector<double> c((size_t)PAD_K * N);
+        for (int i = 0; i < neq; ++i) {
+            const int b = S.blk[(size_t)i * neq + i];
+            OPF_CHECK(b >= 0, OPF_ERR_UNSUPPORTED, "equation %d does not read its own target (no diagonal block)", i);
+            OPF_HIP(hipMemcpyAsync(c.data(), S.coef + (size_t)b * PAD_K * N, sizeof(double) * c.size(),
+                                   hipMemcpyDeviceToHost, st));
+            OPF_HIP(hipStreamSynchronize(st));
+            OPF_TRY(rows_to_level(c, N, S.n, S.dim, S.per[i], &sign[i], L[i]));
+        }
+    }
+    auto hier = [&](Hierarchy& H, int i, int slots, int precision) {
+        if (params[0].pre_smooth > 0) H.pre = params[0].pre_smooth;
+        if (params[0].post_smooth > 0) H.post = params[0].post_smooth;
+        return gmg_build(ctx, H, L[i], S.per[i], S.alpha[i], precision, slots);
+    };
+    double* vec = nullptr;  // x, b over the whole system (block i at i * N)
+    OPF_HIP(hipMalloc(&vec, sizeof(double) * 2 * N * neq + 64));
+    double* x = vec;
+    double* b = vec + N * neq;
+    double* shift = vec + 2 * N * neq;  // [neq]
+    int rc = OPF_OK;
+    std::vector<Hierarchy> H;
+    do {
+        if ((rc = hipMemsetAsync(x, 0, sizeof(double) * N * neq, st) ? OPF_ERR_HIP : OPF_OK)) break;
+        for (int i = 0; i < neq; ++i) {
+            hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(256), 0, st, N, (const double*)(S.bias + (size_t)i * N), sign[i],
+                               b + (size_t)i * N);
+            if (pin[i] && (rc = hipMemsetAsync(b + (size_t)i * N + (N - 1), 0, sizeof(double), st) ? OPF_ERR_HIP : 0))
+                break;  // the pinned row reads x = 0
+        }
+        if (rc) break;
+        opf_solve_state total{};
+        if (!coupled) {
+            // block-diagonal: one multigrid solve per block, blocks with the same operator
+            // batched as the slots of one hierarchy; a pinned block solves the consistent
+            // singular system (its last row's rhs = minus the sum of the others, which the
+            // pinned system implies) and is shifted so that its last unknown is 0
+            std::vector<int> grp(neq, -1);
+            for (int i = 0; i < neq && !rc; ++i) {
+                if (grp[i] >= 0) continue;
+                std::vector<int> mem{i};
+                for (int j = i + 1; j < neq; ++j) {
+                    bool same = grp[j] < 0 && same_level(L[i], L[j]);
+                    for (int d = 0; d < 3 && same; ++d)
+                        same &= S.per[i][d] == S.per[j][d] && S.alpha[i][d][0] == S.alpha[j][d][0] &&
+                                S.alpha[i][d][1] == S.alpha[j][d][1];
+                    if (same) mem.push_back(j);
+                }
+                for (int m : mem) grp[m] = i;
+                H.emplace_back();
+                Hierarchy& G = H.back();
+                const int B = (int)mem.size();
+                if ((rc = hier(G, i, B, params[0].precision == OPF_PREC_MIXED ? OPF_PREC_MIXED : OPF_PREC_FP64)))
+                    break;
+                double* xg = nullptr;
+                if ((rc = hipMalloc(&xg, sizeof(double) * 2 * N * B) ? OPF_ERR_NOMEM : OPF_OK)) break;
+                double* bg = xg + N * B;
+                for (int m = 0; m < B; ++m) {
+                    const int e = mem[m];
+                    hipMemcpyAsync(bg + (size_t)m * N, b + (size_t)e * N, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                   st);
+                    if (pin[e]) {
+                        if (!G.singular) {
+                            set_error("pinValue on a non-singular operator is not supported by the multigrid solver");
+                            rc = OPF_ERR_UNSUPPORTED;
+                            break;
+                        }
+                        const int P = (int)std::min<long long>(256, (N + 255) / 256);
+                        hipLaunchKernelGGL(k_sum_rows, dim3(P), dim3(256), 0, st, N, (const double*)(bg + (size_t)m * N),
+                                           G.part);
+                        hipLaunchKernelGGL(k_pin_rhs, dim3(1), dim3(256), 0, st, N, bg + (size_t)m * N,
+                                           (const double*)G.part, P, N - 1);
+                    }
+                }
+                if (!rc) rc = hipMemsetAsync(xg, 0, sizeof(double) * N * B, st) ? OPF_ERR_HIP : OPF_OK;
+                std::vector<opf_solve_state> ss(B);
+                if (!rc) rc = gmg_solve_slots(ctx, G, bg, xg, tol, maxit, ss.data(), params[0].verbose);
+                if (!rc) {
+                    for (int m = 0; m < B; ++m) {
+                        const int e = mem[m];
+                        double* xe = x + (size_t)e * N;
+                        hipMemcpyAsync(xe, xg + (size_t)m * N, sizeof(double) * N, hipMemcpyDeviceToDevice, st);
+                        if (pin[e]) {
+                            hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, st, (const double*)xe, N - 1, shift + e);
+                            hipLaunchKernelGGL(k_sub_scalar, dim3(nb), dim3(256), 0, st, N, xe,
+                                               (const double*)(shift + e));
+                        }
+                        total.niter = std::max(total.niter, ss[m].niter);
+                        total.nrefine = std::max(total.nrefine, ss[m].nrefine);
+                    }
+                }
+                hipStreamSynchronize(st);
+                hipFree(xg);
+            }
+            if (rc) break;
+        } else {
+            for (int i = 0; i < neq && !rc; ++i) {
+                H.emplace_back();
+                rc = hier(H.back(), i, 1, OPF_PREC_FP64);
+            }
+            if (rc) break;
+            EqsCoupled A{S, H, sign.data(), pin};
+            rc = eqs_bicgstab(A, x, b, tol, maxit, &total, params[0].verbose);
+            if (rc) break;
+        }
+        // the state reports the whole system's true residual (pinned rows included)
+        {
+            EqsCoupled A{S, H, sign.data(), pin};
+            const long long M = N * neq;
+            double* rv = nullptr;
+            if ((rc = hipMalloc(&rv, sizeof(double) * (2 * M + 1024)) ? OPF_ERR_NOMEM : OPF_OK)) break;
+            A.apply(st, x, rv + M);
+            hipLaunchKernelGGL(k_axpy, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, M, rv, (const double*)b,
+                               -1.0, (const double*)(rv + M));
+            std::vector<double> hp(1024);
+            double rr = 0, bb = 0;
+            rc = dots(st, M, rv, rv, b, b, rv + 2 * M, hp, &rr, &bb);
+            hipFree(rv);
+            if (rc) break;
+            total.abserr = std::sqrt(rr);
+            total.relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            total.converged = total.relerr <= tol * (1 + 1e-12) || bb == 0;
+        }
+        // returnValues: write back, updatePadding
+        for (int i = 0; i < neq; ++i) {
+            const opf_field_desc& td = S.tg[i]->d;
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, N, S.n[0], S.n[1], S.s[0], S.s[1], S.s[2],
+                               (double*)S.tg[i]->data, td.offset[0], td.offset[1], td.offset[2], td.extent[0],
+                               td.extent[1], (const double*)(x + (size_t)i * N), (const double*)nullptr);
+        }
+        if ((rc = hipGetLastError() ? OPF_ERR_HIP : OPF_OK)) break;
+        for (int i = 0; i < neq && !rc; ++i) {
+            S.tg[i]->spare_valid = false;
+            rc = launch_ghost_fill(S.tg[i], S.tg[i]->data);
+        }
+        if (rc) break;
+        *state = total;
+    } while (false);
+    hipStreamSynchronize(st);
+    for (Hierarchy& G : H) gmg_free(G);
+    hipFree(vec);
+    return rc;
 }

@@ -1826,17 +1826,23 @@ int read_scal(const Hierarchy& H, hipStream_t st) {
     return OPF_OK;
 }
 
-// A single solve (slot 0) in Krylov precision T: flexible CG with reliable updates
-// (FP32: FP64 update whenever the recurrence residual fell by the slot's threshold;
-// FP64: at the convergence test only), stopped on the device. Two iterations are
-// replayed as a graph per host poll; a stopped slot skips the work inside them.
+// Solves of every slot of H (vectors slot-major) in Krylov precision T: flexible CG
+// with reliable updates (FP32: FP64 update whenever the recurrence residual fell by
+// the slot's threshold; FP64: at the convergence test only), stopped on the device
+// per slot. Two iterations are replayed as a graph per host poll; a stopped slot
+// skips the work inside them.
 template <typename T>
 static int rel_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, int verbose) {
     hipStream_t s = ctx->stream;
+    auto all_done = [&]() {
+        for (int q = 0; q < H.B; ++q)
+            if (H.hscal[(size_t)q * GMG_SSTRIDE + S_DONE] == 0.0) return false;
+        return true;
+    };
     hipLaunchKernelGGL(k_seti, dim3(1), dim3(64), 0, s, H.mk + MK_NEW * H.B, H.B, 1);
     ph_rel_start<T>(H, s, x, b, tol * tol, rel_delta2<T>(), false);
     OPF_TRY(read_scal(H, s));
-    if (H.hscal[S_DONE] != 0.0) return OPF_OK;
+    if (all_done()) return OPF_OK;
     if (!H.iter_exec || H.iter_x != x || H.iter_b != b || H.iter_maxit != maxit) {
         if (H.iter_exec) OPF_HIP(hipGraphExecDestroy(H.iter_exec));
         if (H.iter_graph) OPF_HIP(hipGraphDestroy(H.iter_graph));
@@ -1857,7 +1863,34 @@ static int rel_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, dou
         if (verbose)
             fprintf(stderr, "[gmg] it %d updates %d relres %.3e\n", (int)H.hscal[S_ITS], (int)H.hscal[S_NREP],
                     std::sqrt(H.hscal[S_RR] / H.hscal[S_BB]));
-        if (H.hscal[S_DONE] != 0.0) break;
+        if (all_done()) break;
+    }
+    return OPF_OK;
+}
+
+int gmg_solve_slots(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit,
+                    opf_solve_state* st, int verbose) {
+    hipStream_t s = ctx->stream;
+    if (H.precision == OPF_PREC_MIXED) OPF_TRY(rel_solve<float>(ctx, H, b, x, tol, maxit, verbose));
+    else OPF_TRY(rel_solve<double>(ctx, H, b, x, tol, maxit, verbose));
+    std::vector<int> its(H.B), reps(H.B);
+    for (int q = 0; q < H.B; ++q) {
+        its[q] = (int)H.hscal[(size_t)q * GMG_SSTRIDE + S_ITS];
+        reps[q] = (int)H.hscal[(size_t)q * GMG_SSTRIDE + S_NREP];
+    }
+    // final true residual in FP64
+    ph_resid64<double>(H, s, nullptr, x, b, H.r64, FIN_RR);
+    OPF_TRY(read_scal(H, s));
+    OPF_HIP(hipGetLastError());
+    for (int q = 0; q < H.B; ++q) {
+        const double rr = H.hscal[(size_t)q * GMG_SSTRIDE + S_RR], bb = H.hscal[(size_t)q * GMG_SSTRIDE + S_BB];
+        opf_solve_state& o = st[q];
+        o.niter = its[q];
+        o.nrefine = reps[q];
+        o.abserr = std::sqrt(rr);
+        o.relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        o.converged = o.relerr <= tol * (1 + 1e-12) || bb == 0;
+        if (verbose) fprintf(stderr, "[gmg] slot %d iters %d refine %d relres %.3e\n", q, o.niter, o.nrefine, o.relerr);
     }
     return OPF_OK;
 }
@@ -1865,23 +1898,13 @@ static int rel_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, dou
 int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,
               int verbose) {
     OPF_CHECK(H.B == 1, OPF_ERR_ARG, "gmg_solve drives single-slot hierarchies");
-    hipStream_t s = ctx->stream;
-    if (H.precision == OPF_PREC_MIXED) OPF_TRY(rel_solve<float>(ctx, H, b, x, tol, maxit, verbose));
-    else OPF_TRY(rel_solve<double>(ctx, H, b, x, tol, maxit, verbose));
-    const int total = (int)H.hscal[S_ITS], nref = (int)H.hscal[S_NREP];
-    double rr = 0, bb = 0;
-    // final true residual in FP64
-    ph_resid64<double>(H, s, nullptr, x, b, H.r64, FIN_RR);
-    OPF_TRY(read_scal(H, s));
-    bb = H.hscal[S_BB];
-    rr = H.hscal[S_RR];
+    return gmg_solve_slots(ctx, H, b, x, tol, maxit, st, verbose);
+}
+
+int gmg_vcycle64(opf_ctx* ctx, Hierarchy& H, const double* r, double* z) {
+    OPF_CHECK(H.precision == OPF_PREC_FP64, OPF_ERR_ARG, "gmg_vcycle64 needs an FP64 hierarchy");
+    ph_vcycle<double>(H, ctx->stream, nullptr, 0, r, z);
     OPF_HIP(hipGetLastError());
-    st->niter = total;
-    st->nrefine = nref;
-    st->abserr = std::sqrt(rr);
-    st->relerr = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-    st->converged = st->relerr <= tol * (1 + 1e-12) || bb == 0;
-    if (verbose) fprintf(stderr, "[gmg] iters %d refine %d relres %.3e\n", total, nref, st->relerr);
     return OPF_OK;
 }
 

@@ -148,6 +148,12 @@ void gmg_free(Hierarchy& H);
 // solve A x = b on the stream: b, x FP64 device arrays (N each). Returns iterations etc.
 int gmg_solve(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit, opf_solve_state* st,
               int verbose);
+// every slot of H at once (b, x slot-major, B * N each); st[B]
+int gmg_solve_slots(opf_ctx* ctx, Hierarchy& H, const double* b, double* x, double tol, int maxit,
+                    opf_solve_state* st, int verbose);
+// z = one V-cycle on r from a zero guess, every slot (FP64 hierarchies): the
+// multigrid as a preconditioner of an outer Krylov method
+int gmg_vcycle64(opf_ctx* ctx, Hierarchy& H, const double* r, double* z);
 int gmg_apply64(opf_ctx* ctx, Hierarchy& H, const double* x, double* y);
 
 }  // namespace opf

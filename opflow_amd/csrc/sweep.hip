@@ -24,7 +24,8 @@ struct Lowerer {
     opf_field* const* fields;
     int nfields;
     const opf_mesh* mesh;
-    int unknown_slot = -1;  // field slot standing for OPF_N_UNKNOWN (pad mode)
+    const int* unknown_slots = nullptr;  // field slot of each OPF_N_UNKNOWN index (its layout)
+    int nunknown = 0;
     bool unknown_as_field = false;  // read OPF_N_UNKNOWN as an ordinary field (bias evaluation)
     std::vector<Insn> code;
     int depth = 0, maxdepth = 0;
@@ -77,8 +78,9 @@ struct Lowerer {
                     for (int d = 0; d < 3; ++d) locs[n][d] = fields[e.a]->d.loc[d];
                     break;
                 case OPF_N_UNKNOWN:
-                    if (unknown_slot < 0) return OPF_ERR_ARG;
-                    for (int d = 0; d < 3; ++d) locs[n][d] = fields[unknown_slot]->d.loc[d];
+                    if (e.a < 0 || e.a >= nunknown || unknown_slots[e.a] < 0 || unknown_slots[e.a] >= nfields)
+                        return OPF_ERR_ARG;
+                    for (int d = 0; d < 3; ++d) locs[n][d] = fields[unknown_slots[e.a]]->d.loc[d];
                     break;
                 case OPF_N_CONST: locs[n] = {-1, -1, -1}; break;
                 case OPF_N_ADD:
@@ -114,11 +116,12 @@ struct Lowerer {
         switch (e.kind) {
             case OPF_N_FIELD:
             case OPF_N_UNKNOWN: {
-                int f = e.kind == OPF_N_FIELD ? e.a : unknown_slot;
+                int f = e.kind == OPF_N_FIELD ? e.a : unknown_slots[e.a];
                 const opf_field_desc& fd = fields[f]->d;
                 long long lin = sh[0] + (long long)fd.extent[0] * (sh[1] + (long long)fd.extent[1] * sh[2]);
                 const bool ldu = e.kind == OPF_N_UNKNOWN && !unknown_as_field;
-                code.push_back({ldu ? I_LDU : I_LD, f, (int)lin, 0, 0.0});
+                // pad mode: a = the unknown's index; otherwise the field slot read
+                code.push_back({ldu ? I_LDU : I_LD, ldu ? e.a : f, (int)lin, 0, 0.0});
                 for (int d = 0; d < 3; ++d) {
                     omin[f][d] = std::min(omin[f][d], sh[d]);
                     omax[f][d] = std::max(omax[f][d], sh[d]);
@@ -215,6 +218,13 @@ struct Lowerer {
 
 int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
                const opf_mesh* mesh, int unknown_slot, LoweredProgram* out, bool unknown_as_field) {
+    return lower_tree_set(nodes, nnodes, root, fields, nfields, mesh, &unknown_slot, unknown_slot >= 0 ? 1 : 0, out,
+                          unknown_as_field);
+}
+
+int lower_tree_set(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
+                   const opf_mesh* mesh, const int* unknown_slots, int nunknown, LoweredProgram* out,
+                   bool unknown_as_field) {
     OPF_CHECK(nodes && nnodes > 0 && root >= 0 && root < nnodes, OPF_ERR_ARG, "lowering: bad tree");
     OPF_CHECK(nfields >= 0 && nfields <= SWEEP_MAX_FIELDS, OPF_ERR_UNSUPPORTED, "at most %d fields per sweep",
               SWEEP_MAX_FIELDS);
@@ -224,7 +234,8 @@ int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* cons
     L.fields = fields;
     L.nfields = nfields;
     L.mesh = mesh;
-    L.unknown_slot = unknown_slot;
+    L.unknown_slots = unknown_slots;
+    L.nunknown = nunknown;
     L.unknown_as_field = unknown_as_field;
     L.init();
     int rc = L.compute_locs();

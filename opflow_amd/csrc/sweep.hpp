@@ -12,7 +12,7 @@ constexpr int SWEEP_MAX_DEPTH = 16;
 
 enum InsnOp {
     I_LD = 0,     // push field[a] at linear offset b from the evaluation cell
-    I_LDU = 1,    // push the unknown (pad mode) at packed offset c
+    I_LDU = 1,    // push unknown a (pad mode) at packed offset c
     I_CONST = 2,  // push imm
     I_DX = 3,     // push mesh dx of dim a at (cell index + b)
     I_X = 4,      // push mesh x of dim a at (cell index + b)
@@ -47,6 +47,11 @@ static_assert(sizeof(((SweepArgs*)nullptr)->f) / sizeof(FieldArg) == SWEEP_MAX_F
 // emits I_LDU for it, an ordinary sweep (true) reads that slot like any field
 int lower_tree(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
                const opf_mesh* mesh, int unknown_slot, LoweredProgram* out, bool unknown_as_field = false);
+// several unknowns (equation sets): OPF_N_UNKNOWN's `a` indexes unknown_slots, the field
+// slot whose layout that unknown has; pad mode keeps the index in the I_LDU instruction
+int lower_tree_set(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
+                   const opf_mesh* mesh, const int* unknown_slots, int nunknown, LoweredProgram* out,
+                   bool unknown_as_field = false);
 // the sweep of program P over the box [a.s, a.s + a.n) on the context's stream, by the
 // engine the context's jit mode selects (interpreter or the compiled kernel of P)
 int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P);

@@ -124,6 +124,131 @@ def Solve(getter, target, params=None, precParams=None):
         h.close()
 
 
+@dataclass
+class IJSolverParams(StructSolverParams):
+    """IJSolverParams (Solvers/IJ/IJSolver.hpp:24-71) as SolveEqns reads them: params[0]'s
+    tol / maxIter for the whole system (amgcl defaults 1e-8 / 100 when unset), pinValue
+    per equation."""
+
+
+class ColoredMDRangeMapper:
+    """DS::ColoredMDRangeMapper (DataStructures/Index/LinearMapper/ColoredMDRangeMapper.hpp:20-41):
+    block i ranks range i's cells x-fastest after the cells of blocks 0..i-1."""
+
+    def __init__(self, *ranges):
+        self.ranges = list(ranges)
+        self.offset = [0]
+        for r in self.ranges:
+            n = 1
+            for lo, hi in r:
+                n *= hi - lo
+            self.offset.append(self.offset[-1] + n)
+
+    def __call__(self, idx, color):
+        r = self.ranges[color]
+        rank, stride = 0, 1
+        for (lo, hi), i in zip(r, idx):
+            rank += (i - lo) * stride
+            stride *= hi - lo
+        return self.offset[color] + rank
+
+
+class _EqnHolder:
+    """EqnHolder / StencilHolder (Equation/EquationHolder.hpp:31-99): each getter is called
+    with every target's StencilField placeholder; the lowered trees go to the device."""
+
+    def __init__(self, getters, targets):
+        from . import Expr, StencilUnknown, _node_array
+        if len(getters) != len(targets) or not getters:
+            raise ValueError("an equation set needs one getter per target")
+        self.targets = list(targets)
+        unk = [StencilUnknown(i, t) for i, t in enumerate(self.targets)]
+        n = len(getters)
+        self.descs = (L.EqnDesc * n)()
+        self._keep = []
+        for i, g in enumerate(getters):
+            eqn = g(*unk)
+            if not hasattr(eqn, "lhs"):
+                raise TypeError("equation getter must return `lhs == rhs`")
+            both = Expr(3, eqn.lhs, eqn.rhs)
+            nodes, fields, root = both.flatten()
+            lhs_root, rhs_root = nodes[root][1], nodes[root][2]
+            nodes = nodes[:-1]
+            arr = _node_array(nodes)
+            fh = (C.c_void_p * max(1, len(fields)))(*[f.h.value for f in fields])
+            self._keep += [arr, fh, fields]
+            d = self.descs[i]
+            d.nodes = C.cast(arr, C.POINTER(L.ExprNode))
+            d.nnodes = len(nodes)
+            d.lhs_root, d.rhs_root = lhs_root, rhs_root
+            d.fields = C.cast(fh, C.POINTER(C.c_void_p))
+            d.nfields = len(fields)
+        self.th = (C.c_void_p * n)(*[t.h.value for t in self.targets])
+        self.lib = self.targets[0].lib
+
+    @property
+    def size(self):
+        return len(self.targets)
+
+
+def makeEqnHolder(getters, targets):
+    return _EqnHolder(list(getters), list(targets))
+
+
+def makeStencilHolder(eqn_holder):
+    return eqn_holder
+
+
+@dataclass
+class CSRMatrix:  # DS::CSRMatrix (DataStructures/Matrix/CSRMatrix.hpp)
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    rhs: np.ndarray
+
+
+class CSRMatrixGenerator:
+    """CSRMatrixGenerator::generate (Equation/CSRMatrixGenerator.hpp:26-41, 143-193) of a set:
+    pads evaluated on the device, rows in ColoredMDRangeMapper order, pin_flags[i] makes
+    block i's last row x = 0."""
+
+    @staticmethod
+    def generate(st_holder, mapper=None, pin_flags=None):
+        h = st_holder
+        n = h.size
+        if mapper is not None and len(mapper.ranges) != n:
+            raise ValueError("the mapper needs one range per target")
+        pins = (C.c_int * n)(*[int(bool(p)) for p in (pin_flags or [False] * n)])
+        nr, nz = C.c_longlong(), C.c_longlong()
+        L.check(h.lib.opf_eqns_csr(n, h.descs, h.th, pins, C.byref(nr), C.byref(nz), None, None, None, None))
+        ptr = np.zeros(nr.value + 1, dtype=np.int64)
+        col = np.zeros(nz.value, dtype=np.int64)
+        val = np.zeros(nz.value)
+        rhs = np.zeros(nr.value)
+        L.check(h.lib.opf_eqns_csr(n, h.descs, h.th, pins, C.byref(nr), C.byref(nz), ptr.ctypes.data,
+                                   col.ctypes.data, val.ctypes.data, rhs.ctypes.data))
+        return CSRMatrix(ptr, col, val, rhs)
+
+
+def SolveEqns(*args):
+    """SolveEqns<Solver>(getters..., targets..., mapper, params) (Equation/UnifiedSolve.hpp:69-100):
+    solves the set for every target from x = 0 and writes them back (padding updated)."""
+    if len(args) < 4 or len(args) % 2:
+        raise TypeError("SolveEqns(getters..., targets..., mapper, params)")
+    n = (len(args) - 2) // 2
+    getters, targets, mapper, params = args[:n], args[n:2 * n], args[2 * n], args[2 * n + 1]
+    if mapper is not None and len(mapper.ranges) != n:
+        raise ValueError("the mapper needs one range per target")
+    params = list(params) if params is not None else [IJSolverParams() for _ in range(n)]
+    if len(params) != n:
+        raise ValueError("SolveEqns needs one parameter set per equation")
+    h = _EqnHolder(list(getters), list(targets))
+    pa = (L.SolveParams * n)(*[_params(p, None) for p in params])
+    st = L.SolveState()
+    L.check(h.lib.opf_eqns_solve(n, h.descs, h.th, pa, C.byref(st)))
+    return EqnSolveState(st.niter, st.relerr, st.abserr, st.nrefine, bool(st.converged))
+
+
 def gen_bumps_grid(desc, seed, nbumps, noise=0.0):
     """RHS of seeded bumps (+ noise) on the unit-cube cell centres of a Poisson descriptor."""
     n = 1

@@ -180,3 +180,66 @@ def assign(nodes, root, fields, mesh, out_arr, out_desc, op=0, update_padding=No
     if update_padding is not None:
         res = update_padding(res)
     return res
+
+
+def linear_system(eqs, unknowns, mesh, fill, pins=None):
+    """Dense matrix and right-hand side of a set of equations that are linear in their
+    unknowns, assembled the way CSRMatrixGenerator::generate does
+    (Equation/CSRMatrixGenerator.hpp:26-41, 143-193) -- TEST ONLY, small grids.
+
+    The reference builds row r of block i from the StencilPad of lhs_i - rhs_i at cell
+    r (StencilField.hpp:123-350, StencilPad.hpp:136-331): an affine function of the
+    unknowns whose ghost entries follow the boundary rule (ghost = alpha * mirror +
+    beta). Here the same affine map is taken by linearity instead of pad algebra:
+    with F(u) = evaluate(lhs - rhs) over fields whose interiors are u and whose
+    ghosts are filled by updatePadding, bias = F(0) and column c = F(e_c) - F(0).
+    Blocks follow ColoredMDRangeMapper (ColoredMDRangeMapper.hpp:20-41): block i is
+    target i's cells x-fastest; pins[i] makes block i's last row x = 0 (rhs 0).
+
+    eqs: list of (nodes, lhs_root, rhs_root, fields) with UNKNOWN nodes whose `a` is
+    the target index; fields as for evaluate(). unknowns: list of (padded array,
+    desc) per target (one layout). fill(i, arr) returns arr with target i's ghosts
+    filled. Returns (A, rhs) with rhs = -bias."""
+    neq = len(unknowns)
+    arr0, desc = unknowns[0]
+    dim = mesh.dim
+    lo = [max(desc.assign_start[q], desc.local_start[q]) for q in range(dim)]
+    hi = [min(desc.assign_end[q], desc.local_end[q]) for q in range(dim)]
+    shape = tuple(hi[q] - lo[q] for q in reversed(range(dim)))
+    N = int(np.prod(shape))
+    sl = tuple(slice(lo[q] - desc.offset[q], hi[q] - desc.offset[q]) for q in reversed(range(dim)))
+
+    def field_with(i, interior):
+        a = np.zeros_like(np.asarray(arr0, dtype=np.float64))
+        a[sl] = interior.reshape(shape)
+        return fill(i, a)
+
+    zero = [field_with(j, np.zeros(N)) for j in range(neq)]
+    A = np.zeros((neq * N, neq * N))
+    rhs = np.zeros(neq * N)
+    for i, (nodes, lr, rr, fields) in enumerate(eqs):
+        nf = len(fields)
+        sub = [(FIELD, nf + a, 0, 0, 0.0) if k == UNKNOWN else (k, a, b, d, v) for (k, a, b, d, v) in nodes]
+        sub.append((SUB, lr, rr, 0, 0.0))
+        root = len(sub) - 1
+
+        def F(us):
+            return evaluate(sub, root, list(fields) + [(u, unknowns[j][1]) for j, u in enumerate(us)], mesh, lo,
+                            hi).reshape(-1)
+
+        f0 = F(zero)
+        rhs[i * N:(i + 1) * N] = -f0
+        reads = {a for (k, a, b, d, v) in nodes if k == UNKNOWN}
+        for j in sorted(reads):
+            for c in range(N):
+                e = np.zeros(N)
+                e[c] = 1.0
+                us = list(zero)
+                us[j] = field_with(j, e)
+                A[i * N:(i + 1) * N, j * N + c] = F(us) - f0
+        if pins and pins[i]:
+            r = i * N + N - 1
+            A[r, :] = 0.0
+            A[r, r] = 1.0
+            rhs[r] = 0.0
+    return A, rhs
