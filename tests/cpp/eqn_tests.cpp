@@ -1,5 +1,6 @@
 // C++ restatement of the reference's equation tests (test/Core/Equation/
-// DircEqnTest.cpp, NeumEqnTest.cpp, PeriodicEqnTest.cpp) compiled against the
+// DircEqnTest.cpp, NeumEqnTest.cpp, PeriodicEqnTest.cpp, EqnSetTest.cpp,
+// CSRMatrixGeneratorTest.cpp) compiled against the
 // engine's OpFlow front-end. Prints one PASS/FAIL line per case; exit code 0
 // when all pass.
 #include <OpFlow>
@@ -149,6 +150,70 @@ int main() {
         ok = ok && mat.row[6] - r5 == 5 && mat.col[r5 + 2] == 5 && mat.val[r5 + 2] == 4.0;
         for (double v : mat.rhs) ok = ok && v == -1.0;
         std::printf("%s CSRMatrixGeneratorTest.SimplePoisson: nnz %d\n", ok ? "PASS" : "FAIL", mat.nnz());
+        failures += !ok;
+    }
+    for (BCType bc : {BCType::Dirc, BCType::Neum}) {
+        // EqnSetTest.cpp SimplePoisson_2Eqn / SimplePoisson_Neum_2Eqn and
+        // CSRMatrixGeneratorTest.cpp:163-250 (the second block repeats the first, 16 columns on)
+        using Mesh = CartesianMesh<Meta::int_<2>>;
+        using Field = CartesianField<Real, Mesh>;
+        const bool neum = bc == BCType::Neum;
+        auto m = MeshBuilder<Mesh>().newMesh(5, 5).setMeshOfDim(0, 0., 4.).setMeshOfDim(1, 0., 4.).build();
+        Field p = ExprBuilder<Field>()
+                          .setMesh(m)
+                          .setName("p")
+                          .setBC(0, DimPos::start, bc, 0.)
+                          .setBC(0, DimPos::end, bc, 0.)
+                          .setBC(1, DimPos::start, bc, 0.)
+                          .setBC(1, DimPos::end, bc, 0.)
+                          .setExt(1)
+                          .setLoc({LocOnMesh::Center, LocOnMesh::Center})
+                          .build();
+        p = 0;
+        auto p2 = p;
+        auto p_true = p;
+        auto lap0 = [&](auto&& e, auto&&) {
+            return 1.0 == d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e);
+        };
+        auto lap1 = [&](auto&&, auto&& e) {
+            return 1.0 == d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e);
+        };
+        DS::ColoredMDRangeMapper<2> mapper{p.assignableRange, p.assignableRange};
+        {
+            auto eqn = makeEqnHolder(std::forward_as_tuple(lap0, lap1), std::forward_as_tuple(p, p));
+            auto st = makeStencilHolder(eqn);
+            auto mat = CSRMatrixGenerator::generate(st, mapper, std::vector<bool>{neum, neum});
+            bool ok = mat.row.size() == 33 && mat.rhs.size() == 32;
+            const size_t hr = mat.row.size() / 2, hc = mat.col.size() / 2;
+            for (size_t i = hr, j = 0; ok && i < mat.row.size(); ++i, ++j) ok = mat.row[i] - mat.row[hr] == mat.row[j];
+            for (size_t i = hc, j = 0; ok && i < mat.col.size(); ++i, ++j) ok = mat.col[i] - 16 == mat.col[j];
+            for (size_t i = hc, j = 0; ok && i < mat.val.size(); ++i, ++j) ok = mat.val[i] == mat.val[j];
+            for (size_t i = 16, j = 0; ok && i < 32; ++i, ++j) ok = mat.rhs[i] == mat.rhs[j];
+            std::printf("%s CSRMatrixGeneratorTest.SimplePoisson%s_2Eqn: nnz %d\n", ok ? "PASS" : "FAIL",
+                        neum ? "_Neum" : "", mat.nnz());
+            failures += !ok;
+        }
+        std::vector<IJSolverParams<DeviceIJSolver>> params(2);
+        params[0].pinValue = params[1].pinValue = neum;
+        params[0].p.solver.tol = 1e-13;
+        auto st = SolveEqns<DeviceIJSolver>(lap0, lap1, p, p2, mapper, params);
+        double err = 0, diff = 0;
+        if (!neum) {
+            StructSolverParams<StructSolverType::GMRES> sp;
+            sp.tol = 1e-13;
+            Solve([&](auto&& e) { return 1.0 == d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e); },
+                  p_true, sp);
+        }
+        for (int j = 1; j < 5; ++j)
+            for (int i = 1; i < 5; ++i) {
+                DS::MDIndex<2> k{i, j};
+                diff = std::max(diff, std::abs(p[k] - p2[k]));
+                if (!neum) err = std::max(err, std::abs(p[k] - p_true[k]) / std::abs(p_true[k]));
+            }
+        if (neum) err = std::abs(p[DS::MDIndex<2>{4, 4}]);  // the pinned last unknown
+        const bool ok = diff == 0.0 && err <= 1e-10 && st.relerr <= 1e-13;
+        std::printf("%s EqnSetTest.SimplePoisson%s_2Eqn: |p-p2| %.1e, err %.2e, iters %d, relres %.2e\n",
+                    ok ? "PASS" : "FAIL", neum ? "_Neum" : "", diff, err, st.niter, st.relerr);
         failures += !ok;
     }
     return failures ? 1 : 0;

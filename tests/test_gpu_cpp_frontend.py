@@ -41,4 +41,4 @@ def test_ftcs2d_example_bitexact(tmp_path):
 def test_cpp_equation_tests():
     r = _run(["eqn_tests"])
     assert r.returncode == 0, r.stdout
-    assert r.stdout.count("PASS") == 6
+    assert r.stdout.count("PASS") == 10 and "FAIL" not in r.stdout
