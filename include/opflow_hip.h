@@ -257,6 +257,20 @@ typedef struct {
     int converged;
 } opf_solve_state;
 
+/* ---- field streams (OpFlow's raw binary format) -------------------------- */
+/* Replaces Utils::RawBinaryOStream::operator<< (Utils/Writers/RawBinaryStream.hpp:84-155):
+ * writes `<dir>/<name>_<count>.cart` (name length + name, dim, ranks = 1, time,
+ * mesh range, node coordinates, accessible range, local range, then the local
+ * range's values x-fastest in the field's element type), streamed from device
+ * memory through pinned double buffers. dir null or "" = ".".               */
+int opf_field_write_cart(opf_field* f, const char* dir, const char* name, int count, double time);
+/* Replaces Utils::RawBinaryIStream::operator>> (RawBinaryStream.hpp:157-224):
+ * reads `<dir>/<name>_<count>.cart` into f's local range after checking dim,
+ * rank count, mesh range and coordinates, accessible and local ranges (a name
+ * mismatch only warns, as the reference's OP_EXPECT_MSG does), then
+ * updatePadding. time (optional) receives the stored time stamp.           */
+int opf_field_read_cart(opf_field* f, const char* dir, const char* name, int count, double* time);
+
 /* Replaces makeEqnSolveHandler(getter, target, solver) -> HYPREEqnSolveHandler
  * ::init/initStencil (HYPREEqnSolveHandler.hpp:44-117) and the handler's
  * StencilPad evaluation (StencilField.hpp:123-350): the equation

@@ -23,7 +23,8 @@ __all__ = [
     "d2x", "d2y", "d2z", "dx", "dy", "dz", "d1", "d2", "d1IntpCenterToCorner", "d1IntpCornerToCenter",
     "rangeReduce", "StructSolverParams", "EqnSolveState", "Solve", "makeEqnSolveHandler", "PrecondStructSolver",
     "PoissonGMG", "Server", "SolveEqns", "makeEqnHolder", "makeStencilHolder", "CSRMatrixGenerator",
-    "ColoredMDRangeMapper", "IJSolverParams", "StencilUnknown",
+    "ColoredMDRangeMapper", "IJSolverParams", "StencilUnknown", "TimeStamp", "RawBinaryOStream",
+    "RawBinaryIStream", "Utils",
 ]
 
 
@@ -527,6 +528,61 @@ def gen_uniform(seed, n, lo=0.0, hi=1.0):
     out = np.zeros(n, dtype=np.float64)
     L.check(lib.opf_gen_uniform(C.c_uint64(seed), n, lo, hi, out.ctypes.data_as(L._DP)))
     return out
+
+
+# ---------------------------------------------------------------- field streams
+class TimeStamp(float):
+    """Utils::TimeStamp: the time tag written with the next field."""
+
+
+class RawBinaryOStream:
+    """Utils::RawBinaryOStream (Utils/Writers/RawBinaryStream.hpp:42-60, 98-155):
+    `os << TimeStamp(t) << u` writes <dir>/<name>_<count>.cart, dir being the given
+    path's parent (std::filesystem::path::parent_path), values streamed from the GPU."""
+
+    def __init__(self, path="./"):
+        import os
+        self.dir = os.path.dirname(str(path)) or "."
+        self.time = 0.0
+        self.count = 0
+
+    def setCounterTo(self, c):
+        self.count = int(c)
+
+    def __lshift__(self, x):
+        if isinstance(x, TimeStamp):
+            self.time = float(x)
+            return self
+        L.check(x.lib.opf_field_write_cart(x.h, self.dir.encode(), x.name.encode(), self.count, self.time))
+        self.count += 1
+        return self
+
+
+class RawBinaryIStream:
+    """Utils::RawBinaryIStream (RawBinaryStream.hpp:62-77, 157-224): `is >> u` reads
+    <dir>/<name>_<count>.cart into u (layout checked), then updatePadding."""
+
+    def __init__(self, path="./"):
+        import os
+        self.dir = os.path.dirname(str(path)) or "."
+        self.count = 0
+        self.time = 0.0
+
+    def setCounterTo(self, c):
+        self.count = int(c)
+
+    def __rshift__(self, f):
+        t = C.c_double()
+        L.check(f.lib.opf_field_read_cart(f.h, self.dir.encode(), f.name.encode(), self.count, C.byref(t)))
+        self.time = t.value
+        self.count += 1
+        return self
+
+
+class Utils:  # the reference's OpFlow::Utils namespace
+    TimeStamp = TimeStamp
+    RawBinaryOStream = RawBinaryOStream
+    RawBinaryIStream = RawBinaryIStream
 
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402

@@ -1,7 +1,8 @@
 // examples/FTCS2D/FTCS-MPI.cpp-style program compiled against the engine's
 // OpFlow front-end: cell-centred 256^2 field, Dirichlet 0, dt = 0.2 h^2, 1000
 // steps of `u = u + dt * alpha * (d2x(u) + d2y(u))`. Writes the final storage
-// block to argv[1] for the parity test.
+// block to argv[1] for the parity test and, given argv[4], the final field as
+// `<dir of argv[4]>/u_0.cart`.
 #include <OpFlow>
 
 #include <chrono>
@@ -48,6 +49,10 @@ int main(int argc, char** argv) {
     u2.upload(init);
     u2.updatePadding();
     for (int i = 1; i <= 5; ++i) u2 += dt * alpha * (d2x<D2SecondOrderCentered>(u2) + d2y<D2SecondOrderCentered>(u2));
+    if (argc > 4) {  // the final field as the reference's examples record it (Utils::RawBinaryOStream)
+        Utils::RawBinaryOStream os(argv[4]);
+        os << Utils::TimeStamp(steps * dt) << u;
+    }
     if (argc > 1) {
         FILE* f = std::fopen(argv[1], "wb");
         std::fwrite(h.data(), sizeof(double), h.size(), f);

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from oracle import cart
 from helpers import desc
 
 pytestmark = pytest.mark.gpu
@@ -25,11 +26,14 @@ def _run(args, timeout=240):
 def test_ftcs2d_example_bitexact(tmp_path):
     n, steps = 256, 1000
     out = str(tmp_path / "u.bin")
-    r = _run(["ftcs2d_example", out, str(n), str(steps)])
+    r = _run(["ftcs2d_example", out, str(n), str(steps), str(tmp_path) + "/"])
     assert r.returncode == 0
     raw = np.fromfile(out, dtype=np.float64)
     m = (n + 2) * (n + 2)
     u, u2, init = raw[:m].reshape(n + 2, n + 2), raw[m:2 * m].reshape(n + 2, n + 2), raw[2 * m:].reshape(n + 2, n + 2)
+    c = cart.read(str(tmp_path / "u_0.cart"))  # RawBinaryOStream output of the same field
+    assert c["name"] == "u" and c["time"] == steps * (0.2 / (n * n)) and c["local_range"] == [(0, n), (0, n)]
+    np.testing.assert_array_equal(c["values"], u[1:-1, 1:-1].reshape(-1))
     d = desc(2, [n + 1, n + 1], 1, [(1, 1), (1, 1)], ext=(1, 1))
     om = oracle.Mesh([n + 1, n + 1])
     init = oracle.update_padding(d, om, init.copy())
