@@ -288,6 +288,23 @@ def test_sweep_3d_f64_and_range_error(ctx, engine):
         u.assign(of.dx(of.D1WENO53Upwind, u))
 
 
+@pytest.mark.parametrize("op", [0, 1])
+def test_sweep_3d_zmarch_bitexact(ctx, engine, op):
+    """A 3-D sweep on a ragged 20 x 12 x 40 box with WENO in z (ext 3, periodic),
+    two fields, a z spacing different from x and y, in place and compound: equal to
+    the oracle bit for bit on both engines."""
+    u_spec = (1, [(DIRC, NEUM), (NEUM, DIRC), (PER, PER)], [(0.2, 0.1), (0.3, -0.1), (0, 0)], (3, 3))
+    w_spec = (1, [(DIRC, DIRC)] * 3, [(0.5, -0.5), (0.0, 1.0), (0.25, 0.75)], (1, 1))
+    D2, W, DN = of.D2SecondOrderCentered, of.D1WENO53Upwind, of.D1FirstOrderBiasedDownwind
+
+    def fn(u, w):
+        return (u - 0.001 * of.dz(W, u) + 0.01 * (of.d2x(D2, u) + of.d2z(D2, u))
+                - 0.02 * w * of.dz(DN, w) + of.dy(DN, u) * w)
+    got, ref = _sweep_case(ctx, fn, [u_spec, w_spec], [21, 13, 41], None if op == 0 else w_spec, SEED + 12, op=op,
+                           ranges=[(0.0, 1.0), (0.0, 1.0), (0.0, 2.5)])
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_sweep_engines_agree_at_size(ctx, dtype):
     """A 3-D sweep of 96^3 cells (above the size from which the default engine compiles
