@@ -6,6 +6,8 @@ cfg3: backward-Euler/Poisson solves at 2048^2 and 256^3 (Dirichlet, bumps +
 cfg4: lid-driven cavity 1024^2, 500 steps, mixed vs FP64 pressure solves.
 cfg5: served Poisson solves of mixed sizes at tolerance 1e-8 -- solves/s closed
 loop and p50/p99 latency open loop, next to the plain FP64 solver one at a time.
+eqn_sets: SolveEqns on ten equal-operator equations (batched) vs ten Solve()
+calls, and a coupled two-equation set.
 """
 import time
 
@@ -202,9 +204,75 @@ def cfg5(of, ctx, ev, count=48):
         "throughput_ratio": t_64 / t_srv}}
 
 
+def eqn_sets(of, ctx, n=256, neq=10, reps=3):
+    """SolveEqns (UnifiedSolve.hpp:69-100) on ten copies of a Dirichlet Poisson equation
+    on n^2 cells with different boundary values: the set (one batched hierarchy, ten
+    slots) next to ten single-equation Solve() calls; each call includes the device
+    pad evaluation, the hierarchy build, the solve and the write-back, as the
+    reference's SolveEqns / Solve do per call. Plus a coupled two-equation set."""
+    D2 = of.D2SecondOrderCentered
+    mesh = of.MeshBuilder(ctx).newMesh(n + 1, n + 1).build()
+
+    def field(name, g):
+        b = of.ExprBuilder().setMesh(mesh).setName(name).setLoc([of.LocOnMesh.Center] * 2).setExt(1)
+        for d in range(2):
+            b.setBC(d, of.DimPos.start, of.BCType.Dirc, g).setBC(d, of.DimPos.end, of.BCType.Dirc, g)
+        return b.build()
+    ps = [field(f"p{i}", 0.1 * i) for i in range(neq)]
+
+    def lap(e):
+        return of.d2x(D2, e) + of.d2y(D2, e)
+
+    def getter(i):
+        return lambda *e: of.Equation(1.0, lap(e[i]))
+    eqs = [getter(i) for i in range(neq)]
+    params = [of.IJSolverParams(tol=1e-10)] + [of.IJSolverParams() for _ in range(neq - 1)]
+    ts, st = [], None
+    for r in range(reps + 1):
+        for q in ps:
+            q.assign(0.0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = of.SolveEqns(*eqs, *ps, None, params)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    t_set = min(ts[1:])
+    ts1 = []
+    for r in range(reps + 1):
+        for q in ps:
+            q.assign(0.0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for q in ps:
+            of.Solve(lambda e: 1.0 == lap(e), q, of.StructSolverParams(tol=1e-10))
+        ctx.sync()
+        ts1.append(time.perf_counter() - t0)
+    t_one = min(ts1[1:])
+    # coupled: diffusion-reaction of two species exchanging mass
+    u, v = field("u", 0.0), field("v", 1.0)
+    ceqs = [lambda a, b: of.Equation(1.0, lap(a) - 50.0 * a + 20.0 * b),
+            lambda a, b: of.Equation(0.0, lap(b) - 40.0 * b + 30.0 * a)]
+    tc = []
+    for r in range(reps + 1):
+        u.assign(0.0)
+        v.assign(0.0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        cst = of.SolveEqns(*ceqs, u, v, None, [of.IJSolverParams(tol=1e-10), of.IJSolverParams()])
+        ctx.sync()
+        tc.append(time.perf_counter() - t0)
+    return {f"solve_eqns_{neq}x_poisson_{n}sq_relres1e-10": {
+        "set_s_per_call": t_set, "set_iters": st.niter, "set_relres": st.relerr,
+        "single_solves_s_total": t_one, "speedup": t_one / t_set,
+        "note": "per call: device pad evaluation, hierarchy build, solve, write-back (both sides)"},
+        f"solve_eqns_coupled_2x{n}sq_relres1e-10": {"s_per_call": min(tc[1:]), "bicgstab_iters": cst.niter,
+                                                     "relres": cst.relerr}}
+
+
 def solver_extra(of, ctx, ev, cpu=False):
     out = {}
     out.update(cfg3(of, ctx, ev, cpu))
     out.update(cfg4(of, ctx, ev))
     out.update(cfg5(of, ctx, ev))
+    out.update(eqn_sets(of, ctx))
     return out

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 
 #include "gmg.hpp"
 #include "sweep.hpp"
@@ -904,6 +905,24 @@ static int eqs_eval(EqsSystem& S) {
     return OPF_OK;
 }
 
+// equations i and j read only their own unknown with the same program (up to which
+// unknown), the same fields and the same boundary folding: their diagonal blocks
+// hold the same coefficients (only the biases can differ)
+static bool same_diagonal(const EqsSystem& S, int i, int j) {
+    if (S.refs[i].size() != 1 || S.refs[i][0] != i || S.refs[j].size() != 1 || S.refs[j][0] != j) return false;
+    if (S.fl[i] != S.fl[j] || S.P[i].code.size() != S.P[j].code.size()) return false;
+    for (size_t q = 0; q < S.P[i].code.size(); ++q) {
+        const Insn &x = S.P[i].code[q], &y = S.P[j].code[q];
+        const bool same = x.op == y.op && (x.op == I_LDU || x.a == y.a) && x.b == y.b && x.c == y.c &&
+                          std::memcmp(&x.imm, &y.imm, sizeof(double)) == 0;
+        if (!same) return false;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (S.per[i][d] != S.per[j][d] || S.alpha[i][d][0] != S.alpha[j][d][0] || S.alpha[i][d][1] != S.alpha[j][d][1])
+            return false;
+    return true;
+}
+
 static bool same_level(const GLevelHost& a, const GLevelHost& b) {
     if (a.var != b.var || a.N != b.N) return false;
     for (int q = 0; q < 3; ++q)
@@ -1146,6 +1165,14 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
         for (int i = 0; i < neq; ++i) {
             const int b = S.blk[(size_t)i * neq + i];
             OPF_CHECK(b >= 0, OPF_ERR_UNSUPPORTED, "equation %d does not read its own target (no diagonal block)", i);
+            int twin = -1;  // an earlier equation whose diagonal block has the same coefficients
+            for (int j = 0; j < i && twin < 0; ++j)
+                if (same_diagonal(S, i, j)) twin = j;
+            if (twin >= 0) {
+                L[i] = L[twin];
+                sign[i] = sign[twin];
+                continue;
+            }
             OPF_HIP(hipMemcpyAsync(c.data(), S.coef + (size_t)b * PAD_K * N, sizeof(double) * c.size(),
                                    hipMemcpyDeviceToHost, st));
             OPF_HIP(hipStreamSynchronize(st));
