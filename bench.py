@@ -233,6 +233,23 @@ def main():
                                     "alg_GBps_per_launch": 8.0 * cells / (ms32 / 1e3 / l32) / 1e9}
         extra["ftcs3d_512_fp32"]["hbm_frac"] = extra["ftcs3d_512_fp32"]["alg_GBps_per_launch"] / HBM_PEAK_GBS
         u32.close()
+        # the same sweep one step per launch (no temporal blocking): the streaming kernel's
+        # share of HBM peak, 16 B per cell-update (FP64) / 8 B (FP32)
+        for dt_, bpc in (("f64", 16.0), ("f32", 8.0)):
+            _, us = ftcs_field(of, ctx, n, 3, dt_, of.BCType.Periodic, 0xC0FFEE + 2)
+            us.ftcs_blocked(coef, SETTLE_STEPS, 1)
+            ctx.sync()
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, ctx.stream)
+            us.ftcs_blocked(coef, 40, 1)
+            ev.record(e1, ctx.stream)
+            ctx.sync()
+            ms1 = ev.elapsed_ms(e0, e1) / 40
+            gbs = bpc * cells / (ms1 / 1e3) / 1e9
+            extra[f"ftcs3d_512_{dt_}_one_step_per_launch"] = {
+                "Mcell_updates_per_s": cells / (ms1 / 1e3) / 1e6, "ms_per_step": ms1, "alg_GBps": gbs,
+                "hbm_frac": gbs / HBM_PEAK_GBS}
+            us.close()
         # accuracy of the configuration: FP32 against FP64 after exactly `steps`
         # steps from the same input (the FP64 GPU sweep equals the oracle bit for
         # bit), relative to max|u_FP64|; untimed, fresh fields
@@ -293,7 +310,7 @@ def main():
                      "step_equivalent_frac": alg_bytes * args.steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                      "traffic_GBps": traffic / per_launch_s / 1e9 if traffic else None,
                      "traffic_frac": traffic / per_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None,
-                     "kernel": "ftcs3d_tb_kernel" if steps_per_launch > 1 else "ftcs3d_kernel",
+                     "kernel": "ftcs3d_tb_kernel" if steps_per_launch > 1 else "ftcs3d_lds_kernel",
                      "steps_per_launch": steps_per_launch, "launches": launches},
         "cpu_baseline": cpu,
         "wall_s_timed_region": wall,

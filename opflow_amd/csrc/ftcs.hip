@@ -189,28 +189,96 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
     }
 }
 
-// 3-D: block 64 x 4 owns an x-y tile and walks ZC planes in z, keeping the
-// column's z-1 / z / z+1 values in registers; x/y neighbours come through L1/L2.
-template <typename T, bool POW2, int ZC>
-__global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k0 = blockIdx.z * ZC;
-    if (i >= a.n[0] || j >= a.n[1]) return;
-    const long long sy = a.e0, sz = (long long)a.e0 * a.e1;
-    long long l = (i + 1) + sy * (j + 1) + sz * (k0 + 1);
-    T zm = in[l - sz], zc = in[l];
+// 3-D single step, LDS-tiled: a 64 x NTY workgroup owns a 64 x (NTY * RY) x-y
+// tile (each thread RY rows) and marches ZC planes in z. The current plane of
+// the tile and its one-cell x/y halo sit in LDS (double-buffered, one barrier
+// per plane); each thread keeps its cells' z-1 / z / z+1 values in registers
+// and loads plane k+2 while plane k is computed. The input's ghost layer
+// supplies the halo of boundary tiles, so loads never wrap; the kernel writes
+// interior cells only and the host loop refills the output's ghosts before
+// they are read. Per cell: d2x + d2y, then + d2z, then u + coef * s -- the
+// arithmetic every FTCS kernel here uses, so all give the same bits.
+// Launch shape swept at 512^3 on MI355X (64 x 8 threads, 2 rows each, 8 planes).
+constexpr int FT3_TX = 64, FT3_NTY = 8, FT3_RY = 2, FT3_TY = FT3_NTY * FT3_RY, FT3_ZC = 8;
+
+template <typename T, bool POW2, int ZC, int NTY, int RY>
+__global__ __launch_bounds__(64 * NTY) void ftcs3d_lds_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                              FtcsArgs<T> a) {
+    constexpr int TY = NTY * RY;
+    static_assert(128 + 2 * TY <= 64 * NTY, "one halo element per thread at most");
+    __shared__ T lds[2][TY + 2][FT3_TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * 64 + tx;
+    const int x0 = blockIdx.x * FT3_TX, y0 = blockIdx.y * TY, k0 = blockIdx.z * ZC;
     const int kend = min(k0 + ZC, a.n[2]);
-    for (int k = k0; k < kend; ++k, l += sz) {
-        T zp = in[l + sz];
-        T s = d2<T, POW2>(in[l - 1], zc, in[l + 1], a.hh[0]) + d2<T, POW2>(in[l - sy], zc, in[l + sy], a.hh[1]);
-        s = s + d2<T, POW2>(zm, zc, zp, a.hh[2]);
-        T v = zc + a.coef * s;
-        out[l] = v;
-        int p[3] = {i, j, k};
-        write_images<T, 3>(out, a, p, v);
-        zm = zc;
-        zc = zp;
+    const long long sy = a.e0, sz = (long long)a.e0 * a.e1;
+    const int i = x0 + tx;
+    // storage index of (x, y, k) (ghost layer at -1 and n)
+    auto at = [&](int x, int y, int k) { return (long long)(x + 1) + sy * (y + 1) + sz * (k + 1); };
+    // valid storage positions: interior and the ghost layer
+    auto ok = [&](int x, int y) { return x >= -1 && x <= a.n[0] && y >= -1 && y <= a.n[1]; };
+    // this thread's halo element: rows y0 - 1 and y0 + TY (threads 0..127), then the
+    // columns x0 - 1 and x0 + TX over the tile's rows; corners are not read
+    int hx = 0, hy = 0, hlx = 0, hly = 0;
+    if (t < 128) {
+        hx = x0 + (t & 63);
+        hy = t < 64 ? y0 - 1 : y0 + TY;
+        hlx = (t & 63) + 1;
+        hly = t < 64 ? 0 : TY + 1;
+    } else if (t < 128 + 2 * TY) {
+        const int q = t - 128, side = q / TY, r = q % TY;
+        hx = side == 0 ? x0 - 1 : x0 + FT3_TX;
+        hy = y0 + r;
+        hlx = side == 0 ? 0 : FT3_TX + 1;
+        hly = r + 1;
+    }
+    const bool hl = t < 128 + 2 * TY && ok(hx, hy);
+    bool own[RY];
+    T zm[RY], zc[RY], zp[RY], zq[RY];
+    T hp = T(0), hq = T(0);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int j = y0 + ty * RY + r;
+        own[r] = ok(i, j);
+        zm[r] = own[r] ? in[at(i, j, k0 - 1)] : T(0);
+        zc[r] = own[r] ? in[at(i, j, k0)] : T(0);
+        zp[r] = own[r] ? in[at(i, j, k0 + 1)] : T(0);  // k0 + 1 <= n[2]: the ghost plane at most
+        lds[0][ty * RY + r + 1][tx + 1] = zc[r];
+    }
+    if (hl) {
+        lds[0][hly][hlx] = in[at(hx, hy, k0)];
+        hp = in[at(hx, hy, k0 + 1)];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = k0; k < kend; ++k) {
+        // plane k + 2 is loaded one step ahead of its use (only while a next step needs it)
+        if (k + 1 < kend) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) zq[r] = own[r] ? in[at(i, y0 + ty * RY + r, k + 2)] : T(0);
+            if (hl) hq = in[at(hx, hy, k + 2)];
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int ly = ty * RY + r + 1, j = y0 + ly - 1;
+            if (i >= a.n[0] || j >= a.n[1]) continue;
+            const T c = zc[r];
+            T s = d2<T, POW2>(lds[cur][ly][tx], c, lds[cur][ly][tx + 2], a.hh[0]) +
+                  d2<T, POW2>(lds[cur][ly - 1][tx + 1], c, lds[cur][ly + 1][tx + 1], a.hh[1]);
+            s = s + d2<T, POW2>(zm[r], c, zp[r], a.hh[2]);
+            out[at(i, j, k)] = c + a.coef * s;
+        }
+        const int nxt = cur ^ 1;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            lds[nxt][ty * RY + r + 1][tx + 1] = zp[r];
+            zm[r] = zc[r];
+            zc[r] = zp[r];
+            zp[r] = zq[r];
+        }
+        if (hl) lds[nxt][hly][hlx] = hp;
+        hp = hq;
+        __syncthreads();
+        cur = nxt;
     }
 }
 
@@ -230,12 +298,12 @@ __global__ __launch_bounds__(256) void ftcs3d_kernel(const T* __restrict__ in, T
 // cells, so the input's ghosts are never read; the output's ghosts are left
 // stale and refilled by the host loop before anything reads them (a
 // branch-free load path keeps the load waits counted). Per cell and step the
-// arithmetic is that of ftcs3d_kernel, so the result has the same bits as K
+// arithmetic is that of ftcs3d_lds_kernel, so the result has the same bits as K
 // single steps.
 template <typename T, bool POW2, int TX, int TY, int NT, int K, bool XCD>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
                                                        int zc) {
-    static_assert(K >= 2, "one step per launch is ftcs3d_kernel");
+    static_assert(K >= 2, "one step per launch is ftcs3d_lds_kernel");
     constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RP = RX * RY;
     constexpr int NL = (RP + NT - 1) / NT;
     constexpr int PF = 3;
@@ -321,9 +389,8 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     }
 }
 
-// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch;
-// 3-D single steps march FT_ZC planes per block
-constexpr int FT_ZC = 16, FT_TS2 = 32, FT_TB2 = 8;
+// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch
+constexpr int FT_TS2 = 32, FT_TB2 = 8;
 
 // 3-D blocked launch shape per precision (swept at 512^3 on MI355X): TX x TY
 // columns, NT threads, z chunks of ZC planes, K steps per launch (a remainder of
@@ -412,7 +479,7 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
         const T* in = (const T*)f->data;
         T* out = (T*)f->spare;
         const int k = launch_steps(d.dim, d.dtype, a.n, a.per, (long long)a.e0 * a.e1, nsteps - it, tblock);
-        if (ghosts_stale && !(d.dim == 3 && k > 1)) {
+        if (ghosts_stale && !(d.dim == 3 && k > 1 && a.per[0] && a.per[1] && a.per[2])) {
             OPF_TRY(launch_ghost_fill(f, f->data));
             ghosts_stale = false;
         }
@@ -445,9 +512,14 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
             }
             ghosts_stale = true;
         } else if (d.dim == 3) {
-            dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, (a.n[2] + FT_ZC - 1) / FT_ZC);
-            if (pow2) hipLaunchKernelGGL((ftcs3d_kernel<T, true, FT_ZC>), grid, dim3(64, 4), 0, st, in, out, a);
-            else hipLaunchKernelGGL((ftcs3d_kernel<T, false, FT_ZC>), grid, dim3(64, 4), 0, st, in, out, a);
+            dim3 grid((a.n[0] + FT3_TX - 1) / FT3_TX, (a.n[1] + FT3_TY - 1) / FT3_TY, (a.n[2] + FT3_ZC - 1) / FT3_ZC);
+            if (pow2)
+                hipLaunchKernelGGL((ftcs3d_lds_kernel<T, true, FT3_ZC, FT3_NTY, FT3_RY>), grid, dim3(64, FT3_NTY), 0, st,
+                                   in, out, a);
+            else
+                hipLaunchKernelGGL((ftcs3d_lds_kernel<T, false, FT3_ZC, FT3_NTY, FT3_RY>), grid, dim3(64, FT3_NTY), 0,
+                                   st, in, out, a);
+            ghosts_stale = true;
         } else {
             set_error("ftcs: dim %d not supported by the fast path", d.dim);
             return OPF_ERR_UNSUPPORTED;
