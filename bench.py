@@ -104,8 +104,9 @@ def time_ftcs(of, ctx, ev, u, coef, steps, warmup):
     return ms, wall
 
 
-def cpu_baseline_ftcs3d(n, steps):
-    """Oracle (C/OpenMP restatement of OpFlow's FieldAssigner FTCS sequence) on host cores."""
+def cpu_baseline_ftcs3d(n, steps, nthreads=0):
+    """Oracle (C/OpenMP restatement of OpFlow's FieldAssigner FTCS sequence) on host cores
+    (nthreads = 0: all of them)."""
     import numpy as np
     import oracle
     from tests.helpers import desc as mkdesc
@@ -116,11 +117,11 @@ def cpu_baseline_ftcs3d(n, steps):
     u = of.gen_uniform(0xC0FFEE + 2, cnt, -1.0, 1.0).reshape(d.extent[2], d.extent[1], d.extent[0])
     u = oracle.update_padding(d, om, u)
     h = 1.0 / n
-    oracle.ftcs(d, om, u, 0.2 * h * h, 1)  # warm-up
+    oracle.ftcs(d, om, u, 0.2 * h * h, 1, nthreads)  # warm-up
     t0 = time.perf_counter()
-    oracle.ftcs(d, om, u, 0.2 * h * h, steps)
+    oracle.ftcs(d, om, u, 0.2 * h * h, steps, nthreads)
     dt = time.perf_counter() - t0
-    return n ** 3 * steps / dt / 1e6, dt, oracle.num_threads()
+    return n ** 3 * steps / dt / 1e6, dt, nthreads or oracle.num_threads()
 
 
 def generic_sweeps(of, ctx, ev, n):
@@ -288,6 +289,10 @@ def main():
         cpu = {"value": v_cpu, "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
                "sample": f"oracle FTCS (OpFlow FieldAssigner sequence, C/OpenMP) on the same 512^3 FP64 periodic "
                          f"input, {args.cpu_steps} steps in {t_cpu:.1f} s on {cores} threads of {cpu_model()}"}
+        if not args.no_extra:  # the same sample on one thread, 4 steps
+            v1, t1, _ = cpu_baseline_ftcs3d(n, 4, 1)
+            extra["cpu_baseline_1_thread"] = {"value": v1, "unit": "Mcell-updates/s", "cores": 1,
+                                              "sample": f"4 steps in {t1:.1f} s"}
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "ftcs3d_512_fp64_traffic.json")

@@ -242,12 +242,15 @@ void orc_ftcs(const opf_field_desc* f, const orc_mesh* m, double* u, double coef
         e[d] = imin(f->assign_end[d], f->local_end[d]);
     }
 #ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();  /* this call only */
+#else
+    const int nt = 1;
+    (void)nthreads;
 #endif
     for (int it = 0; it < nsteps; ++it) {
         memcpy(temp, u, n * sizeof(double));
         int dim = f->dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (int k = s[2]; k < e[2]; ++k)
             for (int j = s[1]; j < e[1]; ++j)
                 for (int i = s[0]; i < e[0]; ++i) {
@@ -257,7 +260,7 @@ void orc_ftcs(const opf_field_desc* f, const orc_mesh* m, double* u, double coef
                     temp[lin(f, i, j, k)] = u[lin(f, i, j, k)] + coef * acc;
                 }
         orc_update_padding(f, m, temp);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (int k = s[2]; k < e[2]; ++k)
             for (int j = s[1]; j < e[1]; ++j)
                 for (int i = s[0]; i < e[0]; ++i) u[lin(f, i, j, k)] = temp[lin(f, i, j, k)];
