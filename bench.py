@@ -269,6 +269,22 @@ def main():
             "max_err_rel": float(np.max(np.abs(d32))) / scale,
             "l2_err_rel": float(np.linalg.norm(d32) / np.linalg.norm(r64))})
         del d32, r64
+        # stable control (dt = h^2/8, below the 7-point limit h^2/6), labelled as such:
+        # the same comparison where the solution decays instead of blowing up
+        coef_s = h * h / 8.0
+        outs = []
+        for dt_ in ("f64", "f32"):
+            _, us = ftcs_field(of, ctx, n, 3, dt_, of.BCType.Periodic, 0xC0FFEE + 2)
+            us.ftcs(coef_s, args.steps)
+            outs.append(us.numpy()[1:-1, 1:-1, 1:-1].astype(np.float64))
+            us.close()
+        d32 = outs[1] - outs[0]
+        scale = float(np.max(np.abs(outs[0])))
+        extra["ftcs3d_512_stable_control_dt_h2_over_8"] = {
+            "steps_compared": args.steps, "max_abs_u_fp64": scale,
+            "fp32_max_err_rel": float(np.max(np.abs(d32))) / scale,
+            "fp32_l2_err_rel": float(np.linalg.norm(d32) / np.linalg.norm(outs[0]))}
+        del d32, outs
         n2 = 256
         h2 = 1.0 / n2
         mesh2, u2 = ftcs_field(of, ctx, n2, 2, "f64", of.BCType.Dirc, 0xC0FFEE + 1, init="bumps")
