@@ -282,7 +282,10 @@ int opf_eqn_create(const opf_expr_node* nodes, int nnodes, int lhs_root, int rhs
                    opf_eqn** out);
 /* Replaces HYPREEqnSolveHandler::solve (HYPREEqnSolveHandler.hpp:190-209):
  * regenerate b (and A unless static_mat), solve from the target's current
- * values, write the solution back, updatePadding.                          */
+ * values, write the solution back, updatePadding. Symmetric operators run the
+ * multigrid-preconditioned flexible CG (FP64 or mixed); non-symmetric ones
+ * (advection terms) run BiCGStab in FP64 preconditioned by one V-cycle of the
+ * operator's symmetric part, with pin_value's row kept literally.          */
 int opf_eqn_solve(opf_eqn* e, opf_solve_state* state);
 /* The equation's linear system in the CSR form CSRMatrixGenerator::generate /
  * generate_rhs produce (CSRMatrixGenerator.hpp:55-220, the AMGCL path of
@@ -316,7 +319,8 @@ typedef struct {
  * written back and its padding updated. params[i].pin_value pins block i's
  * last unknown at 0. Blocks that read only their own unknown are solved by
  * multigrid, blocks with equal operators batched as slots of one hierarchy;
- * coupled sets by BiCGStab preconditioned with one V-cycle per diagonal block.
+ * coupled sets, and sets with a non-symmetric diagonal block, by BiCGStab
+ * preconditioned with one V-cycle per diagonal block (its symmetric part).
  * state: Krylov iterations (the most of any block when block-diagonal) and the
  * whole system's residual.                                                  */
 int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, const opf_solve_params* params,

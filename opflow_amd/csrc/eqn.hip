@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 
 #include "gmg.hpp"
 #include "sweep.hpp"
@@ -263,6 +264,7 @@ struct opf_eqn {
     double* shift = nullptr;
     int* err = nullptr;
     double sign = 1.0;
+    bool symmetric = true;  // false: BiCGStab with the symmetric part's multigrid as preconditioner
     bool built = false;
     bool first = true;
     Hierarchy H;
@@ -361,11 +363,16 @@ static int eqn_eval_bias(opf_eqn* e) {
 }
 
 // The rows of one diagonal block (PAD_K coefficient arrays of N rows, host copy) as
-// a multigrid level: face coefficients from the neighbour slots (checked
-// symmetric), the rest of the diagonal as sigma. *sign orients the diagonal
-// positive; a uniform operator becomes the matrix-free constant mode.
+// a multigrid level: face coefficients from the neighbour slots, the rest of the
+// diagonal as sigma. *sign orients the diagonal positive; a uniform operator
+// becomes the matrix-free constant mode. A face whose two rows disagree makes the
+// operator non-symmetric: an error unless `symmetrize`, which takes the mean of
+// the two (the level is then the symmetric part, for use as a preconditioner)
+// and reports *symmetric = false.
 static int rows_to_level(const std::vector<double>& c, long long N, const int nn[3], int dim, const int per[3],
-                         double* sign, GLevelHost& L) {
+                         double* sign, GLevelHost& L, bool symmetrize = false, bool* symmetric = nullptr) {
+    if (symmetric) *symmetric = true;
+    auto differ = [](double u, double v) { return std::fabs(u - v) > 1e-12 * std::max(std::fabs(u), std::fabs(v)); };
     // orientation: diagonal must be positive after scaling by sign
     double dsum = 0;
     for (long long r = 0; r < N; ++r) dsum += c[r];
@@ -393,10 +400,11 @@ static int rows_to_level(const std::vector<double>& c, long long N, const int nn
                         ahi = -s * c[(size_t)(2 + 2 * d) * N + cidx(q[0], q[1], q[2])];
                     }
                     double a = std::isnan(alo) ? ahi : alo;
-                    if (!std::isnan(alo) && !std::isnan(ahi)) {
-                        OPF_CHECK(std::fabs(alo - ahi) <= 1e-12 * std::max(std::fabs(alo), std::fabs(ahi)),
-                                  OPF_ERR_UNSUPPORTED, "operator is not symmetric (face %d of dim %d): %g vs %g", i,
-                                  d, alo, ahi);
+                    if (!std::isnan(alo) && !std::isnan(ahi) && differ(alo, ahi)) {
+                        OPF_CHECK(symmetrize, OPF_ERR_UNSUPPORTED,
+                                  "operator is not symmetric (face %d of dim %d): %g vs %g", i, d, alo, ahi);
+                        a = 0.5 * (alo + ahi);
+                        if (symmetric) *symmetric = false;
                     }
                     L.af[d][(size_t)i + (size_t)fn[0] * (j + (size_t)fn[1] * k)] = a;
                     maxa = std::max(maxa, std::fabs(a));
@@ -410,9 +418,12 @@ static int rows_to_level(const std::vector<double>& c, long long N, const int nn
                         int q[3] = {i, j, k};
                         q[d] = nn[d];
                         double& f0 = L.af[d][(size_t)p[0] + (size_t)fn[0] * (p[1] + (size_t)fn[1] * p[2])];
-                        double fnv = L.af[d][(size_t)q[0] + (size_t)fn[0] * (q[1] + (size_t)fn[1] * q[2])];
-                        OPF_CHECK(std::fabs(f0 - fnv) <= 1e-12 * std::max(std::fabs(f0), std::fabs(fnv)),
-                                  OPF_ERR_UNSUPPORTED, "periodic faces disagree in dim %d", d);
+                        double& fnv = L.af[d][(size_t)q[0] + (size_t)fn[0] * (q[1] + (size_t)fn[1] * q[2])];
+                        if (differ(f0, fnv)) {
+                            OPF_CHECK(symmetrize, OPF_ERR_UNSUPPORTED, "periodic faces disagree in dim %d", d);
+                            f0 = fnv = 0.5 * (f0 + fnv);
+                            if (symmetric) *symmetric = false;
+                        }
                     }
         }
     }
@@ -464,17 +475,20 @@ static int eqn_build_operator(opf_eqn* e) {
     OPF_HIP(hipMemcpyAsync(c.data(), e->coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, e->ctx->stream));
     OPF_HIP(hipStreamSynchronize(e->ctx->stream));
     GLevelHost L;
-    OPF_TRY(rows_to_level(c, N, e->n, e->target->d.dim, e->per, &e->sign, L));
+    OPF_TRY(rows_to_level(c, N, e->n, e->target->d.dim, e->per, &e->sign, L, true, &e->symmetric));
     if (e->built) gmg_free(e->H);
     e->H = Hierarchy{};
     if (e->params.pre_smooth > 0) e->H.pre = e->params.pre_smooth;
     if (e->params.post_smooth > 0) e->H.post = e->params.post_smooth;
-    int rc = gmg_build(e->ctx, e->H, L, e->per, e->alpha,
-                       e->params.precision == OPF_PREC_MIXED ? OPF_PREC_MIXED : OPF_PREC_FP64);
+    const bool mixed = e->params.precision == OPF_PREC_MIXED && e->symmetric;
+    int rc = gmg_build(e->ctx, e->H, L, e->per, e->alpha, mixed ? OPF_PREC_MIXED : OPF_PREC_FP64);
     if (rc != OPF_OK) return rc;
     e->built = true;
     return OPF_OK;
 }
+
+// non-symmetric operators: BiCGStab on the pinned system (defined with the set solvers)
+static int eqn_solve_nonsym(opf_eqn* e, opf_solve_state* st, bool pin);
 
 extern "C" int opf_eqn_create(const opf_expr_node* nodes, int nnodes, int lhs_root, int rhs_root,
                               opf_field* const* fields, int nfields, opf_field* target,
@@ -561,6 +575,7 @@ extern "C" int opf_eqn_solve(opf_eqn* e, opf_solve_state* st) {
     // equivalent consistent singular system replaces that row's rhs by minus the sum of the others, and
     // the solution is shifted so that x[first] = 0.
     const bool pin = e->params.pin_value != 0;
+    if (!e->symmetric) return eqn_solve_nonsym(e, st, pin);
     OPF_CHECK(!pin || e->H.singular, OPF_ERR_UNSUPPORTED,
               "pinValue on a non-singular operator is not supported by the multigrid solver");
     if (pin) {
@@ -1012,6 +1027,9 @@ extern "C" int opf_eqns_csr(int neq, const opf_eqn_desc* eqs, opf_field* const* 
 
 namespace opf {
 
+using ApplyFn = std::function<void(hipStream_t, const double*, double*)>;
+using PrecFn = std::function<int(const double*, double*)>;
+
 static double host_sum(const std::vector<double>& v, size_t from, size_t n) {
     double s = 0;
     for (size_t q = 0; q < n; ++q) s += v[from + q];
@@ -1065,11 +1083,12 @@ struct EqsCoupled {
     }
 };
 
-static int eqs_bicgstab(const EqsCoupled& A, double* x, const double* b, double tol, int maxit, opf_solve_state* out,
-                        int verbose) {
-    opf_ctx* ctx = A.S.ctx;
+// Right-preconditioned BiCGStab on M unknowns from the given x: y = A x by `apply`,
+// z = M^-1 r by `precond`; fixed-order dot products; stops at ||b - A x|| <= tol ||b||
+// (recurrence residual; the final true residual is reported).
+static int bicgstab(opf_ctx* ctx, long long M, const ApplyFn& apply, const PrecFn& precond, double* x, const double* b,
+                    double tol, int maxit, opf_solve_state* out, int verbose) {
     hipStream_t st = ctx->stream;
-    const long long M = A.S.N * A.S.neq;
     const unsigned nb = (unsigned)((M + 255) / 256);
     double* w = nullptr;
     OPF_HIP(hipMalloc(&w, sizeof(double) * (8 * M + 1024)));
@@ -1083,9 +1102,12 @@ static int eqs_bicgstab(const EqsCoupled& A, double* x, const double* b, double 
         rc = code;
         return code;
     };
-    do {
+    // the recurrence residual can drift below the true one: a restart from the true
+    // residual continues until the true residual meets the tolerance (at most 4 times)
+    for (int restart = 0; restart < 5 && it < maxit; ++restart) {
+        rho = alpha = omega = 1.0;
         // r = b - A x; r_hat = r; p = v = 0
-        A.apply(st, x, t);
+        apply(st, x, t);
         hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, b, -1.0, (const double*)t);
         if (fail(hipMemcpyAsync(rh, r, sizeof(double) * M, hipMemcpyDeviceToDevice, st) ? OPF_ERR_HIP : OPF_OK)) break;
         if (fail(hipMemsetAsync(p, 0, sizeof(double) * 2 * M, st) ? OPF_ERR_HIP : OPF_OK)) break;  // p, v
@@ -1099,15 +1121,15 @@ static int eqs_bicgstab(const EqsCoupled& A, double* x, const double* b, double 
             rho = rho1;
             hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(256), 0, st, M, p, (const double*)r, (const double*)v, beta,
                                omega);
-            if (fail(A.precond(ctx, p, ph))) break;
-            A.apply(st, ph, v);
+            if (fail(precond(p, ph))) break;
+            apply(st, ph, v);
             double rv = 0;
             if (fail(dots(st, M, rh, v, nullptr, nullptr, part, hp, &rv, nullptr))) break;
             if (rv == 0.0) break;
             alpha = rho / rv;
             hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, sv, (const double*)r, -alpha, (const double*)v);
-            if (fail(A.precond(ctx, sv, sh))) break;
-            A.apply(st, sh, t);
+            if (fail(precond(sv, sh))) break;
+            apply(st, sh, t);
             double ts = 0, tt = 0;
             if (fail(dots(st, M, t, sv, t, t, part, hp, &ts, &tt))) break;
             omega = tt > 0 ? ts / tt : 0.0;
@@ -1116,15 +1138,16 @@ static int eqs_bicgstab(const EqsCoupled& A, double* x, const double* b, double 
             hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, (const double*)sv, -omega, (const double*)t);
             if (fail(dots(st, M, r, r, nullptr, nullptr, part, hp, &rr, nullptr))) break;
             ++it;
-            if (verbose) fprintf(stderr, "[eqns] bicgstab it %d relres %.3e\n", it, bb > 0 ? std::sqrt(rr / bb) : 0.0);
+            if (verbose) fprintf(stderr, "[bicgstab] it %d relres %.3e\n", it, bb > 0 ? std::sqrt(rr / bb) : 0.0);
             if (omega == 0.0) break;
         }
         if (rc) break;
         // the true residual of the final x
-        A.apply(st, x, t);
+        apply(st, x, t);
         hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, M, r, b, -1.0, (const double*)t);
         if (fail(dots(st, M, r, r, nullptr, nullptr, part, hp, &rr, nullptr))) break;
-    } while (false);
+        if (rr <= tgt * (1 + 1e-12) * (1 + 1e-12) || bb == 0) break;
+    }
     hipStreamSynchronize(st);
     hipFree(w);
     if (rc) return rc;
@@ -1160,6 +1183,7 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
     // diagonal blocks -> multigrid levels
     std::vector<GLevelHost> L(neq);
     std::vector<double> sign(neq, 1.0);
+    std::vector<char> symm(neq, 1);
     {
         std::vector<double> c((size_t)PAD_K * N);
         for (int i = 0; i < neq; ++i) {
@@ -1171,14 +1195,20 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
             if (twin >= 0) {
                 L[i] = L[twin];
                 sign[i] = sign[twin];
+                symm[i] = symm[twin];
                 continue;
             }
             OPF_HIP(hipMemcpyAsync(c.data(), S.coef + (size_t)b * PAD_K * N, sizeof(double) * c.size(),
                                    hipMemcpyDeviceToHost, st));
             OPF_HIP(hipStreamSynchronize(st));
-            OPF_TRY(rows_to_level(c, N, S.n, S.dim, S.per[i], &sign[i], L[i]));
+            bool sy = true;
+            OPF_TRY(rows_to_level(c, N, S.n, S.dim, S.per[i], &sign[i], L[i], true, &sy));
+            symm[i] = sy;
         }
     }
+    // a non-symmetric diagonal block is solved by the Krylov method of coupled sets,
+    // its multigrid level (the symmetric part) as preconditioner
+    for (int i = 0; i < neq; ++i) coupled |= !symm[i];
     auto hier = [&](Hierarchy& H, int i, int slots, int precision) {
         if (params[0].pre_smooth > 0) H.pre = params[0].pre_smooth;
         if (params[0].post_smooth > 0) H.post = params[0].post_smooth;
@@ -1271,7 +1301,10 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
             }
             if (rc) break;
             EqsCoupled A{S, H, sign.data(), pin};
-            rc = eqs_bicgstab(A, x, b, tol, maxit, &total, params[0].verbose);
+            rc = bicgstab(
+                    ctx, N * neq, [&](hipStream_t q, const double* u, double* y) { A.apply(q, u, y); },
+                    [&](const double* r, double* z) { return A.precond(ctx, r, z); }, x, b, tol, maxit, &total,
+                    params[0].verbose);
             if (rc) break;
         }
         // the state reports the whole system's true residual (pinned rows included)
@@ -1311,4 +1344,43 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
     for (Hierarchy& G : H) gmg_free(G);
     hipFree(vec);
     return rc;
+}
+
+// HYPREEqnSolveHandler::solve for an operator whose StencilPads are not symmetric
+// (advection terms): the pinned row (pinValue: the first unknown is 0, as
+// HYPREEqnSolveHandler.hpp:145-163 sets it) is kept literally and the system is
+// solved by BiCGStab from the target's values, preconditioned by one V-cycle of
+// the operator's symmetric part; then the usual write-back.
+static int eqn_solve_nonsym(opf_eqn* e, opf_solve_state* st, bool pin) {
+    opf_ctx* ctx = e->ctx;
+    hipStream_t s = ctx->stream;
+    const long long N = e->N;
+    const unsigned nb = (unsigned)((N + 255) / 256);
+    if (pin) OPF_HIP(hipMemsetAsync(e->b, 0, sizeof(double), s));
+    const opf_field_desc& td = e->target->d;
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, N, e->n[0], e->n[1], e->s[0], e->s[1], e->s[2],
+                       (const double*)e->target->data, td.offset[0], td.offset[1], td.offset[2], td.extent[0],
+                       td.extent[1], e->x);
+    EqsBC bc;
+    for (int d = 0; d < 3; ++d) {
+        bc.per[d] = e->per[d];
+        bc.alpha[d][0] = e->alpha[d][0];
+        bc.alpha[d][1] = e->alpha[d][1];
+    }
+    const int dim = td.dim;
+    auto apply = [&](hipStream_t q, const double* u, double* y) {
+        hipLaunchKernelGGL(k_pad_apply, dim3(nb), dim3(256), 0, q, N, e->n[0], e->n[1], e->n[2], dim,
+                           (const double*)e->coef, bc, u, y, e->sign, 0);
+        if (pin) hipLaunchKernelGGL(k_pin_row, dim3(1), dim3(1), 0, q, y, u, 0LL);
+    };
+    auto precond = [&](const double* r, double* z) { return gmg_vcycle64(ctx, e->H, r, z); };
+    const double tol = e->params.tol > 0 ? e->params.tol : 1e-6;
+    const int maxit = e->params.max_iter > 0 ? e->params.max_iter : 200;
+    OPF_TRY(bicgstab(ctx, N, apply, precond, e->x, e->b, tol, maxit, st, e->params.verbose));
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, N, e->n[0], e->n[1], e->s[0], e->s[1], e->s[2],
+                       (double*)e->target->data, td.offset[0], td.offset[1], td.offset[2], td.extent[0], td.extent[1],
+                       (const double*)e->x, (const double*)nullptr);
+    OPF_HIP(hipGetLastError());
+    e->target->spare_valid = false;
+    return launch_ghost_fill(e->target, e->target->data);
 }

@@ -238,3 +238,68 @@ def test_solve_eqns_coupled_with_pinned_neumann_block(ctx):
     print(f"coupled pinned: iters {st.niter} relerr {st.relerr:.2e}", flush=True)
     assert st.relerr <= 1e-11
     assert np.max(np.abs(got - ref)) <= 1e-8 * np.max(np.abs(ref))
+
+
+def _advdiff(f, c=(8.0, 3.0)):
+    # steady advection-diffusion: non-symmetric StencilPads (upwind / downwind differences)
+    return lambda e: of.Equation(f, _lap(e) - c[0] * of.dx(UP, e) + c[1] * of.dy(DOWN, e))
+
+
+@pytest.mark.parametrize("bct,pin", [(BC.Dirc, False), (BC.Neum, True)], ids=["dirichlet", "neumann-pinned"])
+def test_solve_nonsymmetric_single_equation(ctx, bct, pin):
+    """An operator with advection terms (not symmetric) through Solve(): BiCGStab on the
+    system HYPREEqnSolveHandler builds (pinValue pins the FIRST unknown), checked against
+    a dense solve of the oracle's assembly of the same system."""
+    mesh = _mesh(ctx, 24, 20, 1.0, 1.0)
+    om = oracle.Mesh([25, 21], [(0., 1.), (0., 1.)])
+    p = _field(mesh, [((bct, 0.4), (bct, -0.2)), ((bct, 0.0), (bct, 0.3))] if bct == BC.Dirc else _uni(bct), "p")
+    f = _field(mesh, _uni(BC.Dirc, 0.0), "f")
+    f.initBy(lambda x: np.sin(5 * x[0]) * x[1] - 0.3 if pin else np.sin(5 * x[0]) * x[1])
+    A, rhs = _oracle([_advdiff(f)], [p], om)
+    if pin:
+        A[0, :] = 0.0
+        A[0, 0] = 1.0
+        rhs[0] = 0.0
+    ref = np.linalg.solve(A, rhs)
+    p.assign(0.0)
+    st = of.Solve(_advdiff(f), p, of.StructSolverParams(tol=1e-12, pinValue=pin))
+    got = _interior(p)
+    print(f"non-symmetric {bct}: iters {st.niter} relerr {st.relerr:.2e}", flush=True)
+    assert st.relerr <= 1e-12
+    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_solve_eqns_nonsymmetric_block(ctx):
+    # a block-diagonal set whose blocks are not symmetric goes to BiCGStab as well
+    mesh = _mesh(ctx, 16, 16, 1.0, 1.0)
+    om = oracle.Mesh([17, 17], [(0., 1.), (0., 1.)])
+    u = _field(mesh, _uni(BC.Dirc, 0.5), "u")
+    v = _field(mesh, _uni(BC.Dirc, -0.5), "v")
+    f = _field(mesh, _uni(BC.Dirc, 0.0), "f")
+    f.initBy(lambda x: x[0] * x[1])
+    eqs = [lambda a, b: _advdiff(f)(a), lambda a, b: _advdiff(f, (-4.0, 6.0))(b)]
+    A, rhs = _oracle(eqs, [u, v], om)
+    ref = np.linalg.solve(A, rhs)
+    st = of.SolveEqns(*eqs, u, v, None, [of.IJSolverParams(tol=1e-12), of.IJSolverParams()])
+    got = np.concatenate([_interior(u), _interior(v)])
+    assert st.relerr <= 1e-12
+    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_solve_nonsymmetric_at_size_residual(ctx):
+    import scipy.sparse as sp
+    mesh = _mesh(ctx, 256, 256, 1.0, 1.0)
+    p = _field(mesh, _uni(BC.Dirc, 0.0), "p")
+    f = _field(mesh, _uni(BC.Dirc, 0.0), "f")
+    f.initBy(lambda x: np.exp(-20 * ((x[0] - 0.4) ** 2 + (x[1] - 0.6) ** 2)))
+    eq = _advdiff(f, (40.0, 25.0))
+    h = of.makeEqnSolveHandler(eq, p, of.PrecondStructSolver(of.StructSolverParams(tol=1e-10)))
+    ptr, col, val, rhs = h.csr()
+    p.assign(0.0)
+    st = h.solve()
+    h.close()
+    A = sp.csr_matrix((val, col, ptr))
+    x = _interior(p)
+    rel = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
+    print(f"non-symmetric 256^2: iters {st.niter} relerr {st.relerr:.2e} host residual {rel:.2e}", flush=True)
+    assert st.relerr <= 1e-10 and rel <= 1.5e-10
