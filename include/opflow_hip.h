@@ -121,6 +121,13 @@ int opf_ctx_create(int device, opf_ctx** out);
 int opf_ctx_destroy(opf_ctx* ctx);
 int opf_ctx_sync(opf_ctx* ctx);
 void* opf_ctx_stream(opf_ctx* ctx); /* hipStream_t as void* */
+/* Enqueue the context's work on the caller's HIP stream (hipStream_t as void*;
+ * null: back to the context's own stream), ordered after the work already
+ * enqueued. No counterpart in the reference, which has no stream concept; it
+ * lets a host that owns a stream (a framework's current stream, a graph
+ * capture) place the engine's work on it. The stream must stay valid until it
+ * is replaced or the context is destroyed.                                  */
+int opf_ctx_set_stream(opf_ctx* ctx, void* stream);
 /* Engine of expression sweeps (opf_sweep and equation right-hand sides). No
  * counterpart in the reference, whose expressions are C++ templates compiled
  * ahead of time: 0 = the stack-program interpreter, 1 = (default) a kernel

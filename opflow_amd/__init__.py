@@ -92,6 +92,11 @@ class Context:
     def stream(self):
         return self.lib.opf_ctx_stream(self.h)
 
+    def set_stream(self, stream_ptr):
+        """Enqueue this context's work on a caller-owned HIP stream (an int / c_void_p
+        hipStream_t, e.g. torch.cuda.current_stream().cuda_stream); None: own stream."""
+        L.check(self.lib.opf_ctx_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
     def close(self):
         if self.h:
             self.lib.opf_ctx_destroy(self.h)

@@ -136,6 +136,7 @@ SIGNATURES = {
     "opf_eqn_solve": (_I, [_P, C.POINTER(SolveState)]),
     "opf_eqn_csr": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _P, _P, _P, _P]),
     "opf_eqn_destroy": (_I, [_P]),
+    "opf_ctx_set_stream": (_I, [_P, _P]),
     "opf_field_write_cart": (_I, [_P, C.c_char_p, C.c_char_p, _I, C.c_double]),
     "opf_field_read_cart": (_I, [_P, C.c_char_p, C.c_char_p, _I, C.POINTER(C.c_double)]),
     "opf_eqns_solve": (_I, [_I, C.POINTER(EqnDesc), C.POINTER(_P), C.POINTER(SolveParams), C.POINTER(SolveState)]),

@@ -43,6 +43,7 @@ extern "C" int opf_ctx_create(int device, opf_ctx** out) {
         set_error("hipStreamCreate: %s", hipGetErrorString(e));
         return OPF_ERR_HIP;
     }
+    c->own_stream = c->stream;
     c->red_cap = 4096;
     OPF_HIP(hipMalloc(&c->red_buf, c->red_cap * sizeof(double)));
     OPF_HIP(hipHostMalloc(&c->red_host, 16 * sizeof(double), hipHostMallocDefault));
@@ -57,8 +58,23 @@ extern "C" int opf_ctx_destroy(opf_ctx* c) {
     for (auto& p : c->progs) hipFree(p.dev);
     if (c->red_buf) hipFree(c->red_buf);
     if (c->red_host) hipHostFree(c->red_host);
-    hipStreamDestroy(c->stream);
+    if (c->stream != c->own_stream) hipStreamSynchronize(c->own_stream);
+    hipStreamDestroy(c->own_stream);
     delete c;
+    return OPF_OK;
+}
+
+extern "C" int opf_ctx_set_stream(opf_ctx* c, void* stream) {
+    OPF_CHECK(c, OPF_ERR_ARG, "null ctx");
+    hipStream_t next = stream ? (hipStream_t)stream : c->own_stream;
+    if (next == c->stream) return OPF_OK;
+    // keep the order: work already enqueued on the previous stream precedes what follows
+    hipEvent_t ev;
+    OPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    OPF_HIP(hipEventRecord(ev, c->stream));
+    OPF_HIP(hipStreamWaitEvent(next, ev, 0));
+    OPF_HIP(hipEventDestroy(ev));
+    c->stream = next;
     return OPF_OK;
 }
 

@@ -44,7 +44,8 @@ inline size_t dtype_size(int dt) { return dt == OPF_F32 ? 4 : 8; }
 
 struct opf_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the context's work goes
+    hipStream_t own_stream = nullptr;  // the stream the context created (destroyed with it)
     // scratch for deterministic reductions (partials + result)
     double* red_buf = nullptr;
     size_t red_cap = 0;

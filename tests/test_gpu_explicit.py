@@ -342,3 +342,45 @@ def test_reduce_matches_oracle(ctx):
     # known sum: ones
     u.assign(1.0)
     assert of.rangeReduce(u, "sum") == 100 * 76
+
+
+def test_context_on_caller_stream(ctx):
+    """opf_ctx_set_stream: the same FTCS steps and expression sweep enqueued on a
+    caller-owned HIP stream give the same bits as on the context's own stream."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")  # the runtime the library itself links
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    raw = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(raw)) == 0
+    mesh, om = _mesh(ctx, [65, 49, 33])
+    spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
+    D2 = of.D2SecondOrderCentered
+    outs = []
+
+    class _S:
+        cuda_stream = raw.value
+
+        @staticmethod
+        def synchronize():
+            assert hip.hipStreamSynchronize(raw) == 0
+    s = _S
+    for caller in (False, True):
+        u = build_field(ctx, mesh, 3, spec[0], spec[1], spec[2], ext=spec[3])
+        u.upload(oracle.update_padding(u.desc, om, random_storage(u, SEED + 13, 0.5, 1.5)))
+        ctx.sync()
+        try:
+            if caller:
+                ctx.set_stream(s.cuda_stream)
+                assert ctx.stream == s.cuda_stream
+            u.ftcs(1e-4, 7)
+            u.assign(u + 1e-5 * (of.d2x(D2, u) + of.d2z(D2, u)) * u)
+            if caller:
+                s.synchronize()
+        finally:
+            ctx.set_stream(None)
+        outs.append(u.numpy())
+        u.close()
+    assert hip.hipStreamDestroy(raw) == 0
+    np.testing.assert_array_equal(outs[0], outs[1])
