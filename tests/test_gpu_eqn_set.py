@@ -303,3 +303,21 @@ def test_solve_nonsymmetric_at_size_residual(ctx):
     rel = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
     print(f"non-symmetric 256^2: iters {st.niter} relerr {st.relerr:.2e} host residual {rel:.2e}", flush=True)
     assert st.relerr <= 1e-10 and rel <= 1.5e-10
+
+
+def test_solve_eqns_argument_errors(ctx):
+    """Sets the engine cannot take are refused with a message, not solved wrongly."""
+    mesh = _mesh(ctx, 8, 8, 1.0, 1.0)
+    u = _field(mesh, _uni(BC.Dirc), "u")
+    v = _field(mesh, _uni(BC.Dirc), "v")
+    other = _field(_mesh(ctx, 8, 6, 1.0, 1.0), _uni(BC.Dirc), "w")
+    with pytest.raises(of._lib.OpfError, match="mesh|layout"):
+        of.SolveEqns(lambda a, b: of.Equation(1.0, _lap(a)), lambda a, b: of.Equation(1.0, _lap(b)), u, other,
+                     None, None)
+    with pytest.raises(of._lib.OpfError, match="own target"):
+        of.SolveEqns(lambda a, b: of.Equation(1.0, _lap(b)), lambda a, b: of.Equation(1.0, _lap(a)), u, v, None, None)
+    with pytest.raises(of._lib.OpfError, match="no unknown"):
+        of.SolveEqns(lambda a, b: of.Equation(1.0, 2.0), lambda a, b: of.Equation(1.0, _lap(b)), u, v, None, None)
+    with pytest.raises(of._lib.OpfError, match="5/7-point"):  # a diagonal neighbour of b
+        of.SolveEqns(lambda a, b: of.Equation(1.0, _lap(a) + of.dx(UP, of.dy(UP, b))),
+                     lambda a, b: of.Equation(1.0, _lap(b)), u, v, None, None)
