@@ -689,6 +689,68 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ xf, const T* __
     xf[t] += s;
 }
 
+// (P x_c) at fine cell (i, j, k): the (bi/tri)linear weights of k_prolong's
+// general path, boundary ghosts folded (alpha = 1 - bfac), periodic wrap
+template <typename T, int DIM>
+__device__ __forceinline__ T prolong_at(const GOp<T>& o, const T* __restrict__ xc, const int cn[3], int i, int j,
+                                        int k) {
+    const int f[3] = {i, j, k};
+    int par[3], oth[3];
+    for (int d = 0; d < 3; ++d) {
+        par[d] = f[d] >> 1;
+        oth[d] = (f[d] & 1) ? par[d] + 1 : par[d] - 1;
+    }
+    T s = 0;
+    for (int m = 0; m < (1 << DIM); ++m) {
+        T wgt = 1, fac = 1;
+        int q[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; ++d) {
+            const bool use_oth = (m >> d) & 1;
+            wgt *= use_oth ? (T)0.25 : (T)0.75;
+            int c = use_oth ? oth[d] : par[d];
+            if (c < 0 || c >= cn[d]) {
+                if (o.per[d]) c = c < 0 ? c + cn[d] : c - cn[d];
+                else {
+                    fac *= (T)1 - o.bfac[d][c < 0 ? 0 : 1];
+                    c = par[d];
+                }
+            }
+            q[d] = c;
+        }
+        s += wgt * fac * xc[(uint32_t)q[0] + (uint32_t)cn[0] * ((uint32_t)q[1] + (uint32_t)cn[1] * q[2])];
+    }
+    return s;
+}
+
+// first post-smoothing sweep fused with the prolongation: with y = x + P x_c (never
+// stored), xo = y + w (b - A y) / D; the stencil's neighbours of y are formed on
+// the fly. Replaces k_prolong + one Jacobi launch on 2-D levels.
+template <typename T, bool VAR, int DIM>
+__global__ __launch_bounds__(256) void k_pjacobi(GOp<T> o, const T* __restrict__ xi, const T* __restrict__ xc, int c0,
+                                                 int c1, int c2, const T* __restrict__ b, T* __restrict__ xo, T w,
+                                                 Bat bt) {
+    BAT_SLOT(bt);
+    const uint32_t N = o.s2 * (uint32_t)o.n2;
+    xi += vo;
+    b += vo;
+    xo += vo;
+    xc += (long long)slot * ((long long)c0 * c1 * c2);
+    const uint32_t t = cell_id();
+    if (t >= N) return;
+    int i, j, k;
+    cell_of(t, o.f0, o.f1, i, j, k);
+    const int cn[3] = {c0, c1, c2};
+    auto yv = [&](uint32_t l) {
+        int a, e, c;
+        cell_of(l, o.f0, o.f1, a, e, c);
+        return xi[l] + prolong_at<T, DIM>(o, xc, cn, a, e, c);
+    };
+    const T yc = xi[t] + prolong_at<T, DIM>(o, xc, cn, i, j, k);
+    T D;
+    const T ay = apply_op<T, VAR, DIM>(o, i, j, k, t, yc, yv, D);
+    xo[t] = yc + w * (b[t] - ay) / D;
+}
+
 // coarsest level: x = Cinv b (FP64 inverse), one block
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, int n, const T* __restrict__ b,
@@ -1625,6 +1687,17 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         // the prolongation x += P x_c happens inside the first post-smoothing sweep
         zp_launch<T, ZP_PJAC>(H, l, st, bt, (const T*)cur, (const T*)C.x, rhs, nullptr, tmp, (T*)nullptr, nullptr,
                               C.N);
+        std::swap(cur, tmp);
+        p0 = 1;
+    } else if (H.dim == 2 && (H.post >= 2 || (H.post == 1 && !dot_q))) {
+        // 2-D: one launch fewer per level (in 3-D the 7 on-the-fly prolongations of
+        // 8 coarse values each cost more than the launch they save)
+        if (H.var)
+            hipLaunchKernelGGL((k_pjacobi<T, true, 2>), dim3(nblk(N), B), dim3(256), 0, st, o, (const T*)cur,
+                               (const T*)C.x, C.n[0], C.n[1], C.n[2], rhs, tmp, w, bt);
+        else
+            hipLaunchKernelGGL((k_pjacobi<T, false, 2>), dim3(nblk(N), B), dim3(256), 0, st, o, (const T*)cur,
+                               (const T*)C.x, C.n[0], C.n[1], C.n[2], rhs, tmp, w, bt);
         std::swap(cur, tmp);
         p0 = 1;
     } else if (H.dim == 3) {
