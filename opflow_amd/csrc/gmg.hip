@@ -1689,9 +1689,10 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
                               C.N);
         std::swap(cur, tmp);
         p0 = 1;
-    } else if (H.dim == 2 && (H.post >= 2 || (H.post == 1 && !dot_q))) {
-        // 2-D: one launch fewer per level (in 3-D the 7 on-the-fly prolongations of
-        // 8 coarse values each cost more than the launch they save)
+    } else if (H.dim == 2 && L.N <= (1 << 18) && (H.post >= 2 || (H.post == 1 && !dot_q))) {
+        // 2-D levels up to 512^2, where a launch costs more than the work: one launch
+        // fewer per level. Larger levels (and 3-D, with 7 prolongations of 8 coarse
+        // values per cell) keep the separate prolongation, which streams faster.
         if (H.var)
             hipLaunchKernelGGL((k_pjacobi<T, true, 2>), dim3(nblk(N), B), dim3(256), 0, st, o, (const T*)cur,
                                (const T*)C.x, C.n[0], C.n[1], C.n[2], rhs, tmp, w, bt);
