@@ -389,8 +389,18 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     }
 }
 
-// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch
-constexpr int FT_TS2 = 32, FT_TB2 = 8;
+// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch;
+// grids with fewer than 512 such tiles use FT_TS2S^2 tiles and FT_TB2S steps, so
+// that every CU gets several workgroups (256^2 FP64 x 1000 steps: 32x32/8 would
+// leave CUs idle, 16x16/8 2.06 ms, 8x8/5 1.73 ms; 8x8 with 3..7 steps and
+// 4x4 / 12x12 tiles were slower)
+constexpr int FT_TS2 = 32, FT_TB2 = 8, FT_TS2S = 8, FT_TB2S = 5;
+
+// steps per launch of the 2-D blocked kernel for an n[0] x n[1] grid
+inline int tb2_steps(const int n[3]) {
+    const int tiles = ((n[0] + FT_TS2 - 1) / FT_TS2) * ((n[1] + FT_TS2 - 1) / FT_TS2);
+    return tiles < 512 ? FT_TB2S : FT_TB2;
+}
 
 // 3-D blocked launch shape per precision (swept at 512^3 on MI355X): TX x TY
 // columns, NT threads, z chunks of ZC planes, K steps per launch (a remainder of
@@ -412,7 +422,11 @@ struct Ftcs3dCfg<float> {
 // opf_ftcs_plan. tblock = 1 forces single steps.
 int launch_steps(int dim, int dtype, const int n[3], const int per[3], long long plane, int remaining, int tblock) {
     if (tblock == 1) return 1;
-    if (dim == 2 && remaining >= FT_TB2 && n[0] >= FT_TB2 && n[1] >= FT_TB2) return FT_TB2;
+    if (dim == 2) {
+        const int tb = tb2_steps(n);
+        if (remaining >= 2 && n[0] >= tb && n[1] >= tb) return std::min(tb, remaining);
+        return 1;
+    }
     const int k3 = dtype == OPF_F64 ? Ftcs3dCfg<double>::K : Ftcs3dCfg<float>::K;
     if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= 2) return std::min(k3, remaining);
     return 1;
@@ -483,17 +497,18 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
             OPF_TRY(launch_ghost_fill(f, f->data));
             ghosts_stale = false;
         }
-        if (d.dim == 2 && k == FT_TB2) {
+        if (d.dim == 2 && k > 1) {
+            // nt = k <= the kernel's TB steps (a remainder runs the same tile shape)
             int tiles = ((a.n[0] + FT_TS2 - 1) / FT_TS2) * ((a.n[1] + FT_TS2 - 1) / FT_TS2);
-            if (tiles < 512) {
-                // small grids: 16 x 16 tiles so that every CU gets work (more halo recompute, fewer idle CUs)
-                int t16 = ((a.n[0] + 15) / 16) * ((a.n[1] + 15) / 16);
-                if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, 16, FT_TB2>), dim3(t16), dim3(256), 0, st, in, out, a, FT_TB2);
-                else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, 16, FT_TB2>), dim3(t16), dim3(256), 0, st, in, out, a, FT_TB2);
+            if (tb2_steps(a.n) == FT_TB2S) {
+                // small grids: small tiles so that every CU gets several workgroups
+                int ts = ((a.n[0] + FT_TS2S - 1) / FT_TS2S) * ((a.n[1] + FT_TS2S - 1) / FT_TS2S);
+                if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2S, FT_TB2S>), dim3(ts), dim3(256), 0, st, in, out, a, k);
+                else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2S, FT_TB2S>), dim3(ts), dim3(256), 0, st, in, out, a, k);
             } else if (pow2) {
-                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, k);
             } else {
-                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, FT_TB2);
+                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, k);
             }
         } else if (d.dim == 2) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);

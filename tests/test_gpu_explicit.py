@@ -45,7 +45,7 @@ def test_ghost_fill_random_configs(ctx):
         u.close()
 
 
-@pytest.mark.parametrize("n,steps", [(32, 50), (256, 1000)])
+@pytest.mark.parametrize("n,steps", [(32, 50), (256, 1000), (256, 13), (37, 11)])
 def test_ftcs2d_dirichlet_bitexact(ctx, n, steps):
     """config 1 shape: 256^2 centre field, Dirichlet 0, 4 bumps, dt = 0.2 h^2, 1000 steps."""
     mesh, om = _mesh(ctx, [n + 1, n + 1])
@@ -62,15 +62,17 @@ def test_ftcs2d_dirichlet_bitexact(ctx, n, steps):
     u.close()
 
 
-def test_ftcs2d_nonpow2_and_mixed_bc(ctx):
-    n = 40  # h = 1/40 is not a power of two: the division path
+@pytest.mark.parametrize("n,steps", [(40, 30), (6, 9), (45, 17)])
+def test_ftcs2d_nonpow2_and_mixed_bc(ctx, n, steps):
+    # h = 1.3/n is not a power of two (the division path); a periodic y side, a
+    # 6-cell grid narrower than the blocked kernel's tile plus halo, remainders
     mesh, om = _mesh(ctx, [n + 1, n + 1], [(0.0, 1.3), (0.0, 0.7)])
     u = build_field(ctx, mesh, 2, 1, [(NEUM, DIRC), (PER, PER)], [(0.3, -0.5), (0, 0)])
     init = oracle.update_padding(u.desc, om, random_storage(u, SEED + 2))
     u.upload(init)
     coef = 0.2 * (1.3 / n) ** 2
-    u.ftcs(coef, 30)
-    ref = oracle.ftcs(u.desc, om, init, coef, 30)
+    u.ftcs(coef, steps)
+    ref = oracle.ftcs(u.desc, om, init, coef, steps)
     np.testing.assert_array_equal(u.numpy(), ref)
     u.close()
 
