@@ -1,11 +1,16 @@
 """Oracle: CPU restatement of OpFlow's hot path -- TEST INFRASTRUCTURE ONLY.
 
 Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-leg, as the checker. Two parts:
+leg, as the checker. Three parts:
   * liboracle.so (oracle.c): ranges, updatePadding, FTCS with OpFlow's
     FieldAssigner sequence, reductions, CSR assembly and the CPU PCG solver;
   * expr.py: numpy evaluator of OpFlow expression trees (lazy evalAt
-    recursion of Expr/Expression.hpp:51-65 with each operator's eval).
+    recursion of Expr/Expression.hpp:51-65 with each operator's eval), and
+    linear_system, the CSRMatrixGenerator system of an equation set taken by
+    linearity from that evaluator;
+  * cart.py: the byte layout of the raw binary field streams.
+Each function's pin (golden arrays, closed forms, documented structure) is
+named in the test that checks it.
 """
 import ctypes as C
 import os
