@@ -138,8 +138,10 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
         if (a.per[1]) gy = ((gy % a.n[1]) + a.n[1]) % a.n[1];
         return (gx + 1) + (long long)a.e0 * (gy + 1);
     };
-    for (int t = threadIdx.x; t < wx * wy; t += blockDim.x) {
-        int lx = t % wx, ly = t / wx;
+    // positions walk the full W x W square (a compile-time divisor); clamped regions skip the rest
+    for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+        const int lx = t % W, ly = t / W;
+        if (lx >= wx || ly >= wy) continue;
         buf[0][ly][lx] = in[gidx(lo[0] + lx, lo[1] + ly)];
     }
     __syncthreads();
@@ -147,8 +149,9 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
     for (int s = 0; s < nt; ++s) {
         const int nxt = 1 - cur;
         // interior cells with all four neighbours inside the region
-        for (int t = threadIdx.x; t < wx * wy; t += blockDim.x) {
-            int lx = t % wx, ly = t / wx;
+        for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+            const int lx = t % W, ly = t / W;
+            if (lx >= wx || ly >= wy) continue;
             int gx = lo[0] + lx, gy = lo[1] + ly;
             bool inx = a.per[0] || (gx >= 0 && gx < a.n[0]);
             bool iny = a.per[1] || (gy >= 0 && gy < a.n[1]);
@@ -159,8 +162,10 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
             buf[nxt][ly][lx] = v;
         }
         __syncthreads();
-        // ghosts of non-periodic x sides (rows inside the y interior), then y sides (all columns)
-        if (!a.per[0]) {
+        // ghosts of non-periodic x sides (rows inside the y interior), then y sides (all
+        // columns); only tiles whose region reaches a non-periodic side have any (a
+        // condition uniform over the workgroup, so the barriers stay uniform)
+        if (!a.per[0] && (lo[0] == -1 || hi[0] == a.n[0] + 1)) {
             for (int ly = threadIdx.x; ly < wy; ly += blockDim.x) {
                 int gy = lo[1] + ly;
                 if (!a.per[1] && (gy < 0 || gy >= a.n[1])) continue;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
             }
             __syncthreads();
         }
-        if (!a.per[1]) {
+        if (!a.per[1] && (lo[1] == -1 || hi[1] == a.n[1] + 1)) {
             for (int lx = threadIdx.x; lx < wx; lx += blockDim.x) {
                 if (lo[1] == -1) buf[nxt][0][lx] = rule(a, 1, 0, buf[nxt][1][lx]);
                 if (hi[1] == a.n[1] + 1) buf[nxt][wy - 1][lx] = rule(a, 1, 1, buf[nxt][wy - 2][lx]);
