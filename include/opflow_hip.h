@@ -191,7 +191,7 @@ int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_field* const
 int opf_ftcs(opf_field* u, double coef, int nsteps);
 /* Same update with the temporal-blocking choice made by the caller:
  * tblock = 1 runs one launch per step; tblock = 0 lets the library fuse steps
- * inside one launch where it can (2-D: 8 steps per launch, 5 on small grids, any BC;
+ * inside one launch where it can (2-D: 4 steps per launch, 5 on small grids, any BC;
  * 3-D: 3 steps per launch when periodic in every dimension, one otherwise),
  * which is what opf_ftcs does. Results are bit-identical either way.       */
 int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);

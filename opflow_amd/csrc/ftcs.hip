@@ -394,12 +394,13 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     }
 }
 
-// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch;
+// launch shapes: 2-D tiles of FT_TS2^2 cells advanced FT_TB2 steps per launch
+// (swept at 1024^2..4096^2: 32x32 with 4 steps beat 2, 3, 5, 6, 8 and 10 steps
+// and 16..64-cell tiles; 4096^2 FP64 288 -> 390 G cell-updates/s against 8 steps);
 // grids with fewer than 512 such tiles use FT_TS2S^2 tiles and FT_TB2S steps, so
-// that every CU gets several workgroups (256^2 FP64 x 1000 steps: 32x32/8 would
-// leave CUs idle, 16x16/8 2.06 ms, 8x8/5 1.73 ms; 8x8 with 3..7 steps and
-// 4x4 / 12x12 tiles were slower)
-constexpr int FT_TS2 = 32, FT_TB2 = 8, FT_TS2S = 8, FT_TB2S = 5;
+// that every CU gets several workgroups (256^2 FP64 x 1000 steps: 16x16/8 2.06 ms,
+// 8x8/5 1.73 ms; 8x8 with 3..7 steps and 4x4 / 12x12 tiles were slower)
+constexpr int FT_TS2 = 32, FT_TB2 = 4, FT_TS2S = 8, FT_TB2S = 5;
 
 // steps per launch of the 2-D blocked kernel for an n[0] x n[1] grid
 inline int tb2_steps(const int n[3]) {
