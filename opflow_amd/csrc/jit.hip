@@ -7,7 +7,8 @@
 // reference's compile-time expression templates (Expression.hpp:51-65), at
 // run time: the source is compiled for gfx950 with hipRTC (-O3, no FMA
 // contraction, no packed FP32), loaded as a module and cached in the process
-// and on disk (code objects keyed by a hash of source, options and device), so
+// and on disk (code objects keyed by a hash of source, options, compiler
+// version and device), so
 // each expression is compiled once per machine. The context's jit mode picks
 // the engine per sweep: the interpreter, the compiled kernel for sweeps of at
 // least 2^18 cells (default), or the compiled kernel always.
@@ -242,6 +243,8 @@ int jit_sweep_function(const LoweredProgram& P, int dtype, int op, hipFunction_t
     OPF_HIP(hipGetDevice(&dev));
     uint64_t h = fnv1a64(src.data(), src.size());
     for (int q = 0; q < kNOpts; ++q) h = fnv1a64(kOpts[q], strlen(kOpts[q]), h);
+    int ver[2] = {0, 0};  // a compiler update invalidates the disk cache
+    if (hiprtcVersion(&ver[0], &ver[1]) == HIPRTC_SUCCESS) h = fnv1a64(ver, sizeof ver, h);
     const uint64_t key = fnv1a64(&dev, sizeof dev, h);
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_fns.find(key);
