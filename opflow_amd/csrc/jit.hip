@@ -39,6 +39,9 @@ const char* const kArgsSrc =
 const char* const kWenoSrc =
 #include "weno_device.inc"
     ;
+const char* const kDivSrc =
+#include "div_device.inc"
+    ;
 #undef OPF_SHARED_SRC
 
 const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
@@ -64,6 +67,7 @@ int gen_source(const LoweredProgram& P, int dtype, int op, std::string& s) {
     s += kArgsSrc;
     s += "\n";
     s += kWenoSrc;
+    s += kDivSrc;
     s += "\nextern \"C\" __global__ __launch_bounds__(256) void opf_sweep_jit(SweepArgs a) {\n"
          "  const int i = a.s[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
          "  const int j = a.s[1] + (int)blockIdx.y * 4 + (int)threadIdx.y;\n"
@@ -127,7 +131,11 @@ int gen_source(const LoweredProgram& P, int dtype, int op, std::string& s) {
             case I_DIV: {
                 int x, y;
                 if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
-                const char o = in.op == I_ADD ? '+' : in.op == I_SUB ? '-' : in.op == I_MUL ? '*' : '/';
+                if (in.op == I_DIV) {
+                    snprintf(line, sizeof line, "  const T v%d = opf_div(v%d, v%d);\n", id, x, y);
+                    break;
+                }
+                const char o = in.op == I_ADD ? '+' : in.op == I_SUB ? '-' : '*';
                 snprintf(line, sizeof line, "  const T v%d = v%d %c v%d;\n", id, x, o, y);
             } break;
             case I_NEG: {
@@ -158,7 +166,7 @@ int gen_source(const LoweredProgram& P, int dtype, int op, std::string& s) {
         case OPF_OP_ADD: snprintf(line, sizeof line, "  ((T*)a.out)[lo] = ((const T*)a.old)[lo] + v%d;\n}\n", r); break;
         case OPF_OP_MINUS: snprintf(line, sizeof line, "  ((T*)a.out)[lo] = ((const T*)a.old)[lo] - v%d;\n}\n", r); break;
         case OPF_OP_MUL: snprintf(line, sizeof line, "  ((T*)a.out)[lo] = ((const T*)a.old)[lo] * v%d;\n}\n", r); break;
-        case OPF_OP_DIV: snprintf(line, sizeof line, "  ((T*)a.out)[lo] = ((const T*)a.old)[lo] / v%d;\n}\n", r); break;
+        case OPF_OP_DIV: snprintf(line, sizeof line, "  ((T*)a.out)[lo] = opf_div(((const T*)a.old)[lo], v%d);\n}\n", r); break;
         default: return OPF_ERR_ARG;
     }
     s += line;

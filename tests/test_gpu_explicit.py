@@ -224,6 +224,7 @@ OPS = {
     "down": lambda u: u - 0.01 * of.dx(of.D1FirstOrderBiasedDownwind, u) / u,
     "intp": lambda u: of.d1IntpCornerToCenter(0, of.d1IntpCenterToCorner(0, u)) * -1.0,
     "neg": lambda u: -(u * u) / (u + 3.0),
+    "pow2div": lambda u: u / 0.25 - u / -1024.0 + u / 3.0,
 }
 
 
