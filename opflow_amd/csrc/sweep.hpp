@@ -56,7 +56,7 @@ int lower_tree_set(const opf_expr_node* nodes, int nnodes, int root, opf_field* 
 // engine the context's jit mode selects (interpreter or the compiled kernel of P)
 int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P);
 // the compiled kernel of P (hipRTC, cached) and its launch
-int jit_sweep_function(const LoweredProgram& P, int dtype, int op, hipFunction_t* fn);
+int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, hipFunction_t* fn);
 int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
                  const int n[3]);
