@@ -22,7 +22,3 @@ def test_ghost_index_math(tmp_path):
 
 def test_fastdiv_matches_hardware_divide(tmp_path):
     assert "passed" in _build_run("fastdiv_check", tmp_path)
-
-
-def test_jit_division_matches_ieee_divide(tmp_path):
-    assert "passed" in _build_run("div_check", tmp_path)
