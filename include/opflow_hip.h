@@ -200,6 +200,20 @@ int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);
  * counterpart in the reference (its FTCS loop is one rangeFor pass per step);
  * it lets a caller relate launch timings to steps.                          */
 int opf_ftcs_plan(const opf_field* u, int nsteps, int tblock, int* launches, int* max_steps);
+/* The update of opf_ftcs with no device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc). opf_ftcs keeps a
+ * second time level per field, allocated on the field's first update; this
+ * entry point uses a fixed device scratch of OPF_FTCS_NOALLOC_MAX_BYTES per
+ * device instead (held from library load to process end), copies the field
+ * into it, ping-pongs between the two and leaves the result in the field's own
+ * storage. It calls no hipMalloc, hipFree or other allocating routine. A field
+ * whose storage exceeds the scratch is rejected with OPF_ERR_NOMEM before any
+ * work. Results are bit-identical to opf_ftcs. The scratch is shared by every
+ * call on a device: calls that overlap on one device (other threads, other
+ * contexts' streams) race on it; calls ordered on one stream do not.       */
+#define OPF_FTCS_NOALLOC_MAX_BYTES (64u << 20)
+int opf_ftcs_noalloc(opf_field* u, double coef, int nsteps);
 
 /* ---- reductions ----------------------------------------------------------- */
 /* Replaces rangeReduce (Loops/RangeFor.hpp:86-121) for the common reducers;

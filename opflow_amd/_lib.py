@@ -19,6 +19,8 @@ OPF_ERR_NOMEM = -5
 OPF_ERR_NOTREADY = -6
 
 OPF_F64, OPF_F32 = 0, 1
+# device scratch of opf_ftcs_noalloc (OPF_FTCS_NOALLOC_MAX_BYTES in the header)
+OPF_FTCS_NOALLOC_MAX_BYTES = 64 << 20
 
 
 class OpfError(RuntimeError):
@@ -123,6 +125,7 @@ SIGNATURES = {
     "opf_ftcs": (_I, [_P, _D, _I]),
     "opf_ftcs_blocked": (_I, [_P, _D, _I, _I]),
     "opf_ftcs_plan": (_I, [_P, _I, _I, _IP, _IP]),
+    "opf_ftcs_noalloc": (_I, [_P, _D, _I]),
     "opf_reduce": (_I, [_P, _I, _IP, _IP, _DP]),
     "opf_buf_create": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "opf_buf_destroy": (_I, [_P, _P]),

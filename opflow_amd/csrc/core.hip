@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "opf_internal.hpp"
 #include "ghost_index.hpp"
@@ -27,6 +28,33 @@ extern "C" const char* opf_last_error(void) { return g_err; }
 extern "C" const char* opf_version(void) { return "opflow-hip 0.1 (gfx950)"; }
 
 // ---------------------------------------------------------------- context
+namespace {
+// Device scratch of the allocation-free entry point opf_ftcs_noalloc: the
+// second time-level buffer of the FTCS ping-pong. Every device the library runs
+// on holds its own copy until the process ends. Host code never passes
+// `scratch` itself to a kernel or a copy (on the host the name is only a
+// stand-in); it uses the per-device address looked up once below.
+__device__ __align__(256) unsigned char scratch[OPF_FTCS_NOALLOC_MAX_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current: the address of its copy.
+int scratch_setup(int device, void** out) {
+    OPF_CHECK(device >= 0 && device < kMaxDevices, OPF_ERR_ARG, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        OPF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        OPF_CHECK(p != nullptr, OPF_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return OPF_OK;
+}
+}  // namespace
+
 extern "C" int opf_ctx_create(int device, opf_ctx** out) {
     OPF_CHECK(out, OPF_ERR_ARG, "opf_ctx_create: null out");
     int n = 0;
@@ -44,6 +72,11 @@ extern "C" int opf_ctx_create(int device, opf_ctx** out) {
         return OPF_ERR_HIP;
     }
     c->own_stream = c->stream;
+    if (int rc = scratch_setup(device, &c->noalloc_scratch); rc != OPF_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     c->red_cap = 4096;
     OPF_HIP(hipMalloc(&c->red_buf, c->red_cap * sizeof(double)));
     OPF_HIP(hipHostMalloc(&c->red_host, 16 * sizeof(double), hipHostMallocDefault));

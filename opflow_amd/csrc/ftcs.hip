@@ -445,8 +445,11 @@ bool is_pow2(double h) {
     return m == 0.5;
 }
 
+// ext_spare: null = the field's own second buffer (allocated on first use);
+// otherwise a caller-provided device buffer of at least f->nbytes used as the
+// second time level, with the result copied back into f->data at the end.
 template <typename T>
-int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
+int ftcs_t(opf_field* f, double coef, int nsteps, int tblock, void* ext_spare = nullptr) {
     const opf_field_desc& d = f->d;
     const opf_mesh* m = f->mesh;
     FtcsArgs<T> a{};
@@ -486,21 +489,30 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
     if (pow2)
         for (int q = 0; q < d.dim; ++q) a.hh[q] = (T)(1.0 / m->h[q]);
     hipStream_t st = f->ctx->stream;
-    if (!f->spare) {
-        OPF_HIP(hipMalloc(&f->spare, f->nbytes));
-        f->spare_valid = false;
-    }
-    if (!f->spare_valid) {
-        OPF_HIP(hipMemcpyAsync(f->spare, f->data, f->nbytes, hipMemcpyDeviceToDevice, st));
-        f->spare_valid = true;
+    void* cur = f->data;  // time level the next launch reads
+    void* nxt = nullptr;  // time level it writes
+    if (ext_spare) {
+        // every cell the kernels do not write keeps the field's value
+        OPF_HIP(hipMemcpyAsync(ext_spare, f->data, f->nbytes, hipMemcpyDeviceToDevice, st));
+        nxt = ext_spare;
+    } else {
+        if (!f->spare) {
+            OPF_HIP(hipMalloc(&f->spare, f->nbytes));
+            f->spare_valid = false;
+        }
+        if (!f->spare_valid) {
+            OPF_HIP(hipMemcpyAsync(f->spare, f->data, f->nbytes, hipMemcpyDeviceToDevice, st));
+            f->spare_valid = true;
+        }
+        nxt = f->spare;
     }
     bool ghosts_stale = false;  // the 3-D blocked kernel writes interior cells only
     for (int it = 0; it < nsteps;) {
-        const T* in = (const T*)f->data;
-        T* out = (T*)f->spare;
+        const T* in = (const T*)cur;
+        T* out = (T*)nxt;
         const int k = launch_steps(d.dim, d.dtype, a.n, a.per, (long long)a.e0 * a.e1, nsteps - it, tblock);
         if (ghosts_stale && !(d.dim == 3 && k > 1 && a.per[0] && a.per[1] && a.per[2])) {
-            OPF_TRY(launch_ghost_fill(f, f->data));
+            OPF_TRY(launch_ghost_fill(f, cur));
             ghosts_stale = false;
         }
         if (d.dim == 2 && k > 1) {
@@ -546,10 +558,16 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock) {
             return OPF_ERR_UNSUPPORTED;
         }
         it += k;
-        std::swap(f->data, f->spare);
+        std::swap(cur, nxt);
     }
-    if (ghosts_stale) OPF_TRY(launch_ghost_fill(f, f->data));
+    if (ghosts_stale) OPF_TRY(launch_ghost_fill(f, cur));
     OPF_HIP(hipGetLastError());
+    if (ext_spare) {
+        if (cur != f->data) OPF_HIP(hipMemcpyAsync(f->data, cur, f->nbytes, hipMemcpyDeviceToDevice, st));
+    } else {
+        f->data = cur;
+        f->spare = nxt;
+    }
     return OPF_OK;
 }
 
@@ -559,6 +577,18 @@ extern "C" int opf_ftcs(opf_field* u, double coef, int nsteps) {
     OPF_CHECK(u && nsteps >= 0, OPF_ERR_ARG, "opf_ftcs: bad args");
     if (nsteps == 0) return OPF_OK;
     return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps, 0) : ftcs_t<float>(u, coef, nsteps, 0);
+}
+
+extern "C" int opf_ftcs_noalloc(opf_field* u, double coef, int nsteps) {
+    OPF_CHECK(u && nsteps >= 0, OPF_ERR_ARG, "opf_ftcs_noalloc: bad args");
+    OPF_CHECK(u->nbytes <= (size_t)OPF_FTCS_NOALLOC_MAX_BYTES, OPF_ERR_NOMEM,
+              "opf_ftcs_noalloc: field storage of %zu bytes exceeds the %zu-byte device scratch", u->nbytes,
+              (size_t)OPF_FTCS_NOALLOC_MAX_BYTES);
+    OPF_CHECK(u->ctx->noalloc_scratch, OPF_ERR_HIP, "opf_ftcs_noalloc: no scratch address for device %d",
+              u->ctx->device);
+    if (nsteps == 0) return OPF_OK;
+    void* s = u->ctx->noalloc_scratch;
+    return u->d.dtype == OPF_F64 ? ftcs_t<double>(u, coef, nsteps, 0, s) : ftcs_t<float>(u, coef, nsteps, 0, s);
 }
 
 extern "C" int opf_ftcs_plan(const opf_field* u, int nsteps, int tblock, int* launches, int* max_steps) {

@@ -57,6 +57,8 @@ struct opf_ctx {
         size_t bytes;
     };
     std::vector<ProgEntry> progs;
+    // this device's copy of the allocation-free FTCS scratch (core.hip), looked up at creation
+    void* noalloc_scratch = nullptr;
     // expression sweeps: 0 = interpreter, 1 = compiled kernel for >= 2^18 cells, 2 = compiled kernel always
     int jit_mode = 1;
 };

@@ -74,3 +74,10 @@ def test_library_refuses_without_gpu_cleanly():
     assert rc in (L.OPF_OK, L.OPF_ERR_ARG, L.OPF_ERR_HIP)
     if rc == L.OPF_OK:
         lib.opf_ctx_destroy(h)
+
+
+def test_noalloc_scratch_size_matches_header():
+    txt = open(os.path.join(ROOT, "include", "opflow_hip.h")).read()
+    m = re.search(r"#define OPF_FTCS_NOALLOC_MAX_BYTES \((\d+)u << (\d+)\)", txt)
+    assert m, "OPF_FTCS_NOALLOC_MAX_BYTES not found"
+    assert int(m.group(1)) << int(m.group(2)) == L.OPF_FTCS_NOALLOC_MAX_BYTES
