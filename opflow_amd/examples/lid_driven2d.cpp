@@ -4,12 +4,18 @@
 // Operator vocabulary follows examples/LidDriven/LidDriven2D.cpp:29-41 and its
 // pressure handler (:67-74); the momentum step is explicit here.
 //
-// usage: lid_driven2d [n=1024] [steps=500] [mixed=1] [out.bin]
+// usage: lid_driven2d [n=1024] [steps=500] [mixed=1] [out.bin] [dump_dir dump_steps]
+//   dump_steps: comma-separated step indices; at each, the storage blocks of
+//   u, v before the step (<dir>/s<i>_u_n.bin, _v_n), after the momentum update
+//   (_u_star, _v_star), p after the solve (_p) and u, v after the projection
+//   (_u_np1, _v_np1) are written as raw doubles, for checks outside the engine.
 #include <OpFlow>
 
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <set>
+#include <string>
 
 using namespace OpFlow;
 
@@ -19,7 +25,25 @@ int main(int argc, char** argv) {
     const int n = argc > 1 ? std::atoi(argv[1]) : 1024;
     const int steps = argc > 2 ? std::atoi(argv[2]) : 500;
     const bool mixed = argc > 3 ? std::atoi(argv[3]) != 0 : true;
-    const char* out = argc > 4 ? argv[4] : nullptr;
+    const char* out = argc > 4 && argv[4][0] ? argv[4] : nullptr;
+    const std::string dump_dir = argc > 5 ? argv[5] : "";
+    std::set<int> dump_steps;
+    if (argc > 6)
+        for (const char* c = argv[6]; *c;) {
+            char* e;
+            dump_steps.insert((int)std::strtol(c, &e, 10));
+            c = *e ? e + 1 : e;
+        }
+    auto dump = [&](int step, const char* tag, const auto& f) {
+        const auto& h = f.hostData();
+        std::string path = dump_dir + "/s" + std::to_string(step) + "_" + tag + ".bin";
+        FILE* fp = std::fopen(path.c_str(), "wb");
+        if (!fp || std::fwrite(h.data(), sizeof(h[0]), h.size(), fp) != h.size()) {
+            std::fprintf(stderr, "cannot write %s\n", path.c_str());
+            std::exit(1);
+        }
+        std::fclose(fp);
+    };
 
     auto mesh = MeshBuilder<Mesh>().newMesh(n + 1, n + 1).setMeshOfDim(0, 0., 1.).setMeshOfDim(1, 0., 1.).build();
     auto builder = ExprBuilder<Field>().setMesh(mesh).setExt(1)
@@ -79,15 +103,23 @@ int main(int argc, char** argv) {
             opf_ctx_sync(globalContext()->h);
             t_first = std::chrono::steady_clock::now();
         }
+        const bool dumping = !dump_dir.empty() && dump_steps.count(i);
+        if (dumping) dump(i, "u_n", u), dump(i, "v_n", v);
         du = dt * (nu * (d2x<D2SecondOrderCentered>(u) + d2y<D2SecondOrderCentered>(u)) - (conv_xx(u, u) + conv_xy(u, v)));
         dv = dt * (nu * (d2x<D2SecondOrderCentered>(v) + d2y<D2SecondOrderCentered>(v)) - (conv_yx(u, v) + conv_yy(v, v)));
         u = u + du;
         v = v + dv;
+        if (dumping) dump(i, "u_star", u), dump(i, "v_star", v);
         auto st = p_handler->solve();
         iters += st.niter;
         worst = std::max(worst, st.relerr);
+        if (dumping) {
+            dump(i, "p", p);
+            std::printf("{\"dump_step\": %d, \"niter\": %d, \"relres\": %.6e}\n", i, st.niter, st.relerr);
+        }
         u = u - dt * dx<D1FirstOrderCentered>(p);
         v = v - dt * dy<D1FirstOrderCentered>(p);
+        if (dumping) dump(i, "u_np1", u), dump(i, "v_np1", v);
     }
     const auto& hu = u.hostData();
     const auto& hv = v.hostData();

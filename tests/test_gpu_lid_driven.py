@@ -36,3 +36,109 @@ def test_lid_driven_mixed_vs_fp64(tmp_path):
     assert a["max_div"] < 1e-5 * n
     # the lid drags the fluid: u below the lid is positive, the centre return flow negative
     assert a["u_center"] < 0
+
+
+def _lid_fields(ctx, n):
+    """Python fields with the example's layouts and BCs (for descriptors and expression trees)."""
+    import opflow_amd as of
+    from helpers import build_field
+    DIRC, NEUM = 1, 2
+    mesh = of.MeshBuilder(ctx).newMesh(n + 1, n + 1).setMeshOfDim(0, 0., 1.).setMeshOfDim(1, 0., 1.).build()
+    u = build_field(ctx, mesh, 2, [0, 1], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 1.0)], name="u")
+    v = build_field(ctx, mesh, 2, [1, 0], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 0)], name="v")
+    p = build_field(ctx, mesh, 2, [1, 1], [(NEUM, NEUM), (NEUM, NEUM)], [(0, 0), (0, 0)], name="p")
+    du = build_field(ctx, mesh, 2, [0, 1], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 1.0)], name="du")
+    dv = build_field(ctx, mesh, 2, [1, 0], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 0)], name="dv")
+    return u, v, p, du, dv
+
+
+def _assign(expr, arrays, om, out, out_arr):
+    import oracle
+    nodes, flds, root = expr.flatten()
+    slots = [(arrays[f.name], f.desc) for f in flds]
+    return oracle.assign(nodes, root, slots, om, out_arr, out.desc, 0,
+                         update_padding=lambda r: oracle.update_padding(out.desc, om, r))
+
+
+def _evaluate(expr, arrays, om, n):
+    import oracle
+    from oracle import expr as oexpr
+    nodes, flds, root = expr.flatten()
+    slots = [(arrays[f.name], f.desc) for f in flds]
+    return oexpr.evaluate(nodes, root, slots, om, [0, 0], [n, n])
+
+
+def test_lid_driven_full_size_against_oracle(ctx, tmp_path):
+    """The 1024^2 x 500-step configuration, FP64 and mixed, checked outside the engine at a
+    second step and the last one: the momentum update and the projection equal the oracle's
+    evaluation of the same expressions bit for bit, the dumped pressure meets the equation
+    d2x p + d2y p = div(u*)/dt to the solver tolerance (residual recomputed by the oracle,
+    first unknown pinned at 0, HYPREEqnSolveHandler.hpp:145-163), the projected velocity's
+    divergence equals -dt times that residual, and the mixed run's velocity equals the FP64
+    run's to 1e-6. Reference: examples/LidDriven/LidDriven2D.cpp:30-41,67-74."""
+    import opflow_amd as of
+    import oracle
+    n, steps, dump_steps = 1024, 500, (1, 499)
+    h = 1.0 / n
+    nu = 1e-3
+    dt = min(0.25 * h * h / nu, 0.5 * h)
+    u, v, p, du, dv = _lid_fields(ctx, n)
+    om = oracle.Mesh([n + 1, n + 1])
+    D1, D2 = of.D1FirstOrderCentered, of.D2SecondOrderCentered
+    I_c2n, I_n2c = of.d1IntpCenterToCorner, of.d1IntpCornerToCenter
+    du_e = dt * (nu * (of.d2x(D2, u) + of.d2y(D2, u)) - (of.dx(D1, I_n2c(0, u) * I_n2c(0, u)) +
+                                                         of.dy(D1, I_c2n(1, u) * I_c2n(0, v))))
+    dv_e = dt * (nu * (of.d2x(D2, v) + of.d2y(D2, v)) - (of.dx(D1, I_c2n(1, u) * I_c2n(0, v)) +
+                                                         of.dy(D1, I_n2c(1, v) * I_n2c(1, v))))
+    lap_e = of.d2x(D2, p) + of.d2y(D2, p)
+    div_e = of.dx(D1, u) + of.dy(D1, v)
+    shapes = {"u": u.shape, "v": v.shape, "p": p.shape}
+    finals = {}
+    for mixed in (False, True):
+        d = tmp_path / ("mixed" if mixed else "fp64")
+        d.mkdir()
+        r = subprocess.run([os.path.join(BIN, "lid_driven2d"), str(n), str(steps), str(int(mixed)),
+                            str(d / "final.bin"), str(d), ",".join(map(str, dump_steps))],
+                           capture_output=True, text=True, timeout=300)
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0
+        lines = [json.loads(x) for x in r.stdout.strip().splitlines()]
+        summary = lines[-1]
+        assert summary["worst_relres"] <= 1e-8
+        finals[mixed] = np.fromfile(d / "final.bin")
+
+        def load(step, tag):
+            return np.fromfile(d / f"s{step}_{tag}.bin").reshape(shapes[tag.split("_")[0]])
+
+        for s in dump_steps:
+            un, vn = load(s, "u_n"), load(s, "v_n")
+            us, vs, ps = load(s, "u_star"), load(s, "v_star"), load(s, "p")
+            # momentum: du = dt*(nu*lap - conv), then u = u + du (bit for bit)
+            a = {"u": un, "v": vn}
+            du_ref = _assign(du_e, a, om, du, np.zeros(du.shape))
+            dv_ref = _assign(dv_e, a, om, dv, np.zeros(dv.shape))
+            us_ref = _assign(u + du, {"u": un, "du": du_ref}, om, u, un)
+            vs_ref = _assign(v + dv, {"v": vn, "dv": dv_ref}, om, v, vn)
+            np.testing.assert_array_equal(us, us_ref)
+            np.testing.assert_array_equal(vs, vs_ref)
+            # pressure: residual of the equation, recomputed outside the engine
+            rhs = _evaluate(div_e / dt, {"u": us, "v": vs}, om, n)
+            res = _evaluate(lap_e, {"p": ps}, om, n) - rhs
+            assert ps[1, 1] == 0.0  # first unknown pinned (storage has one ghost layer)
+            rel = np.linalg.norm(res.ravel()[1:]) / np.linalg.norm(rhs)
+            # projection: u = u - dt*dx(p) (bit for bit), and div(u^{n+1}) = dt*(rhs - lap p)
+            un1_ref = _assign(u - dt * of.dx(D1, p), {"u": us, "p": ps}, om, u, us)
+            vn1_ref = _assign(v - dt * of.dy(D1, p), {"v": vs, "p": ps}, om, v, vs)
+            np.testing.assert_array_equal(load(s, "u_np1"), un1_ref)
+            np.testing.assert_array_equal(load(s, "v_np1"), vn1_ref)
+            div1 = _evaluate(div_e, {"u": un1_ref, "v": vn1_ref}, om, n)
+            ident = np.max(np.abs(div1 + dt * res))
+            print(f"mixed={mixed} step {s}: oracle relres {rel:.3e}, max|div u^(n+1)| {np.max(np.abs(div1)):.3e}, "
+                  f"|div + dt*res| {ident:.3e}, max|div u*| {np.max(np.abs(rhs)) * dt:.3e}", flush=True)
+            assert rel <= 1.05e-8
+            assert ident <= 1e-9
+    rel_uv = np.max(np.abs(finals[True] - finals[False])) / np.max(np.abs(finals[False]))
+    print("velocity max |mixed - fp64| / max|u| =", rel_uv)
+    assert rel_uv <= 1e-6
+    for f in (u, v, p, du, dv):
+        f.close()

@@ -115,3 +115,26 @@ def test_device_rhs_generator_matches_host(ctx):
         dev = buf.numpy()
         buf.close()
         assert np.max(np.abs(dev - host)) <= 1e-14 * np.max(np.abs(host)), (n, nb, noise)
+
+
+LARGE = [([1024, 1024], 21, 3), ([128, 128, 128], 22, 5), ([2048, 2048], 23, 8), ([256, 256, 256], 24, 2),
+         ([64, 64], 25, 1)]
+
+
+def test_served_large_shapes(ctx):
+    """The configuration's largest grids (1024^2, 2048^2, 128^3, 256^3) served in one mix, some
+    arriving while others run: every result meets 1e-8 with the residual recomputed by the
+    oracle's operator, and the 256^3 request's result equals the one it gets served alone."""
+    outs, res = _serve(ctx, LARGE[:2], late=LARGE[2:])
+    assert len(res) == len(LARGE)
+    for k, (n, seed, nb) in enumerate(LARGE):
+        r = res[k]
+        assert r.status == 0 and r.state.converged, (n, r.state.relerr)
+        b = of.gen_bumps_grid(_desc(n), seed, nb)
+        rel = np.linalg.norm(b - oracle.poisson_apply(n, [1.0 / n[0]] * len(n), [(D, D)] * len(n), outs[k])) \
+            / np.linalg.norm(b)
+        print(f"{n}: iters {r.state.niter} refine {r.state.nrefine} engine relres {r.state.relerr:.2e} "
+              f"oracle relres {rel:.2e}", flush=True)
+        assert rel <= 1.01e-8
+    alone, _ = _serve(ctx, [LARGE[3]])
+    np.testing.assert_array_equal(alone[0], outs[3])
