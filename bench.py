@@ -84,6 +84,7 @@ def ftcs_field(of, ctx, n, dim, dtype, bc_type, seed, init="uniform"):
     return mesh, u
 
 
+CONFIG_STEPS = 200  # the 3-D configuration's step count (FP32-vs-FP64 accuracy is compared there)
 SETTLE_STEPS = 64  # untimed steps before any timed region: the GPU's clocks take ~20 ms to ramp up
 
 
@@ -102,26 +103,6 @@ def time_ftcs(of, ctx, ev, u, coef, steps, warmup):
     wall = time.perf_counter() - t0
     ms = ev.elapsed_ms(e0, e1)
     return ms, wall
-
-
-def cpu_baseline_ftcs3d(n, steps, nthreads=0):
-    """Oracle (C/OpenMP restatement of OpFlow's FieldAssigner FTCS sequence) on host cores
-    (nthreads = 0: all of them)."""
-    import numpy as np
-    import oracle
-    from tests.helpers import desc as mkdesc
-    d = mkdesc(3, [n + 1] * 3, 1, [(3, 3)] * 3, ext=(1, 1))
-    om = oracle.Mesh([n + 1] * 3)
-    import opflow_amd as of
-    cnt = d.extent[0] * d.extent[1] * d.extent[2]
-    u = of.gen_uniform(0xC0FFEE + 2, cnt, -1.0, 1.0).reshape(d.extent[2], d.extent[1], d.extent[0])
-    u = oracle.update_padding(d, om, u)
-    h = 1.0 / n
-    oracle.ftcs(d, om, u, 0.2 * h * h, 1, nthreads)  # warm-up
-    t0 = time.perf_counter()
-    oracle.ftcs(d, om, u, 0.2 * h * h, steps, nthreads)
-    dt = time.perf_counter() - t0
-    return n ** 3 * steps / dt / 1e6, dt, nthreads or oracle.num_threads()
 
 
 def generic_sweeps(of, ctx, ev, n):
@@ -173,6 +154,31 @@ def generic_sweeps(of, ctx, ev, n):
     return out
 
 
+def cpp_programs(n):
+    """The FTCS time loops written as OpFlow programs (tests/cpp/ftcs2d_example.cpp, ftcs3d_example.cpp)
+    on the C++ front-end, which defers the repeated assignment into fused launches: loop time from
+    the first assignment to the device finishing the last (2-D: until the host read of the result)."""
+    import subprocess
+    bind = os.path.join(ROOT, "opflow_amd", "bin")
+    out = {}
+    for dt_ in ("f64", "f32"):
+        r = subprocess.run([os.path.join(bind, "ftcs3d_example"), str(n), str(CONFIG_STEPS), dt_],
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode == 0:
+            j = json.loads(r.stdout.strip().splitlines()[-1])
+            out[f"cpp_ftcs3d_{n}_{dt_}_{CONFIG_STEPS}steps"] = {"Mcell_updates_per_s": j["Mcell_updates_per_s"],
+                                                                 "s_loop": j["s_loop"]}
+        else:
+            out[f"cpp_ftcs3d_{n}_{dt_}_{CONFIG_STEPS}steps"] = {"error": r.stderr[-300:]}
+    r = subprocess.run([os.path.join(bind, "ftcs2d_example"), "", "256", "1000"], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode == 0:
+        j = json.loads([x for x in r.stdout.splitlines() if x.startswith("{")][0])
+        out["cpp_ftcs2d_256_fp64_1000steps"] = {"ms_loop_and_read": j["s_loop_and_read"] * 1e3,
+                                                "Mcell_updates_per_s": 256 * 256 * 1000 / j["s_loop_and_read"] / 1e6}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +187,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=24,
+                    help="steps of each timed run of the 3-D FTCS CPU baseline (median of 5 runs)")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
                     help="precision of the headline sweep (f64 is the configuration's headline)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -195,6 +202,7 @@ def main():
         sys.exit(2)
 
     import opflow_amd as of
+    from opflow_amd import _lib as L
     ctx = of.Context(0)
     ev = HipEvents()
     n = args.n
@@ -251,21 +259,21 @@ def main():
                 "Mcell_updates_per_s": cells / (ms1 / 1e3) / 1e6, "ms_per_step": ms1, "alg_GBps": gbs,
                 "hbm_frac": gbs / HBM_PEAK_GBS}
             us.close()
-        # accuracy of the configuration: FP32 against FP64 after exactly `steps`
-        # steps from the same input (the FP64 GPU sweep equals the oracle bit for
-        # bit), relative to max|u_FP64|; untimed, fresh fields
+        # accuracy of the configuration: FP32 against FP64 after the configuration's
+        # 200 steps (whatever --steps is) from the same input (the FP64 GPU sweep
+        # equals the oracle bit for bit), relative to max|u_FP64|; untimed, fresh fields
         import numpy as np
         _, a64 = ftcs_field(of, ctx, n, 3, "f64", of.BCType.Periodic, 0xC0FFEE + 2)
-        a64.ftcs(coef, args.steps)
+        a64.ftcs(coef, CONFIG_STEPS)
         r64 = a64.numpy()[1:-1, 1:-1, 1:-1]
         a64.close()
         _, a32 = ftcs_field(of, ctx, n, 3, "f32", of.BCType.Periodic, 0xC0FFEE + 2)
-        a32.ftcs(coef, args.steps)
+        a32.ftcs(coef, CONFIG_STEPS)
         d32 = a32.numpy()[1:-1, 1:-1, 1:-1].astype(np.float64) - r64
         a32.close()
         scale = float(np.max(np.abs(r64)))
         extra["ftcs3d_512_fp32"].update({
-            "steps_compared": args.steps, "max_abs_u_fp64": scale,
+            "steps_compared": CONFIG_STEPS, "max_abs_u_fp64": scale,
             "max_err_rel": float(np.max(np.abs(d32))) / scale,
             "l2_err_rel": float(np.linalg.norm(d32) / np.linalg.norm(r64))})
         del d32, r64
@@ -275,40 +283,64 @@ def main():
         outs = []
         for dt_ in ("f64", "f32"):
             _, us = ftcs_field(of, ctx, n, 3, dt_, of.BCType.Periodic, 0xC0FFEE + 2)
-            us.ftcs(coef_s, args.steps)
+            us.ftcs(coef_s, CONFIG_STEPS)
             outs.append(us.numpy()[1:-1, 1:-1, 1:-1].astype(np.float64))
             us.close()
         d32 = outs[1] - outs[0]
         scale = float(np.max(np.abs(outs[0])))
         extra["ftcs3d_512_stable_control_dt_h2_over_8"] = {
-            "steps_compared": args.steps, "max_abs_u_fp64": scale,
+            "steps_compared": CONFIG_STEPS, "max_abs_u_fp64": scale,
             "fp32_max_err_rel": float(np.max(np.abs(d32))) / scale,
             "fp32_l2_err_rel": float(np.linalg.norm(d32) / np.linalg.norm(outs[0]))}
         del d32, outs
         n2 = 256
         h2 = 1.0 / n2
         mesh2, u2 = ftcs_field(of, ctx, n2, 2, "f64", of.BCType.Dirc, 0xC0FFEE + 1, init="bumps")
+        init2 = (u2.numpy(), L.FieldDesc.from_buffer_copy(u2.desc))
         ms2, _ = time_ftcs(of, ctx, ev, u2, 0.2 * h2 * h2, 1000, 50)
         extra["ftcs2d_256_fp64_1000steps"] = {"Mcell_updates_per_s": n2 * n2 * 1000 / (ms2 / 1e3) / 1e6,
                                               "ms_total": ms2}
         u2.close()
         extra.update(generic_sweeps(of, ctx, ev, n))
-        try:
-            from bench_solver import solver_extra
-            extra.update(solver_extra(of, ctx, ev, cpu=not args.no_cpu))
-        except ImportError:
-            pass
+        extra.update(cpp_programs(n))
+        # 3-D FTCS with Dirichlet sides (blocked, ghosts rebuilt inside each launch)
+        _, ud = ftcs_field(of, ctx, n, 3, "f64", of.BCType.Dirc, 0xC0FFEE + 2)
+        msd, _ = time_ftcs(of, ctx, ev, ud, 0.125 * h * h, 60, 10)
+        ld, kd = ud.ftcs_plan(60)
+        extra["ftcs3d_512_fp64_dirichlet_60steps"] = {
+            "Mcell_updates_per_s": cells * 60 / (msd / 1e3) / 1e6, "ms_per_step": msd / 60, "steps_per_launch": kd,
+            "alg_GBps_per_launch": 16.0 * cells / (msd / 1e3 / ld) / 1e9}
+        ud.close()
+        from bench_solver import solver_extra
+        extra.update(solver_extra(of, ctx, ev, cpu=not args.no_cpu))
 
     cpu = None
     if not args.no_cpu:
-        v_cpu, t_cpu, cores = cpu_baseline_ftcs3d(n, args.cpu_steps)
-        cpu = {"value": v_cpu, "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
+        import bench_cpu
+        import oracle
+        from tests.helpers import desc as mkdesc
+        d3 = mkdesc(3, [n + 1] * 3, 1, [(3, 3)] * 3, ext=(1, 1))
+        om3 = oracle.Mesh([n + 1] * 3)
+        cnt = d3.extent[0] * d3.extent[1] * d3.extent[2]
+        init3 = of.gen_uniform(0xC0FFEE + 2, cnt, -1.0, 1.0).reshape(d3.extent[2], d3.extent[1], d3.extent[0])
+        init3 = oracle.update_padding(d3, om3, init3)
+        b3 = bench_cpu.ftcs3d_512(init3, d3, om3, coef, n=n, steps_all=args.cpu_steps)
+        del init3
+        a = b3["all_threads"]
+        cpu = {"value": a["Mcell_updates_per_s"], "unit": "Mcell-updates/s", "cores": a["threads"], "kind": "port",
                "sample": f"oracle FTCS (OpFlow FieldAssigner sequence, C/OpenMP) on the same 512^3 FP64 periodic "
-                         f"input, {args.cpu_steps} steps in {t_cpu:.1f} s on {cores} threads of {cpu_model()}"}
-        if not args.no_extra:  # the same sample on one thread, 4 steps
-            v1, t1, _ = cpu_baseline_ftcs3d(n, 4, 1)
-            extra["cpu_baseline_1_thread"] = {"value": v1, "unit": "Mcell-updates/s", "cores": 1,
-                                              "sample": f"4 steps in {t1:.1f} s"}
+                         f"input, {args.cpu_steps} steps per run, median of {a['runs']} runs after a warm-up run "
+                         f"({a['s_per_run']:.2f} s per run) on {a['threads']} threads of {cpu_model()}"}
+        if not args.no_extra and args.dtype == "f64":
+            cb = extra.setdefault("cpu_baselines", {})
+            cb["cpu_model"] = cpu_model()
+            cb["ftcs3d_512_fp64"] = b3
+            d2 = init2[1]
+            cb["ftcs2d_256_fp64_1000steps"] = bench_cpu.ftcs2d_256(init2[0], d2, oracle.Mesh([n2 + 1] * 2),
+                                                                   0.2 * h2 * h2)
+            cb["lid_driven_1024sq_per_step"] = bench_cpu.lid_driven(of, ctx)
+            from bench_solver import _requests
+            cb["served_poisson_one_at_a_time"] = bench_cpu.served(of, _requests(512, 0xC0FFEE + 5)[:16])
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "ftcs3d_512_fp64_traffic.json")

@@ -49,16 +49,25 @@ def cfg3(of, ctx, ev, cpu=False):
                       "max_err_vs_fp64": err},
             "mixed_speedup": t64 / t32,
         }
+        N = int(np.prod(n))
+        # algorithmic bytes of one PCG iteration at maximal fusion (1 pre + 1 post smoothing):
+        # 24.3 s N in 2-D, 22.4 s N in 3-D (s = bytes per value); mixed: FP32 iterations plus
+        # 6 FP64 vectors per refinement (true residual + update)
+        c = 24.3 if dim == 2 else 22.4
+        b64 = c * 8 * N * s64.niter
+        b32 = c * 4 * N * s32.niter + 6 * 8 * N * s32.nrefine
+        ent = out[f"poisson_{key}_relres1e-10"]
+        for tag, byt, t in (("fp64", b64, t64), ("mixed", b32, t32)):
+            ent[tag]["roofline"] = {"bound": "hbm", "achieved": byt / t / 1e9, "peak": 8000.0, "unit": "GB/s",
+                                    "frac": byt / t / 1e9 / 8000.0, "alg_bytes_per_solve": byt}
         if cpu:
-            # reported CPU baseline: the oracle's C/OpenMP restatement of the reference's
-            # CPU solver (flexible PCG + V(1,1) weighted-Jacobi multigrid, PFMG defaults)
-            import oracle
-            t0 = time.perf_counter()
-            xc, itc, relc = oracle.pcg_mg(n, [1.0 / n[0]] * dim, [(1, 1)] * dim, b, tol=1e-10, maxit=400)
-            tc = time.perf_counter() - t0
-            out[f"poisson_{key}_relres1e-10"]["cpu_baseline"] = {
-                "s_per_solve": tc, "iters": itc, "relres": relc, "cores": oracle.num_threads(), "kind": "port",
-                "max_err_vs_gpu_fp64": float(np.max(np.abs(xc - x64)) / np.max(np.abs(x64)))}
+            # reported CPU baseline: the oracle's C/OpenMP restatement of the reference's CPU
+            # solver (flexible PCG + V(1,1) weighted-Jacobi multigrid, PFMG defaults), median
+            # of several runs at all threads, and a smaller sample at one thread
+            import bench_cpu
+            ent["cpu_baseline"], xc = bench_cpu.poisson_one(of, n, b)
+            ent["cpu_baseline"].update({"kind": "port",
+                                        "max_err_vs_gpu_fp64": float(np.max(np.abs(xc - x64)) / np.max(np.abs(x64)))})
     return out
 
 
@@ -148,11 +157,19 @@ class _Fp64Server:
             g.close(); bb.close(); xx.close()
 
 
+def _alg_bytes(n, st):
+    """Algorithmic bytes of one mixed-precision solve: per PCG iteration 24.3 s N (2-D) or
+    22.4 s N (3-D) at maximal fusion with s = 4 B, plus 6 FP64 vectors per refinement."""
+    N = int(np.prod(n))
+    c = 24.3 if len(n) == 2 else 22.4
+    return c * 4 * N * st.niter + 6 * 8 * N * st.nrefine
+
+
 def _serve_run(of, srv, reqs, tol, arrivals=None):
     """Submit reqs (all at once, or at their arrival times on the server clock) and run rounds until done."""
     t0 = time.perf_counter()
     pending = list(range(len(reqs)))
-    done, lat = 0, []
+    done, lat, byt, ids = 0, [], 0.0, {}
     base = srv.clock()
     while done < len(reqs):
         now = srv.clock() - base
@@ -160,7 +177,7 @@ def _serve_run(of, srv, reqs, tol, arrivals=None):
             k = pending.pop(0)
             n, seed, nb = reqs[k]
             t_arr = -1.0 if arrivals is None else base + arrivals[k]
-            srv.submit(_desc(of, n), seed, nb, t_submit=t_arr)
+            ids[srv.submit(_desc(of, n), seed, nb, t_submit=t_arr)] = k
         if srv.pending() == (0, 0) and pending:
             time.sleep(max(0.0, min(0.002, arrivals[pending[0]] - now)))
             continue
@@ -169,10 +186,11 @@ def _serve_run(of, srv, reqs, tol, arrivals=None):
             done += 1
             lat.append(r.t_done - r.t_submit)
             assert r.state.converged
-    return time.perf_counter() - t0, lat
+            byt += _alg_bytes(reqs[ids[r.id]][0], r.state)
+    return time.perf_counter() - t0, lat, byt
 
 
-def cfg5(of, ctx, ev, count=48):
+def cfg5(of, ctx, ev, count=512):
     tol = 1e-8
     reqs = _requests(count, 0xC0FFEE + 5)
     shapes = sorted({tuple(n) for n, _, _ in reqs})
@@ -184,13 +202,13 @@ def cfg5(of, ctx, ev, count=48):
     # else carries over: every request regenerates its RHS and starts from x = 0)
     _serve_run(of, srv, reqs, tol)
     base.run(reqs, tol)
-    t_srv, _ = _serve_run(of, srv, reqs, tol)
+    t_srv, _, byt_srv = _serve_run(of, srv, reqs, tol)
     t_64, _ = base.run(reqs, tol)
     # open loop: Poisson arrivals at a rate that keeps the FP64 one-at-a-time server at ~85% utilisation
     lam = 0.85 * count / t_64
     rng = np.random.default_rng(0xC0FFEE + 55)
     arr = np.cumsum(rng.exponential(1.0 / lam, count))
-    _, lat_srv = _serve_run(of, srv, reqs, tol, arrivals=arr)
+    _, lat_srv, _ = _serve_run(of, srv, reqs, tol, arrivals=arr)
     _, lat_64 = base.run(reqs, tol, arrivals=arr)
     srv.close()
     base.close()
@@ -199,7 +217,11 @@ def cfg5(of, ctx, ev, count=48):
         "requests": count, "shapes": [list(n) for n in shapes], "arrival_rate_per_s": lam,
         "server": {"mem_budget_GiB": 64, "max_inflight": 64, "warm_up": "one untimed pass of the same request list (both servers)"},
         "rhs": "seeded bumps generated on the device at admission (both servers)",
-        "served_mixed_batched": {"solves_per_s": count / t_srv, "p50_s": pct(lat_srv, 50), "p99_s": pct(lat_srv, 99)},
+        "served_mixed_batched": {"solves_per_s": count / t_srv, "p50_s": pct(lat_srv, 50), "p99_s": pct(lat_srv, 99),
+                                 "roofline": {"bound": "hbm", "achieved": byt_srv / t_srv / 1e9, "peak": 8000.0,
+                                              "unit": "GB/s", "frac": byt_srv / t_srv / 1e9 / 8000.0,
+                                              "note": "closed loop; algorithmic bytes of each request's FP32 "
+                                                      "iterations and FP64 refinements"}},
         "fp64_one_at_a_time": {"solves_per_s": count / t_64, "p50_s": pct(lat_64, 50), "p99_s": pct(lat_64, 99)},
         "throughput_ratio": t_64 / t_srv}}
 

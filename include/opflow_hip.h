@@ -104,8 +104,33 @@ enum {
     OPF_N_WENO_DOWN = 12, /* D1WENO53Downwind                 D1WENO53Downwind.hpp:75-154 */
     OPF_N_INTP_C2N = 13,  /* D1Linear<dim, Cen2Cor>           Interpolator/D1Linear.hpp:34-42 */
     OPF_N_INTP_N2C = 14,  /* D1Linear<dim, Cor2Cen>           Interpolator/D1Linear.hpp:43-45 */
-    OPF_N_UNKNOWN = 15    /* the StencilField of the solve target (StencilField.hpp:123-350);
+    OPF_N_UNKNOWN = 15,   /* the StencilField of the solve target (StencilField.hpp:123-350);
                              a = the target's index in an equation set (0 otherwise) */
+    /* D1FluxLimiterGen<Kernel>::Op<dim, dir>(a, b) (Interpolator/D1FluxLimiter.hpp:43-202): b
+     * interpolated to the faces in `dim`, upwinded by the sign of a (a > 0: upwind form, else
+     * downwind form); Cen2Cor when b is centre-located in `dim`, Cor2Cen when corner-located;
+     * value = the limiter kernel OPF_LIM_* (FluxLimiterKernels.hpp) + 32 x the direction the
+     * caller names (0: from b's location, 1: Cen2Cor, 2: Cor2Cen; a mismatch is OPF_ERR_ARG,
+     * as the reference's prepare() asserts)                                          */
+    OPF_N_FLUXLIM = 16,
+    OPF_N_MIN = 17,       /* std::min(a, b) = b < a ? b : a                          */
+    OPF_N_MAX = 18,       /* std::max(a, b) = a < b ? b : a                          */
+    OPF_N_ABS = 19,       /* std::fabs(a)                                            */
+    OPF_N_GT = 20,        /* a > b as 1 / 0 (the condition of conditional)           */
+    OPF_N_LT = 21,        /* a < b                                                   */
+    OPF_N_GE = 22,        /* a >= b                                                  */
+    OPF_N_LE = 23,        /* a <= b                                                  */
+    /* conditional(c, t, f) (Operator/Conditional.hpp:22-60): a = c, b = t, dim = the node index
+     * of f; c != 0 selects t                                                          */
+    OPF_N_COND = 24
+};
+/* limiter kernels of OPF_N_FLUXLIM (Interpolator/FluxLimiterKernels.hpp); the kappa schemes use
+ * the linear form of D1FluxLimiter.hpp, the others the ratio form                    */
+enum {
+    OPF_LIM_CDS = 0, OPF_LIM_QUICK = 1, OPF_LIM_CUI = 2, OPF_LIM_FROMM = 3, OPF_LIM_LUI = 4,
+    OPF_LIM_MINMOD = 5, OPF_LIM_SUPERBEE = 6, OPF_LIM_MUSCL = 7, OPF_LIM_HARMONIC = 8, OPF_LIM_VANALBADA = 9,
+    OPF_LIM_UMIST = 10, OPF_LIM_SPL13 = 11, OPF_LIM_SPLMAX12 = 12, OPF_LIM_SPLMAX13 = 13,
+    OPF_LIM_KOREN = 14, OPF_LIM_SMART = 15, OPF_LIM_BSOU = 16
 };
 typedef struct {
     int kind;
@@ -135,6 +160,16 @@ int opf_ctx_set_stream(opf_ctx* ctx, void* stream);
  * process and under $OPF_JIT_CACHE or ~/.cache/opflow_amd/jit) for sweeps of at
  * least 2^18 cells, 2 = that kernel for every sweep. Both give the same bits. */
 int opf_ctx_set_jit(opf_ctx* ctx, int mode);
+/* Counters of the engine's caches (no counterpart in the reference):
+ * stats[0] hipRTC compilations in this process, [1] compiled sweep kernels
+ * loaded in this process, [2] of them loaded now (at most 512 per device,
+ * least recently used unloaded first), [3] sweep/equation programs this context
+ * uploaded, [4] of them resident now (sweep programs: at most 1024, least
+ * recently used freed first). Scalar constants of an expression are launch
+ * arguments, not part of a program or kernel, so a sweep repeated with a new
+ * constant (an adaptive dt) reuses both. A HIP graph that captured a sweep must
+ * not be replayed after its program or kernel was evicted.                  */
+int opf_cache_stats(opf_ctx* ctx, long long* stats);
 const char* opf_last_error(void);
 const char* opf_version(void);
 
@@ -192,7 +227,7 @@ int opf_ftcs(opf_field* u, double coef, int nsteps);
 /* Same update with the temporal-blocking choice made by the caller:
  * tblock = 1 runs one launch per step; tblock = 0 lets the library fuse steps
  * inside one launch where it can (2-D: 4 steps per launch, 5 on small grids, any BC;
- * 3-D: 3 steps per launch when periodic in every dimension, one otherwise),
+ * 3-D: 3 steps per launch, the ghosts of non-periodic sides rebuilt inside the launch),
  * which is what opf_ftcs does. Results are bit-identical either way.       */
 int opf_ftcs_blocked(opf_field* u, double coef, int nsteps, int tblock);
 /* Launch plan of opf_ftcs_blocked(u, ., nsteps, tblock) without running it:

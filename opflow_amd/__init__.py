@@ -9,6 +9,7 @@ for OpFlow's `field = expr`, rangeReduce, Solve/makeEqnSolveHandler and
 StructSolverParams. Every compute call goes to the HIP library; there is no
 CPU path in this package.
 """
+import builtins
 import ctypes as C
 import enum
 
@@ -87,6 +88,14 @@ class Context:
         """Engine of expression sweeps: 0 interpreter, 1 compiled kernels for >= 2^18 cells
         (default), 2 compiled kernels always (same bits either way)."""
         L.check(self.lib.opf_ctx_set_jit(self.h, int(mode)))
+
+    def cache_stats(self):
+        """Counters of the sweep caches (opf_cache_stats): hipRTC compiles and compiled kernels
+        loaded in the process, kernels loaded now, programs this context uploaded, resident now."""
+        a = (C.c_longlong * 5)()
+        L.check(self.lib.opf_cache_stats(self.h, a))
+        keys = ("jit_compiles", "jit_loaded", "jit_resident", "programs_uploaded", "programs_resident")
+        return dict(zip(keys, list(a)))
 
     @property
     def stream(self):
@@ -176,6 +185,7 @@ class MeshBuilder:
 _N_FIELD, _N_CONST, _N_ADD, _N_SUB, _N_MUL, _N_DIV, _N_NEG = 0, 1, 2, 3, 4, 5, 6
 _N_D2C, _N_D1C, _N_D1UP, _N_D1DOWN, _N_WENO_UP, _N_WENO_DOWN = 7, 8, 9, 10, 11, 12
 _N_INTP_C2N, _N_INTP_N2C, _N_UNKNOWN = 13, 14, 15
+_N_FLUXLIM, _N_MIN, _N_MAX, _N_ABS, _N_GT, _N_LT, _N_GE, _N_LE, _N_COND = 16, 17, 18, 19, 20, 21, 22, 23, 24
 
 
 def _wrap(v):
@@ -189,8 +199,9 @@ def _wrap(v):
 class Expr:
     """Lazy expression node (Expr/Expression.hpp); lowered by the library."""
 
-    def __init__(self, kind, a=None, b=None, dim=0, value=0.0, field=None):
+    def __init__(self, kind, a=None, b=None, dim=0, value=0.0, field=None, c=None):
         self.kind, self.a, self.b, self.dim, self.value, self.field = kind, a, b, dim, value, field
+        self.c = c  # conditional: the value selected when the condition fails
 
     def __add__(self, o): return Expr(_N_ADD, self, _wrap(o))
     def __radd__(self, o): return Expr(_N_ADD, _wrap(o), self)
@@ -201,6 +212,11 @@ class Expr:
     def __truediv__(self, o): return Expr(_N_DIV, self, _wrap(o))
     def __rtruediv__(self, o): return Expr(_N_DIV, _wrap(o), self)
     def __neg__(self): return Expr(_N_NEG, self)
+    # comparisons (Operator/Logical/Compare.hpp) for conditional()
+    def __gt__(self, o): return Expr(_N_GT, self, _wrap(o))
+    def __lt__(self, o): return Expr(_N_LT, self, _wrap(o))
+    def __ge__(self, o): return Expr(_N_GE, self, _wrap(o))
+    def __le__(self, o): return Expr(_N_LE, self, _wrap(o))
 
     def __eq__(self, o):  # Equation: lhs == rhs (Equation/Equation.hpp:27-55)
         return Equation(self, _wrap(o))
@@ -230,7 +246,8 @@ class Expr:
             else:
                 ia = visit(e.a)
                 ib = visit(e.b) if e.b is not None else 0
-                node = (e.kind, ia, ib, e.dim, 0.0)
+                dim = visit(e.c) if e.kind == _N_COND else e.dim
+                node = (e.kind, ia, ib, dim, float(e.value) if e.kind == _N_FLUXLIM else 0.0)
             nodes.append(node)
             memo[id(e)] = len(nodes) - 1
             return memo[id(e)]
@@ -309,12 +326,58 @@ def dy(kernel, e): return d1(kernel, 1, e)
 def dz(kernel, e): return d1(kernel, 2, e)
 
 
-def d1IntpCenterToCorner(d, e):  # IntpInterface.hpp:26-34, D1Linear<d, Cen2Cor>
-    return Expr(_N_INTP_C2N, _wrap(e), dim=d)
+class IntpDirection:  # Interpolator/IntpInterface.hpp:20
+    Cor2Cen, Cen2Cor = 0, 1
 
 
-def d1IntpCornerToCenter(d, e):  # D1Linear<d, Cor2Cen>
-    return Expr(_N_INTP_N2C, _wrap(e), dim=d)
+class D1FluxLimiterGen:
+    """D1FluxLimiterGen<Kernel>::Op<d, dir> (Interpolator/D1FluxLimiter.hpp:205-209): the kernel
+    id of FluxLimiterKernels.hpp; the interpolation functions below take it as `kernel`."""
+
+    def __init__(self, kernel_id):
+        self.id = kernel_id
+
+
+(D1Central, D1QUICK, D1CUI, D1Fromm, D1LinearUpwind, D1Minmod, D1Superbee, D1MUSCL, D1Harmonic, D1Albada,
+ D1UMIST, D1SPL13, D1SPLMax12, D1SPLMax13, D1KorenLimitedCUI, D1SMART, D1BSOU) = [D1FluxLimiterGen(k) for k in range(17)]
+
+
+def _fluxlim(d, u, e, kernel, direction):
+    # value: kernel id + 32 x the named direction (1 Cen2Cor, 2 Cor2Cen), checked by the library
+    return Expr(_N_FLUXLIM, _wrap(u), _wrap(e), dim=d,
+                value=float(kernel.id + 32 * (1 if direction == IntpDirection.Cen2Cor else 2)))
+
+
+def d1IntpCenterToCorner(d, *args, kernel=None):
+    """IntpInterface.hpp:26-34: d1IntpCenterToCorner(d, e) is D1Linear<d, Cen2Cor>(e);
+    d1IntpCenterToCorner(d, u, e, kernel=D1QUICK) the flux-limiter interpolation of e
+    upwinded by u (D1FluxLimiter.hpp)."""
+    if kernel is None:
+        (e,) = args
+        return Expr(_N_INTP_C2N, _wrap(e), dim=d)
+    u, e = args
+    return _fluxlim(d, u, e, kernel, IntpDirection.Cen2Cor)
+
+
+def d1IntpCornerToCenter(d, *args, kernel=None):  # D1Linear<d, Cor2Cen>, or the limiter form
+    if kernel is None:
+        (e,) = args
+        return Expr(_N_INTP_N2C, _wrap(e), dim=d)
+    u, e = args
+    return _fluxlim(d, u, e, kernel, IntpDirection.Cor2Cen)
+
+
+def conditional(c, t, f):
+    """conditional(c, t, f) (Operator/Conditional.hpp:181): t where c holds, else f."""
+    return Expr(_N_COND, _wrap(c), _wrap(t), c=_wrap(f))
+
+
+def max(a, b):  # noqa: A001  OpFlow::max (Operator/Arithmetic/MinMax.hpp): std::max elementwise
+    return Expr(_N_MAX, _wrap(a), _wrap(b))
+
+
+def min(a, b):  # noqa: A001  OpFlow::min: std::min elementwise
+    return Expr(_N_MIN, _wrap(a), _wrap(b))
 
 
 # ---------------------------------------------------------------- fields
@@ -362,6 +425,12 @@ class CartesianField(Expr):
         a = np.ascontiguousarray(arr, dtype=self.dtype).reshape(self.shape)
         L.check(self.lib.opf_field_upload(self.h, a.ctypes.data, a.nbytes))
 
+    def __getitem__(self, idx):
+        """field[i] inside a rangeReduce functor: the field's value expression."""
+        if isinstance(idx, SymIndex):
+            return self
+        raise TypeError("field[i] takes the symbolic index of rangeReduce; read values with numpy()")
+
     def index(self, *idx):
         """storage tuple (numpy index) of a field index (i0, i1, ...)."""
         return tuple(idx[d] - self.desc.offset[d] for d in reversed(range(self.desc.dim)))
@@ -376,7 +445,7 @@ class CartesianField(Expr):
             return self
         nodes, fields, root = _wrap(rhs).flatten()
         arr = _node_array(nodes)
-        fh = (C.c_void_p * max(1, len(fields)))(*[f.h.value for f in fields])
+        fh = (C.c_void_p * builtins.max(1, len(fields)))(*[f.h.value for f in fields])
         L.check(self.lib.opf_sweep(arr, len(nodes), root, fh, len(fields), self.h, int(op)))
         return self
 
@@ -394,7 +463,7 @@ class CartesianField(Expr):
         d = self.desc
         out = []
         for q in range(d.dim):
-            s_, e_ = max(d.assign_start[q], d.local_start[q]), min(d.assign_end[q], d.local_end[q])
+            s_, e_ = builtins.max(d.assign_start[q], d.local_start[q]), builtins.min(d.assign_end[q], d.local_end[q])
             idx = np.arange(s_, e_)
             x = self.mesh.xs[q][idx + self.mesh.pad]
             if d.loc[q] == LocOnMesh.Center:
@@ -509,9 +578,51 @@ class ExprBuilder:
 _RED = {"sum": 0, "max": 1, "min": 2, "sumsq": 3, "maxabs": 4}
 
 
-def rangeReduce(field, red, rng=None):
-    """rangeReduce(range, op, f) for op in {sum,max,min,sumsq,maxabs} (RangeFor.hpp:86-121)."""
-    rng = rng or field.assignableRange
+class SymIndex:
+    """The index a rangeReduce functor receives (RangeFor.hpp:86-121 passes each MDIndex of the
+    range): a symbolic one, so `f[idx]` is the field's value expression and the functor's
+    result is reduced on the device."""
+
+
+def _reduction_kind(op):
+    if isinstance(op, str):
+        return op
+    f = lambda a, b: float(op(a, b))  # noqa: E731
+    if f(1.0, 2.0) == 3.0 and f(-4.0, 7.5) == 3.5 and f(0.25, 0.5) == 0.75:
+        return "sum"
+    if f(1.0, 2.0) == 2.0 and f(7.5, -4.0) == 7.5 and f(-3.0, -2.0) == -2.0:
+        return "max"
+    if f(1.0, 2.0) == 1.0 and f(7.5, -4.0) == -4.0 and f(-3.0, -2.0) == -3.0:
+        return "min"
+    raise L.OpfError(L.OPF_ERR_UNSUPPORTED, "rangeReduce: the reduction operator is not a sum, max or min")
+
+
+def rangeReduce(a, b, c=None):
+    """rangeReduce(range, op, func) (RangeFor.hpp:86-121), e.g.
+        rangeReduce(p.assignableRange, lambda x, y: x + y, lambda i: p[i])
+    op: a binary operator (sum, max or min, told from its values) or one of
+    "sum", "max", "min", "sumsq", "maxabs"; func receives a symbolic index and returns a
+    field or an expression of fields (evaluated into a temporary with the layout of the
+    first field it reads). Deterministic on the device.
+    Also rangeReduce(field, op[, range]) (range defaults to the assignable range)."""
+    if isinstance(a, CartesianField):
+        return _reduce_field(a, _reduction_kind(b), c or a.assignableRange)
+    rng, red = a, _reduction_kind(b)
+    e = _wrap(c(SymIndex()))
+    if isinstance(e, CartesianField):
+        return _reduce_field(e, red, rng)
+    nodes, fields, _ = e.flatten()
+    if not fields:
+        raise L.OpfError(L.OPF_ERR_ARG, "rangeReduce: the functor reads no field")
+    tmp = CartesianField(fields[0].mesh, fields[0].desc, "reduce_tmp")
+    try:
+        tmp.assign(e)
+        return _reduce_field(tmp, red, rng)
+    finally:
+        tmp.close()
+
+
+def _reduce_field(field, red, rng):
     dim = field.desc.dim
     rs = (C.c_int * 3)(*([r[0] for r in rng] + [0] * (3 - dim)))
     re_ = (C.c_int * 3)(*([r[1] for r in rng] + [1] * (3 - dim)))

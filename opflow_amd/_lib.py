@@ -126,6 +126,7 @@ SIGNATURES = {
     "opf_ftcs_blocked": (_I, [_P, _D, _I, _I]),
     "opf_ftcs_plan": (_I, [_P, _I, _I, _IP, _IP]),
     "opf_ftcs_noalloc": (_I, [_P, _D, _I]),
+    "opf_cache_stats": (_I, [_P, C.POINTER(C.c_longlong)]),
     "opf_reduce": (_I, [_P, _I, _IP, _IP, _DP]),
     "opf_buf_create": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "opf_buf_destroy": (_I, [_P, _P]),

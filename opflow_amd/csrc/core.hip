@@ -136,18 +136,34 @@ static uint64_t fnv1a(const void* p, size_t n) {
     return h ^ (uint64_t)n;
 }
 
-int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev) {
+int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev, bool evictable) {
     uint64_t h = fnv1a(bytes, n);
     for (auto& p : ctx->progs)
-        if (p.hash == h && p.bytes == n) {
+        if (p.hash == h && p.bytes == n && p.evictable == evictable && std::memcmp(p.host.data(), bytes, n) == 0) {
+            p.used = ++ctx->prog_tick;
             *dev = p.dev;
             return OPF_OK;
         }
+    if (evictable) {
+        size_t ne = 0;
+        for (auto& p : ctx->progs) ne += p.evictable;
+        if (ne >= opf_ctx::kMaxPrograms) {
+            // least recently used first, after the context's queued work has drained
+            auto victim = ctx->progs.end();
+            for (auto it = ctx->progs.begin(); it != ctx->progs.end(); ++it)
+                if (it->evictable && (victim == ctx->progs.end() || it->used < victim->used)) victim = it;
+            OPF_HIP(hipStreamSynchronize(ctx->stream));
+            OPF_HIP(hipFree(victim->dev));
+            ctx->progs.erase(victim);
+        }
+    }
     void* d = nullptr;
     OPF_HIP(hipMalloc(&d, n));
     OPF_HIP(hipMemcpyAsync(d, bytes, n, hipMemcpyHostToDevice, ctx->stream));
     OPF_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->progs.push_back({h, d, n});
+    const unsigned char* b = (const unsigned char*)bytes;
+    ctx->progs.push_back({h, d, n, ++ctx->prog_tick, evictable, std::vector<unsigned char>(b, b + n)});
+    ++ctx->prog_uploads;
     *dev = d;
     return OPF_OK;
 }

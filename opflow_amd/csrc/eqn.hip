@@ -24,7 +24,7 @@ namespace opf {
 
 constexpr int PAD_K = 7;       // centre + 2 per dim
 constexpr int PAD_W = PAD_K + 2;  // coefficients, bias, linear flag
-constexpr int PAD_BLOCK = 32;  // 16-deep stack of 9 doubles per lane: 36 KB of LDS
+constexpr int PAD_BLOCK = 32;  // 24-deep stack of 9 doubles per lane: 54 KB of LDS
 constexpr int EQS_MAX = 16;    // unknowns of an equation set
 
 // boundary handling of one unknown: periodic dims and the ghost rule's affine constant
@@ -148,9 +148,19 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
             case I_NEG:
                 for (int q = 0; q <= PAD_K; ++q) t[q] = -t[q];
                 break;
-            default:  // WENO on linear forms is nonlinear
+            case I_WENO_UP:
+            case I_WENO_DOWN:  // WENO on linear forms is nonlinear
                 bad |= 2;
                 sp -= 6;
+                break;
+            case I_SEL:  // selections and comparisons are not linear forms either
+                bad |= 2;
+                sp -= 2;
+                break;
+            case I_ABS: bad |= 2; break;
+            default:  // I_MIN, I_MAX and the comparisons: binary
+                bad |= 2;
+                --sp;
                 break;
         }
     }

@@ -8,6 +8,7 @@
 // by 1/h (same bits).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fastdiv.hpp"
 #include "opf_internal.hpp"
@@ -27,16 +28,47 @@ struct FtcsArgs {
     int n[3];       // interior cells
     int e0, e1;     // storage extents (n+2)
     T coef;
-    T hh[3];        // 1/h (pow2 mode) or h (division mode)
+    T coefh;        // coef / h^2 (FTCS_FACTORED: exact, h a power of two)
+    T hh[3];        // 1/h (FTCS_MUL, FTCS_FACTORED) or h (FTCS_DIV)
     int per[3];
     int rtype[3][2];  // ghost rule per dim/side: 1 dirichlet, 2 neumann
     T rc1[3][2], rc2[3][2], rc3[3][2];
 };
 
-template <typename T, bool POW2>
+// Arithmetic modes of the update (all give the reference's bits):
+//  FTCS_DIV      D2 = ((r-c)/h - (c-l)/h)/h per dimension, as D2SecondOrderCentered computes it;
+//  FTCS_MUL      h a power of two: the divisions become multiplications by 1/h (exact);
+//  FTCS_FACTORED h a power of two and equal in every dimension. Scaling by a power of two
+//                commutes with rounding (no overflow: |u| h^-2 below the largest finite
+//                value), so ((r-c)*k - (c-l)*k)*k = ((r-c) - (c-l)) * k^2 exactly, the sum
+//                over dimensions keeps the common factor k^2, and coef * (k^2 S) =
+//                (coef k^2) * S with coef k^2 exact: u + coefh * (Dx + Dy [+ Dz]) with
+//                D = (r-c) - (c-l) -- 10 operations per 3-D cell instead of 19.
+enum { FTCS_DIV = 0, FTCS_MUL = 1, FTCS_FACTORED = 2 };
+
+template <typename T, int MODE>
 __device__ __forceinline__ T d2(T l, T c, T r, T hh) {
-    if constexpr (POW2) return ((r - c) * hh - (c - l) * hh) * hh;
+    if constexpr (MODE != FTCS_DIV) return ((r - c) * hh - (c - l) * hh) * hh;
     else return ((r - c) / hh - (c - l) / hh) / hh;
+}
+
+// u^{n+1} at one cell from its value and its x / y (/ z) neighbours
+template <typename T, int MODE>
+__device__ __forceinline__ T ftcs2(T c, T xl, T xr, T yl, T yr, const FtcsArgs<T>& a) {
+    if constexpr (MODE == FTCS_FACTORED) return c + a.coefh * (((xr - c) - (c - xl)) + ((yr - c) - (c - yl)));
+    else return c + a.coef * (d2<T, MODE>(xl, c, xr, a.hh[0]) + d2<T, MODE>(yl, c, yr, a.hh[1]));
+}
+template <typename T, int MODE>
+__device__ __forceinline__ T ftcs3(T c, T xl, T xr, T yl, T yr, T zl, T zr, const FtcsArgs<T>& a) {
+    if constexpr (MODE == FTCS_FACTORED) {
+        T s = ((xr - c) - (c - xl)) + ((yr - c) - (c - yl));
+        s = s + ((zr - c) - (c - zl));
+        return c + a.coefh * s;
+    } else {
+        T s = d2<T, MODE>(xl, c, xr, a.hh[0]) + d2<T, MODE>(yl, c, yr, a.hh[1]);
+        s = s + d2<T, MODE>(zl, c, zr, a.hh[2]);
+        return c + a.coef * s;
+    }
 }
 
 template <typename T>
@@ -85,15 +117,14 @@ __device__ __forceinline__ void write_images(T* __restrict__ out, const FtcsArgs
 }
 
 // 2-D: one cell per thread, block 64 x 4.
-template <typename T, bool POW2>
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void ftcs2d_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a) {
     int i = blockIdx.x * 64 + threadIdx.x;
     int j = blockIdx.y * 4 + threadIdx.y;
     if (i >= a.n[0] || j >= a.n[1]) return;
     long long l = (i + 1) + (long long)a.e0 * (j + 1);
     T c = in[l];
-    T v = c + a.coef * (d2<T, POW2>(in[l - 1], c, in[l + 1], a.hh[0]) +
-                        d2<T, POW2>(in[l - a.e0], c, in[l + a.e0], a.hh[1]));
+    T v = ftcs2<T, MODE>(c, in[l - 1], in[l + 1], in[l - a.e0], in[l + a.e0], a);
     out[l] = v;
     int p[3] = {i, j, 0};
     write_images<T, 2>(out, a, p, v);
@@ -113,7 +144,7 @@ struct Ftcs2dTile {
     static constexpr int W = TS + 2 * TB;
 };
 
-template <typename T, bool POW2, int TS, int TB>
+template <typename T, int MODE, int TS, int TB>
 __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
                                                         int nt) {
     constexpr int W = Ftcs2dTile<TS, TB>::W;
@@ -157,8 +188,8 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
             bool iny = a.per[1] || (gy >= 0 && gy < a.n[1]);
             if (!inx || !iny || lx == 0 || ly == 0 || lx == wx - 1 || ly == wy - 1) continue;
             T c = buf[cur][ly][lx];
-            T v = c + a.coef * (d2<T, POW2>(buf[cur][ly][lx - 1], c, buf[cur][ly][lx + 1], a.hh[0]) +
-                                d2<T, POW2>(buf[cur][ly - 1][lx], c, buf[cur][ly + 1][lx], a.hh[1]));
+            T v = ftcs2<T, MODE>(c, buf[cur][ly][lx - 1], buf[cur][ly][lx + 1], buf[cur][ly - 1][lx],
+                                 buf[cur][ly + 1][lx], a);
             buf[nxt][ly][lx] = v;
         }
         __syncthreads();
@@ -206,7 +237,7 @@ __global__ __launch_bounds__(256) void ftcs2d_tb_kernel(const T* __restrict__ in
 // Launch shape swept at 512^3 on MI355X (64 x 8 threads, 2 rows each, 8 planes).
 constexpr int FT3_TX = 64, FT3_NTY = 8, FT3_RY = 2, FT3_TY = FT3_NTY * FT3_RY, FT3_ZC = 8;
 
-template <typename T, bool POW2, int ZC, int NTY, int RY>
+template <typename T, int MODE, int ZC, int NTY, int RY>
 __global__ __launch_bounds__(64 * NTY) void ftcs3d_lds_kernel(const T* __restrict__ in, T* __restrict__ out,
                                                               FtcsArgs<T> a) {
     constexpr int TY = NTY * RY;
@@ -267,10 +298,8 @@ __global__ __launch_bounds__(64 * NTY) void ftcs3d_lds_kernel(const T* __restric
             const int ly = ty * RY + r + 1, j = y0 + ly - 1;
             if (i >= a.n[0] || j >= a.n[1]) continue;
             const T c = zc[r];
-            T s = d2<T, POW2>(lds[cur][ly][tx], c, lds[cur][ly][tx + 2], a.hh[0]) +
-                  d2<T, POW2>(lds[cur][ly - 1][tx + 1], c, lds[cur][ly + 1][tx + 1], a.hh[1]);
-            s = s + d2<T, POW2>(zm[r], c, zp[r], a.hh[2]);
-            out[at(i, j, k)] = c + a.coef * s;
+            out[at(i, j, k)] = ftcs3<T, MODE>(c, lds[cur][ly][tx], lds[cur][ly][tx + 2], lds[cur][ly - 1][tx + 1],
+                                              lds[cur][ly + 1][tx + 1], zm[r], zp[r], a);
         }
         const int nxt = cur ^ 1;
 #pragma unroll
@@ -287,25 +316,35 @@ __global__ __launch_bounds__(64 * NTY) void ftcs3d_lds_kernel(const T* __restric
     }
 }
 
-// 3-D temporal blocking, K steps per launch, for fields periodic in every
-// dimension. A workgroup owns a TX x TY column of the x-y plane and the z chunk
-// [z0, z0 + zc) and streams the planes of u^n with a wrapped K-cell halo in x
-// and y. Every thread owns fixed positions of the (TX+2K) x (TY+2K) region and
-// keeps, for each, the last three planes of levels 0 .. K-1 (u^n .. u^{n+K-1})
-// in registers, so the centre and the z neighbours never go through LDS; LDS
-// holds only the current plane of each level (double-buffered), read for the x
-// and y neighbours. Step zz stores level-0 plane zz and computes level L at
-// plane zz - L (L = 1 .. K) on the region shrunk by L cells; level K is exactly
-// the tile and goes straight to global memory. One barrier per step separates
-// a step's LDS writes from the next step's reads (levels read the plane the
-// level below wrote one step earlier). Level-0 planes are loaded three planes
-// ahead; the unroll by 3 keeps the history slots static. Reads wrap to interior
-// cells, so the input's ghosts are never read; the output's ghosts are left
-// stale and refilled by the host loop before anything reads them (a
-// branch-free load path keeps the load waits counted). Per cell and step the
+// 3-D temporal blocking, K steps per launch. A workgroup owns a TX x TY column
+// of the x-y plane and the z chunk [z0, z0 + zc) and streams the planes of u^n
+// with a K-cell halo in x and y. Every thread owns fixed positions of the
+// (TX+2K) x (TY+2K) region and keeps, for each, the last three planes of
+// levels 0 .. K-1 (u^n .. u^{n+K-1}) in registers, so the centre and the z
+// neighbours never go through LDS; LDS holds only the current plane of each
+// level (double-buffered), read for the x and y neighbours. Step zz stores
+// level-0 plane zz and computes level L at plane zz - L (L = 1 .. K) on the
+// region shrunk by L cells; level K is exactly the tile and goes straight to
+// global memory. One barrier per step separates a step's LDS writes from the
+// next step's reads (levels read the plane the level below wrote one step
+// earlier). Level-0 planes are loaded three planes ahead; the unroll by 3 keeps
+// the history slots static; every load is unconditional so the compiler keeps
+// counted load waits.
+// Periodic dimensions wrap the halo to interior cells (the input's ghosts are
+// never read). Dirichlet / Neumann dimensions (ALLPER = false) clamp the halo to
+// the ghost layer: level 0 reads the input's ghosts (refilled by the host before
+// the launch), and the ghosts of the intermediate levels 1 .. K-1 are rebuilt
+// from the level's fresh boundary cells with the BC rules, as updatePadding
+// does after every step: x / y face ghosts in LDS at the start of the next step
+// (one extra barrier, only in workgroups whose region reaches such a side), z
+// ghost planes in the register history when the level computes plane 0 or
+// reaches plane n. The 7-point stencil reads face ghosts only, so edge and
+// corner ghosts are never needed; region cells beyond the ghost layer hold
+// values that never reach a valid cell. The output's ghosts are left stale and
+// refilled by the host loop before anything reads them. Per cell and step the
 // arithmetic is that of ftcs3d_lds_kernel, so the result has the same bits as K
 // single steps.
-template <typename T, bool POW2, int TX, int TY, int NT, int K, bool XCD>
+template <typename T, int MODE, int TX, int TY, int NT, int K, bool XCD, bool ALLPER>
 __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in, T* __restrict__ out, FtcsArgs<T> a,
                                                        int zc) {
     static_assert(K >= 2, "one step per launch is ftcs3d_lds_kernel");
@@ -322,8 +361,9 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     const int z1 = min(z0 + zc, a.n[2]);
     const long long sz = (long long)a.e0 * a.e1;
     const int tid = threadIdx.x;
-    // this thread's region positions: wrapped input offset, region coordinates, output offset
-    int off[NL], lxm[NL], lym[NL];
+    // this thread's region positions: input offset, region coordinates, output offset;
+    // gh: 0 interior, 1 / 2 the low / high x face ghost, 3 / 4 the low / high y face ghost
+    int off[NL], lxm[NL], lym[NL], gh[NL];
     bool valid[NL], inr[NL];
     long long oo[NL];
 #pragma unroll
@@ -334,8 +374,17 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
         const int lx = ee % RX, ly = ee / RX;
         lxm[m] = lx;
         lym[m] = ly;
-        const int gx = ((x0 - K + lx) % a.n[0] + a.n[0]) % a.n[0];
-        const int gy = ((y0 - K + ly) % a.n[1] + a.n[1]) % a.n[1];
+        int gx = x0 - K + lx, gy = y0 - K + ly;
+        gh[m] = 0;
+        if (ALLPER || a.per[0]) gx = (gx % a.n[0] + a.n[0]) % a.n[0];
+        if (ALLPER || a.per[1]) gy = (gy % a.n[1] + a.n[1]) % a.n[1];
+        if (!ALLPER) {
+            const bool inx = gx >= 0 && gx < a.n[0], iny = gy >= 0 && gy < a.n[1];
+            if (iny && (gx == -1 || gx == a.n[0])) gh[m] = gx < 0 ? 1 : 2;
+            if (inx && (gy == -1 || gy == a.n[1])) gh[m] = gy < 0 ? 3 : 4;
+            gx = min(max(gx, -1), a.n[0]);  // clamped: inside the storage
+            gy = min(max(gy, -1), a.n[1]);
+        }
         off[m] = (gx + 1) + a.e0 * (gy + 1);
         const int i = x0 + lx - K, j = y0 + ly - K;
         inr[m] = i < a.n[0] && j < a.n[1];
@@ -345,10 +394,15 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
     auto at_level = [&](int m, int L) {
         return valid[m] && lxm[m] >= L && lxm[m] <= RX - 1 - L && lym[m] >= L && lym[m] <= RY - 1 - L;
     };
+    // the region reaches a non-periodic x / y side: face ghosts to rebuild (uniform over the block)
+    const bool edge = !ALLPER && ((!a.per[0] && (x0 - K < 0 || x0 + TX + K > a.n[0])) ||
+                                  (!a.per[1] && (y0 - K < 0 || y0 + TY + K > a.n[1])));
+    const bool zper = ALLPER || a.per[2];
     const int zb = z0 - K, ze = z1 + K;  // level-0 planes zb .. ze - 1
     T pre[PF][NL], hist[K][NL][3];       // history slot of plane p: (p - zb) % 3
     auto load = [&](T(&r)[NL], int z) {
-        z = (z % a.n[2] + a.n[2]) % a.n[2];
+        if (zper) z = (z % a.n[2] + a.n[2]) % a.n[2];
+        else z = min(max(z, -1), a.n[2]);
         const T* p = in + sz * (z + 1);
 #pragma unroll
         for (int m = 0; m < NL; ++m) r[m] = p[off[m]];
@@ -368,22 +422,42 @@ __global__ __launch_bounds__(NT) void ftcs3d_tb_kernel(const T* __restrict__ in,
                 w0[tid + m * NT] = pre[u][m];
             }
             load(pre[u], min(zz + PF, ze - 1));
+            if (!ALLPER && edge) {
+                // face ghosts of the planes levels 1 .. K-1 wrote in the last step, from their
+                // boundary cells (read after the barrier above), before the next level reads them
+#pragma unroll
+                for (int L = 1; L < K; ++L) {
+                    if (zz - 1 < z0 - K + 2 * L) continue;  // level L had no plane yet (uniform)
+                    T* P = lds[L][(zz - 1 - L) & 1];
+#pragma unroll
+                    for (int m = 0; m < NL; ++m) {
+                        if (!gh[m] || !at_level(m, L)) continue;
+                        const int e = tid + m * NT;
+                        const int g = gh[m];
+                        const int nb = g == 1 ? e + 1 : g == 2 ? e - 1 : g == 3 ? e + RX : e - RX;
+                        P[e] = rule(a, g <= 2 ? 0 : 1, (g - 1) & 1, P[nb]);
+                    }
+                }
+                __syncthreads();
+            }
 #pragma unroll
             for (int L = 1; L <= K; ++L) {
                 if (zz < z0 - K + 2 * L) break;  // plane zz - L of level L not computable yet (uniform)
                 // level L-1 planes zz-L+1, zz-L, zz-L-1 sit in slots sp, sc, sm
                 const int sp = ((u - L + 1) % 3 + 3) % 3, sc = ((u - L) % 3 + 3) % 3, sm = ((u - L - 1) % 3 + 3) % 3;
                 const T* P = lds[L - 1][(zz - L) & 1];
+                const int pz = zz - L;  // the plane level L computes
 #pragma unroll
                 for (int m = 0; m < NL; ++m) {
                     if (L < K ? !at_level(m, L) : !(at_level(m, K) && inr[m])) continue;
                     const int e = tid + m * NT;
                     const T c = hist[L - 1][m][sc];
-                    T s = d2<T, POW2>(P[e - 1], c, P[e + 1], a.hh[0]) + d2<T, POW2>(P[e - RX], c, P[e + RX], a.hh[1]);
-                    s = s + d2<T, POW2>(hist[L - 1][m][sm], c, hist[L - 1][m][sp], a.hh[2]);
-                    const T v = c + a.coef * s;
+                    T v = ftcs3<T, MODE>(c, P[e - 1], P[e + 1], P[e - RX], P[e + RX], hist[L - 1][m][sm],
+                                         hist[L - 1][m][sp], a);
                     if (L < K) {
+                        if (!zper && pz == a.n[2]) v = rule(a, 2, 1, hist[L][m][sm]);  // ghost plane n
                         hist[L][m][sc] = v;
+                        if (!zper && pz == 0) hist[L][m][sm] = rule(a, 2, 0, v);  // ghost plane -1
                         lds[L][(zz - L) & 1][e] = v;
                     } else {
                         out[sz * (zz - K + 1) + oo[m]] = v;
@@ -434,7 +508,8 @@ int launch_steps(int dim, int dtype, const int n[3], const int per[3], long long
         return 1;
     }
     const int k3 = dtype == OPF_F64 ? Ftcs3dCfg<double>::K : Ftcs3dCfg<float>::K;
-    if (dim == 3 && per[0] && per[1] && per[2] && plane < (1LL << 31) && remaining >= 2) return std::min(k3, remaining);
+    (void)per;
+    if (dim == 3 && plane < (1LL << 31) && remaining >= 2) return std::min(k3, remaining);
     return 1;
 }
 
@@ -486,8 +561,14 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock, void* ext_spare = 
         pow2 &= is_pow2(h);
         a.hh[q] = (T)h;
     }
-    if (pow2)
+    bool same = true;
+    for (int q = 1; q < d.dim; ++q) same &= m->h[q] == m->h[0];
+    const int mode = !pow2 ? FTCS_DIV : same ? FTCS_FACTORED : FTCS_MUL;
+    if (pow2) {
         for (int q = 0; q < d.dim; ++q) a.hh[q] = (T)(1.0 / m->h[q]);
+        const T k2 = a.hh[0] * a.hh[0];  // exact: a power of two
+        a.coefh = a.coef * k2;            // exact: a power-of-two scaling of the rounded coef
+    }
     hipStream_t st = f->ctx->stream;
     void* cur = f->data;  // time level the next launch reads
     void* nxt = nullptr;  // time level it writes
@@ -515,43 +596,60 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock, void* ext_spare = 
             OPF_TRY(launch_ghost_fill(f, cur));
             ghosts_stale = false;
         }
+        // launches the kernel instantiated for the arithmetic mode
+        auto by_mode = [&](auto launch) {
+            if (mode == FTCS_FACTORED) launch(std::integral_constant<int, FTCS_FACTORED>{});
+            else if (mode == FTCS_MUL) launch(std::integral_constant<int, FTCS_MUL>{});
+            else launch(std::integral_constant<int, FTCS_DIV>{});
+        };
         if (d.dim == 2 && k > 1) {
             // nt = k <= the kernel's TB steps (a remainder runs the same tile shape)
             int tiles = ((a.n[0] + FT_TS2 - 1) / FT_TS2) * ((a.n[1] + FT_TS2 - 1) / FT_TS2);
             if (tb2_steps(a.n) == FT_TB2S) {
                 // small grids: small tiles so that every CU gets several workgroups
                 int ts = ((a.n[0] + FT_TS2S - 1) / FT_TS2S) * ((a.n[1] + FT_TS2S - 1) / FT_TS2S);
-                if (pow2) hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2S, FT_TB2S>), dim3(ts), dim3(256), 0, st, in, out, a, k);
-                else hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2S, FT_TB2S>), dim3(ts), dim3(256), 0, st, in, out, a, k);
-            } else if (pow2) {
-                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, true, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, k);
+                by_mode([&](auto M) {
+                    hipLaunchKernelGGL((ftcs2d_tb_kernel<T, decltype(M)::value, FT_TS2S, FT_TB2S>), dim3(ts), dim3(256),
+                                       0, st, in, out, a, k);
+                });
             } else {
-                hipLaunchKernelGGL((ftcs2d_tb_kernel<T, false, FT_TS2, FT_TB2>), dim3(tiles), dim3(256), 0, st, in, out, a, k);
+                by_mode([&](auto M) {
+                    hipLaunchKernelGGL((ftcs2d_tb_kernel<T, decltype(M)::value, FT_TS2, FT_TB2>), dim3(tiles), dim3(256),
+                                       0, st, in, out, a, k);
+                });
             }
         } else if (d.dim == 2) {
             dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4);
-            if (pow2) hipLaunchKernelGGL((ftcs2d_kernel<T, true>), grid, dim3(64, 4), 0, st, in, out, a);
-            else hipLaunchKernelGGL((ftcs2d_kernel<T, false>), grid, dim3(64, 4), 0, st, in, out, a);
+            by_mode([&](auto M) {
+                hipLaunchKernelGGL((ftcs2d_kernel<T, decltype(M)::value>), grid, dim3(64, 4), 0, st, in, out, a);
+            });
         } else if (d.dim == 3 && k > 1) {
             using C3 = Ftcs3dCfg<T>;
             int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
-            if (k == C3::K) {
-                if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-                else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-            } else {
-                static_assert(C3::K == 3, "remainder launches assume K = 3");
-                if (pow2) hipLaunchKernelGGL((ftcs3d_tb_kernel<T, true, C3::TX, C3::TY, C3::NT, 2, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-                else hipLaunchKernelGGL((ftcs3d_tb_kernel<T, false, C3::TX, C3::TY, C3::NT, 2, C3::XCD>), dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
-            }
+            static_assert(C3::K == 3, "remainder launches assume K = 3");
+            const bool allper = a.per[0] && a.per[1] && a.per[2];
+            by_mode([&](auto M) {
+                constexpr int MD = decltype(M)::value;
+                if (k == C3::K && allper)
+                    hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD, true>),
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                else if (k == C3::K)
+                    hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD, false>),
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                else if (allper)
+                    hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, 2, C3::XCD, true>),
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                else
+                    hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, 2, C3::XCD, false>),
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+            });
             ghosts_stale = true;
         } else if (d.dim == 3) {
             dim3 grid((a.n[0] + FT3_TX - 1) / FT3_TX, (a.n[1] + FT3_TY - 1) / FT3_TY, (a.n[2] + FT3_ZC - 1) / FT3_ZC);
-            if (pow2)
-                hipLaunchKernelGGL((ftcs3d_lds_kernel<T, true, FT3_ZC, FT3_NTY, FT3_RY>), grid, dim3(64, FT3_NTY), 0, st,
-                                   in, out, a);
-            else
-                hipLaunchKernelGGL((ftcs3d_lds_kernel<T, false, FT3_ZC, FT3_NTY, FT3_RY>), grid, dim3(64, FT3_NTY), 0,
-                                   st, in, out, a);
+            by_mode([&](auto M) {
+                hipLaunchKernelGGL((ftcs3d_lds_kernel<T, decltype(M)::value, FT3_ZC, FT3_NTY, FT3_RY>), grid,
+                                   dim3(64, FT3_NTY), 0, st, in, out, a);
+            });
             ghosts_stale = true;
         } else {
             set_error("ftcs: dim %d not supported by the fast path", d.dim);

@@ -12,9 +12,13 @@
 // each expression is compiled once per machine. The context's jit mode picks
 // the engine per sweep: the interpreter, the compiled kernel for sweeps of at
 // least 2^18 cells (default), or the compiled kernel always.
+#include <dirent.h>
 #include <hip/hiprtc.h>
 #include <sys/stat.h>
 #include <unistd.h>
+
+#include <algorithm>
+#include <utility>
 
 #include <cerrno>
 #include <cmath>
@@ -131,6 +135,10 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
                                  in.b + (long long)r * a.f[in.a].e0 * a.f[in.a].e1);
                     break;
                 case I_CONST: {
+                    if (in.a) {  // launch-time constant: the source does not depend on its value
+                        snprintf(line, sizeof line, "  const T v%d = (T)a.cst[%d];\n", id, in.b);
+                        break;
+                    }
                     long long bits;
                     std::memcpy(&bits, &in.imm, 8);
                     snprintf(line, sizeof line, "  const T v%d = (T)__longlong_as_double(%lldLL);\n", id, bits);
@@ -154,6 +162,32 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
                     int x;
                     if (!pop(x)) return OPF_ERR_ARG;
                     snprintf(line, sizeof line, "  const T v%d = -v%d;\n", id, x);
+                } break;
+                case I_MIN:
+                case I_MAX:
+                case I_GT:
+                case I_LT:
+                case I_GE:
+                case I_LE: {
+                    int x, y;
+                    if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                    if (in.op == I_MIN)  // std::min(a, b) = b < a ? b : a
+                        snprintf(line, sizeof line, "  const T v%d = v%d < v%d ? v%d : v%d;\n", id, y, x, y, x);
+                    else if (in.op == I_MAX)  // std::max(a, b) = a < b ? b : a
+                        snprintf(line, sizeof line, "  const T v%d = v%d < v%d ? v%d : v%d;\n", id, x, y, y, x);
+                    else
+                        snprintf(line, sizeof line, "  const T v%d = v%d %s v%d ? (T)1 : (T)0;\n", id, x,
+                                 in.op == I_GT ? ">" : in.op == I_LT ? "<" : in.op == I_GE ? ">=" : "<=", y);
+                } break;
+                case I_ABS: {
+                    int x;
+                    if (!pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "  const T v%d = (T)fabs((double)v%d);\n", id, x);
+                } break;
+                case I_SEL: {
+                    int c, t_, f_;
+                    if (!pop(f_) || !pop(t_) || !pop(c)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "  const T v%d = v%d != (T)0 ? v%d : v%d;\n", id, c, t_, f_);
                 } break;
                 case I_WENO_UP:
                 case I_WENO_DOWN: {
@@ -196,6 +230,314 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
     return OPF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Tiled plane-marching form. A 64 x NTY workgroup owns a 64 x TY tile of the
+// x-y plane (TY = NTY * RY: each thread RY rows of one column) and marches a
+// chunk of ZC planes in z. Every field the program reads is staged in LDS as a
+// ring of planes covering the tile plus the program's x/y read offsets; the
+// ring holds the planes at the z offsets the program reads plus one, so one
+// barrier per plane separates a plane's store from its reads (the slot written
+// at plane k + 1 was last read at plane k - 1). The next plane is loaded into
+// registers while the current one is computed. Each load of the program reads
+// its field's ring at a constant offset from the thread's base. Same operations
+// per cell as the direct form (and as the interpreter), so the same bits.
+struct TileField {
+    bool used = false;
+    int oxn = 0, oxp = 0, oyn = 0, oyp = 0, ozn = 0, ozp = 0;  // read offsets (min, max) per dimension
+    int W = 0, H = 0, NZ = 0, NQ = 0;                          // tile width, height, ring planes, loads per thread
+};
+constexpr int kTileNTY = 4;
+
+bool tile_layout(const LoweredProgram& P, bool plane, int RY, TileField (&tf)[SWEEP_MAX_FIELDS], size_t* lds_bytes,
+                 int tsize) {
+    for (auto& f : tf) f = TileField{};
+    for (const Insn& in : P.code) {
+        if (in.op != I_LD) continue;
+        TileField& f = tf[in.a];
+        const int dx = (in.c & 255) - 64, dy = ((in.c >> 8) & 255) - 64, dz = ((in.c >> 16) & 255) - 64;
+        if (!f.used) {
+            f.used = true;
+            f.oxn = f.oxp = dx;
+            f.oyn = f.oyp = dy;
+            f.ozn = f.ozp = dz;
+        }
+        f.oxn = std::min(f.oxn, dx), f.oxp = std::max(f.oxp, dx);
+        f.oyn = std::min(f.oyn, dy), f.oyp = std::max(f.oyp, dy);
+        f.ozn = std::min(f.ozn, dz), f.ozp = std::max(f.ozp, dz);
+    }
+    size_t bytes = 0;
+    for (auto& f : tf) {
+        if (!f.used) continue;
+        if (plane && (f.ozn != 0 || f.ozp != 0)) return false;
+        f.W = 64 + f.oxp - f.oxn;
+        f.H = kTileNTY * RY + f.oyp - f.oyn;
+        f.NZ = plane ? 1 : f.ozp - f.ozn + 2;
+        f.NQ = (f.W * f.H + 255) / 256;
+        bytes += (size_t)f.NZ * f.W * f.H * tsize;
+    }
+    *lds_bytes = bytes;
+    return true;
+}
+
+int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, bool plane,
+                    const TileField (&tf)[SWEEP_MAX_FIELDS], std::string& s) {
+    s.clear();
+    s += dtype == OPF_F64 ? "typedef double T;\n" : "typedef float T;\n";
+    s += kArgsSrc;
+    s += "\n";
+    s += kWenoSrc;
+    const int TY = kTileNTY * RY;
+    char line[640];
+    snprintf(line, sizeof line,
+             "\nextern \"C\" __global__ __launch_bounds__(256) void opf_sweep_jit(SweepArgs a) {\n"
+             "  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;\n"
+             "  const int x0 = a.s[0] + (int)blockIdx.x * 64, y0 = a.s[1] + (int)blockIdx.y * %d;\n"
+             "  const int kb = a.s[2] + (int)blockIdx.z * %d;\n"
+             "  const int ke = min(kb + %d, a.s[2] + a.n[2]);\n"
+             "  const int xe = a.s[0] + a.n[0], ye = a.s[1] + a.n[1];\n"
+             "  const int i = x0 + tx, jb = y0 + ty * %d;\n",
+             TY, ZC, ZC, RY);
+    s += line;
+    bool dxu[3] = {}, xu[3] = {};
+    for (const Insn& in : P.code) {
+        if (in.op == I_DX) dxu[in.a] = true;
+        if (in.op == I_X) xu[in.a] = true;
+        OPF_CHECK(in.op != I_LDU, OPF_ERR_ARG, "jit: pad-mode program");
+    }
+    // per field: the ring, the tile origin in storage coordinates, this thread's load slots
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        const TileField& F = tf[f];
+        if (!F.used) continue;
+        const int HW = F.W * F.H;
+        snprintf(line, sizeof line,
+                 "  __shared__ T L%d[%d];\n"
+                 "  const T* __restrict__ G%d = (const T*)a.f[%d].ptr;\n"
+                 "  const long long PS%d = (long long)a.f[%d].e0 * a.f[%d].e1;\n"
+                 "  const int gx%d = x0 + (%d) - a.f[%d].off0, gy%d = y0 + (%d) - a.f[%d].off1;\n"
+                 "  const int xl%d = xe - 1 + (%d) - a.f[%d].off0, yl%d = ye - 1 + (%d) - a.f[%d].off1;\n",
+                 f, F.NZ * HW, f, f, f, f, f, f, F.oxn, f, f, F.oyn, f, f, F.oxp, f, f, F.oyp, f);
+        s += line;
+        for (int q = 0; q < F.NQ; ++q) {
+            // position e = t + 256 q of the W x H tile: valid inside the tile and the sweep's read box
+            snprintf(line, sizeof line,
+                     "  const int e%d_%d = t + %d, lx%d_%d = e%d_%d %% %d, ly%d_%d = e%d_%d / %d;\n"
+                     "  const bool ok%d_%d = e%d_%d < %d && gx%d + lx%d_%d <= xl%d && gy%d + ly%d_%d <= yl%d;\n"
+                     "  const long long o%d_%d = (long long)(gx%d + lx%d_%d) + (long long)a.f[%d].e0 * (gy%d + ly%d_%d);\n",
+                     f, q, 256 * q, f, q, f, q, F.W, f, q, f, q, F.W,
+                     f, q, f, q, HW, f, f, q, f, f, f, q, f,
+                     f, q, f, f, q, f, f, f, q);
+            s += line;
+        }
+        snprintf(line, sizeof line, "  const int tb%d = (ty * %d - (%d)) * %d + (tx - (%d));\n", f, RY, F.oyn, F.W,
+                 F.oxn);
+        s += line;
+    }
+    // mesh tables (clamped to the last live cell, so dead lanes read inside the tables)
+    if (dxu[0] || xu[0]) s += "  const int ic = min(i, xe - 1);\n";
+    for (int d = 0; d < 2; ++d) {
+        if (dxu[d] && d == 0) s += "  const T* __restrict__ DX0 = (const T*)a.dx[0] + (ic - a.morg[0]);\n";
+        if (xu[d] && d == 0) s += "  const T* __restrict__ X0 = (const T*)a.x[0] + (ic - a.morg[0]);\n";
+    }
+    if (dxu[1] || xu[1])
+        for (int r = 0; r < RY; ++r) {
+            if (dxu[1]) {
+                snprintf(line, sizeof line,
+                         "  const T* __restrict__ DX1_%d = (const T*)a.dx[1] + (min(jb + %d, ye - 1) - a.morg[1]);\n",
+                         r, r);
+                s += line;
+            }
+            if (xu[1]) {
+                snprintf(line, sizeof line,
+                         "  const T* __restrict__ X1_%d = (const T*)a.x[1] + (min(jb + %d, ye - 1) - a.morg[1]);\n", r,
+                         r);
+                s += line;
+            }
+        }
+    // prologue: planes kb + ozn .. kb + ozp - 1 into their slots, plane kb + ozp into registers
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        const TileField& F = tf[f];
+        if (!F.used) continue;
+        const int HW = F.W * F.H;
+        if (!plane) {
+            snprintf(line, sizeof line, "  for (int p = kb + (%d); p < kb + (%d); ++p) {\n", F.ozn, F.ozp);
+            s += line;
+            for (int q = 0; q < F.NQ; ++q) {
+                snprintf(line, sizeof line,
+                         "    if (ok%d_%d) L%d[((p - kb - (%d)) %% %d) * %d + e%d_%d] = G%d[o%d_%d + PS%d * (p - a.f[%d].off2)];\n",
+                         f, q, f, F.ozn, F.NZ, HW, f, q, f, f, q, f, f);
+                s += line;
+            }
+            s += "  }\n";
+        }
+        for (int q = 0; q < F.NQ; ++q) {
+            snprintf(line, sizeof line,
+                     "  T pf%d_%d = ok%d_%d ? G%d[o%d_%d + PS%d * (kb + (%d) - a.f[%d].off2)] : (T)0;\n", f, q, f, q, f,
+                     f, q, f, F.ozp, f);
+            s += line;
+        }
+    }
+    s += "  int ring = 0;  // (k - kb) modulo each ring's length, advanced per plane\n";
+    s += "  for (int k = kb; k < ke; ++k) {\n";
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        const TileField& F = tf[f];
+        if (!F.used) continue;
+        const int HW = F.W * F.H;
+        // store the prefetched plane k + ozp into its slot
+        snprintf(line, sizeof line, "    { int sl = ring %% %d + (%d); sl -= sl >= %d ? %d : 0;\n", F.NZ, F.ozp - F.ozn,
+                 F.NZ, F.NZ);
+        s += line;
+        for (int q = 0; q < F.NQ; ++q) {
+            snprintf(line, sizeof line, "      if (ok%d_%d) L%d[sl * %d + e%d_%d] = pf%d_%d;\n", f, q, f, HW, f, q, f, q);
+            s += line;
+        }
+        s += "    }\n";
+    }
+    s += "    __syncthreads();\n";
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        const TileField& F = tf[f];
+        if (!F.used || plane) continue;
+        s += "    if (k + 1 < ke) {\n";
+        for (int q = 0; q < F.NQ; ++q) {
+            snprintf(line, sizeof line, "      if (ok%d_%d) pf%d_%d = G%d[o%d_%d + PS%d * (k + 1 + (%d) - a.f[%d].off2)];\n",
+                     f, q, f, q, f, f, q, f, F.ozp, f);
+            s += line;
+        }
+        s += "    }\n";
+    }
+    // ring bases of the planes the program reads
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        const TileField& F = tf[f];
+        if (!F.used) continue;
+        const int HW = F.W * F.H;
+        for (int dz = F.ozn; dz <= F.ozp; ++dz) {
+            snprintf(line, sizeof line,
+                     "    const T* __restrict__ B%d_%d; { int sl = ring %% %d + (%d); sl -= sl >= %d ? %d : 0; "
+                     "B%d_%d = L%d + sl * %d + tb%d; }\n",
+                     f, dz + 64, F.NZ, dz - F.ozn, F.NZ, F.NZ, f, dz + 64, f, HW, f);
+            s += line;
+        }
+    }
+    if (dxu[2]) s += "    const T* __restrict__ DX2 = (const T*)a.dx[2] + (k - a.morg[2]);\n";
+    if (xu[2]) s += "    const T* __restrict__ X2 = (const T*)a.x[2] + (k - a.morg[2]);\n";
+    std::vector<int> st;
+    std::vector<int> res;
+    const int NV = (int)P.code.size();
+    auto pop = [&](int& v) -> bool {
+        if (st.empty()) return false;
+        v = st.back();
+        st.pop_back();
+        return true;
+    };
+    for (int r = 0; r < RY; ++r) {
+        int nv = r * NV;
+        st.clear();
+        for (const Insn& in : P.code) {
+            const int id = nv;
+            switch (in.op) {
+                case I_LD: {
+                    const TileField& F = tf[in.a];
+                    const int dx = (in.c & 255) - 64, dy = ((in.c >> 8) & 255) - 64, dz = ((in.c >> 16) & 255) - 64;
+                    snprintf(line, sizeof line, "    const T v%d = B%d_%d[%d];\n", id, in.a, dz + 64,
+                             (r + dy) * F.W + dx);
+                } break;
+                case I_CONST: {
+                    if (in.a) {
+                        snprintf(line, sizeof line, "    const T v%d = (T)a.cst[%d];\n", id, in.b);
+                        break;
+                    }
+                    long long bits;
+                    std::memcpy(&bits, &in.imm, 8);
+                    snprintf(line, sizeof line, "    const T v%d = (T)__longlong_as_double(%lldLL);\n", id, bits);
+                } break;
+                case I_DX:
+                case I_X: {
+                    const char* tb = in.op == I_DX ? "DX" : "X";
+                    if (in.a == 1) snprintf(line, sizeof line, "    const T v%d = %s1_%d[%d];\n", id, tb, r, in.b);
+                    else snprintf(line, sizeof line, "    const T v%d = %s%d[%d];\n", id, tb, in.a, in.b);
+                } break;
+                case I_ADD:
+                case I_SUB:
+                case I_MUL:
+                case I_DIV: {
+                    int x, y;
+                    if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                    const char o = in.op == I_ADD ? '+' : in.op == I_SUB ? '-' : in.op == I_MUL ? '*' : '/';
+                    snprintf(line, sizeof line, "    const T v%d = v%d %c v%d;\n", id, x, o, y);
+                } break;
+                case I_NEG: {
+                    int x;
+                    if (!pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "    const T v%d = -v%d;\n", id, x);
+                } break;
+                case I_MIN:
+                case I_MAX:
+                case I_GT:
+                case I_LT:
+                case I_GE:
+                case I_LE: {
+                    int x, y;
+                    if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                    if (in.op == I_MIN)  // std::min(a, b) = b < a ? b : a
+                        snprintf(line, sizeof line, "      const T v%d = v%d < v%d ? v%d : v%d;\n", id, y, x, y, x);
+                    else if (in.op == I_MAX)  // std::max(a, b) = a < b ? b : a
+                        snprintf(line, sizeof line, "      const T v%d = v%d < v%d ? v%d : v%d;\n", id, x, y, y, x);
+                    else
+                        snprintf(line, sizeof line, "      const T v%d = v%d %s v%d ? (T)1 : (T)0;\n", id, x,
+                                 in.op == I_GT ? ">" : in.op == I_LT ? "<" : in.op == I_GE ? ">=" : "<=", y);
+                } break;
+                case I_ABS: {
+                    int x;
+                    if (!pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "      const T v%d = (T)fabs((double)v%d);\n", id, x);
+                } break;
+                case I_SEL: {
+                    int c, t_, f_;
+                    if (!pop(f_) || !pop(t_) || !pop(c)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "      const T v%d = v%d != (T)0 ? v%d : v%d;\n", id, c, t_, f_);
+                } break;
+                case I_WENO_UP:
+                case I_WENO_DOWN: {
+                    int h, v[6];
+                    if (!pop(h)) return OPF_ERR_ARG;
+                    for (int q = 5; q >= 0; --q)
+                        if (!pop(v[q])) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line,
+                             "    const T v%d = weno_kernel<T>(v%d, v%d, v%d, v%d, v%d, v%d, v%d, %s);\n", id, v[0],
+                             v[1], v[2], v[3], v[4], v[5], h, in.op == I_WENO_UP ? "true" : "false");
+                } break;
+                default: return OPF_ERR_ARG;
+            }
+            s += line;
+            st.push_back(id);
+            ++nv;
+        }
+        if (st.size() != 1) return OPF_ERR_ARG;
+        res.push_back(st.back());
+    }
+    s += "    if (i < xe) {\n";
+    s += "      const long long lo = (i - a.oo[0]) + (long long)a.oe0 * ((jb - a.oo[1]) + (long long)a.oe1 * (k - a.oo[2]));\n";
+    for (int r = 0; r < RY; ++r) {
+        char at[48];
+        snprintf(at, sizeof at, "lo + %d * a.oe0", r);
+        const int v = res[r];
+        char val[160];
+        switch (op) {
+            case OPF_OP_EQ: snprintf(val, sizeof val, "v%d", v); break;
+            case OPF_OP_ADD: snprintf(val, sizeof val, "((const T*)a.old)[%s] + v%d", at, v); break;
+            case OPF_OP_MINUS: snprintf(val, sizeof val, "((const T*)a.old)[%s] - v%d", at, v); break;
+            case OPF_OP_MUL: snprintf(val, sizeof val, "((const T*)a.old)[%s] * v%d", at, v); break;
+            case OPF_OP_DIV: snprintf(val, sizeof val, "((const T*)a.old)[%s] / v%d", at, v); break;
+            default: return OPF_ERR_ARG;
+        }
+        snprintf(line, sizeof line, "      if (jb + %d < ye) ((T*)a.out)[%s] = %s;\n", r, at, val);
+        s += line;
+    }
+    s += "    }\n";
+    s += "    ring = ring + 1;\n";
+    s += "  }\n}\n";
+    return OPF_OK;
+}
+
 std::string cache_dir() {
     const char* e = getenv("OPF_JIT_CACHE");
     if (e && *e) return e;
@@ -228,6 +570,26 @@ bool read_file(const std::string& path, std::vector<char>& out) {
     return ok;
 }
 
+// at most this many code objects stay in the disk cache (oldest removed first)
+constexpr size_t kMaxDiskEntries = 1024;
+
+void prune_dir(const std::string& dir) {
+    DIR* dp = opendir(dir.c_str());
+    if (!dp) return;
+    std::vector<std::pair<double, std::string>> files;
+    while (dirent* de = readdir(dp)) {
+        const std::string n = de->d_name;
+        if (n.size() < 10 || n.compare(0, 6, "sweep_") != 0 || n.compare(n.size() - 3, 3, ".co") != 0) continue;
+        struct stat sb;
+        if (stat((dir + "/" + n).c_str(), &sb) == 0)
+            files.emplace_back((double)sb.st_mtim.tv_sec + 1e-9 * sb.st_mtim.tv_nsec, n);
+    }
+    closedir(dp);
+    if (files.size() <= kMaxDiskEntries) return;
+    std::sort(files.begin(), files.end());
+    for (size_t q = 0; q + kMaxDiskEntries < files.size(); ++q) unlink((dir + "/" + files[q].second).c_str());
+}
+
 void write_file(const std::string& dir, const std::string& name, const std::vector<char>& code) {
     if (dir.empty() || !mkdirs(dir)) return;  // the disk cache is an optimisation only
     const std::string tmp = dir + "/" + name + ".tmp" + std::to_string((long)getpid());
@@ -237,6 +599,7 @@ void write_file(const std::string& dir, const std::string& name, const std::vect
     fclose(f);
     if (ok) rename(tmp.c_str(), (dir + "/" + name).c_str());
     else unlink(tmp.c_str());
+    if (ok) prune_dir(dir);
 }
 
 int compile(const std::string& src, std::vector<char>& code) {
@@ -262,14 +625,54 @@ int compile(const std::string& src, std::vector<char>& code) {
     return OPF_OK;
 }
 
+// Loaded kernels, key: source, options, compiler version, device. At most
+// kMaxModules stay loaded; past that the least recently used one is unloaded
+// after the device has drained (a graph that captured it must not be replayed).
+constexpr size_t kMaxModules = 512;
+struct Loaded {
+    hipFunction_t fn;
+    hipModule_t mod;
+    int dev;
+    uint64_t used;
+};
 std::mutex g_mu;
-std::unordered_map<uint64_t, hipFunction_t> g_fns;  // key: source, options, device
+std::unordered_map<uint64_t, Loaded> g_fns;
+uint64_t g_tick = 0;
+long long g_compiles = 0, g_loads = 0;
 
+int evict_lru_module(int dev) {
+    auto victim = g_fns.end();
+    for (auto it = g_fns.begin(); it != g_fns.end(); ++it)
+        if (it->second.dev == dev && (victim == g_fns.end() || it->second.used < victim->second.used)) victim = it;
+    if (victim == g_fns.end()) return OPF_OK;
+    OPF_HIP(hipDeviceSynchronize());
+    OPF_HIP(hipModuleUnload(victim->second.mod));
+    g_fns.erase(victim);
+    return OPF_OK;
+}
+
+}  // namespace
+
+void jit_stats(long long* compiles, long long* loaded, long long* resident) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    *compiles = g_compiles;
+    *loaded = g_loads;
+    *resident = (long long)g_fns.size();
+}
+
+namespace {
+int function_for_source(const std::string& src, hipFunction_t* fn);
 }  // namespace
 
 int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, hipFunction_t* fn) {
     std::string src;
     OPF_CHECK(gen_source(P, dtype, op, R, a, src) == OPF_OK, OPF_ERR_ARG, "jit: malformed sweep program");
+    return function_for_source(src, fn);
+}
+
+namespace {
+// the loaded kernel of a generated source: in-process cache, disk cache, then hipRTC
+int function_for_source(const std::string& src, hipFunction_t* fn) {
     int dev = 0;
     OPF_HIP(hipGetDevice(&dev));
     uint64_t h = fnv1a64(src.data(), src.size());
@@ -280,31 +683,66 @@ int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const 
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_fns.find(key);
     if (it != g_fns.end()) {
-        *fn = it->second;
+        it->second.used = ++g_tick;
+        *fn = it->second.fn;
         return OPF_OK;
     }
     const std::string dir = cache_dir(), name = "sweep_" + hexkey(h) + ".co";
     std::vector<char> code;
     if (dir.empty() || !read_file(dir + "/" + name, code)) {
         OPF_TRY(compile(src, code));
+        ++g_compiles;
         write_file(dir, name, code);
     }
+    size_t on_dev = 0;
+    for (auto& e : g_fns) on_dev += e.second.dev == dev;
+    if (on_dev >= kMaxModules) OPF_TRY(evict_lru_module(dev));
     hipModule_t mod;
     OPF_HIP(hipModuleLoadData(&mod, code.data()));
     hipFunction_t f;
-    OPF_HIP(hipModuleGetFunction(&f, mod, "opf_sweep_jit"));
-    g_fns.emplace(key, f);
+    if (hipModuleGetFunction(&f, mod, "opf_sweep_jit") != hipSuccess) {
+        hipModuleUnload(mod);
+        set_error("jit: no opf_sweep_jit in the code object");
+        return OPF_ERR_HIP;
+    }
+    g_fns.emplace(key, Loaded{f, mod, dev, ++g_tick});
+    ++g_loads;
     *fn = f;
     return OPF_OK;
 }
+}  // namespace
 
 int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a) {
     hipFunction_t fn;
+    // tiled form: fields staged in LDS rings, 4 / 2 / 1 rows per thread by LDS need
+    const bool plane = a.n[2] == 1;
+    constexpr int kZC = 32;
+    const int tsize = dtype == OPF_F64 ? 8 : 4;
+    for (int RY = 4; RY >= 1; RY /= 2) {
+        TileField tf[SWEEP_MAX_FIELDS];
+        size_t lds = 0;
+        if (!tile_layout(P, plane, RY, tf, &lds, tsize)) break;
+        int nq = 0;
+        for (auto& f : tf) nq += f.used ? f.NQ : 0;
+        if (lds > (size_t)64 * 1024 || nq > 24 || P.code.size() * RY > (size_t)SWEEP_JIT_MAX_INSNS) continue;
+        std::string src;
+        OPF_CHECK(gen_source_tile(P, dtype, a.op, RY, kZC, plane, tf, src) == OPF_OK, OPF_ERR_ARG,
+                  "jit: malformed sweep program");
+        OPF_TRY(function_for_source(src, &fn));
+        SweepArgs args = a;
+        void* params[] = {&args};
+        const int TY = kTileNTY * RY;
+        OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((a.n[0] + 63) / 64), (unsigned)((a.n[1] + TY - 1) / TY),
+                                      (unsigned)(plane ? 1 : (a.n[2] + kZC - 1) / kZC), 64, kTileNTY, 1, 0, st, params,
+                                      nullptr));
+        return OPF_OK;
+    }
+    // direct form: loads from global memory
     // each thread computes R cells along z (R = 4, 2 or 1, dividing the z extent):
     // 512^3 FTCS expression 126 -> 152 G cell-updates/s FP64, 209 -> 242 FP32 at R = 4; R = 8 is slower
     int R = 1;
     for (int c = 4; c > 1; c /= 2)
-        if (a.n[2] % c == 0) { R = c; break; }
+        if (a.n[2] % c == 0 && P.code.size() * c <= (size_t)SWEEP_JIT_MAX_INSNS) { R = c; break; }
     OPF_TRY(jit_sweep_function(P, dtype, a.op, R, a, &fn));
     SweepArgs args = a;
     void* params[] = {&args};

@@ -50,13 +50,22 @@ struct opf_ctx {
     double* red_buf = nullptr;
     size_t red_cap = 0;
     double* red_host = nullptr;  // pinned
-    // cache of lowered device programs keyed by content hash
+    // cache of lowered device programs keyed by content (the bytes a sweep uploads
+    // carry no constant values). Sweep programs are evictable: at most kMaxPrograms
+    // of them stay resident, least recently used freed first. Programs that an
+    // equation handler keeps using are not evictable and live as long as the context.
     struct ProgEntry {
         uint64_t hash;
         void* dev;
         size_t bytes;
+        uint64_t used;
+        bool evictable;
+        std::vector<unsigned char> host;
     };
+    static constexpr size_t kMaxPrograms = 1024;
     std::vector<ProgEntry> progs;
+    uint64_t prog_tick = 0;
+    long long prog_uploads = 0;
     // this device's copy of the allocation-free FTCS scratch (core.hip), looked up at creation
     void* noalloc_scratch = nullptr;
     // expression sweeps: 0 = interpreter, 1 = compiled kernel for >= 2^18 cells, 2 = compiled kernel always
@@ -108,6 +117,8 @@ void gen_bumps_grid(int dim, const int n[3], uint64_t seed, int nbumps, double n
 int gen_bumps_grid_dev(hipStream_t st, int dim, const int n[3], uint64_t seed, int nbumps, double noise, double* out);
 
 // device program cache
-int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev);
+// evictable: the caller uses *dev only for launches enqueued right away (a sweep),
+// so the cache may free it later, after a drain of the context's stream
+int upload_program(opf_ctx* ctx, const void* bytes, size_t n, void** dev, bool evictable = false);
 
 }  // namespace opf

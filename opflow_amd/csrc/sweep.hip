@@ -52,10 +52,24 @@ struct Lowerer {
     }
     void op(int o) {
         code.push_back({o, 0, 0, 0, 0.0});
-        push(o == I_NEG ? 0 : o == I_WENO_UP || o == I_WENO_DOWN ? -6 : -1);
+        push(o == I_NEG || o == I_ABS ? 0 : o == I_WENO_UP || o == I_WENO_DOWN ? -6 : o == I_SEL ? -2 : -1);
     }
     void cst(double v) {
         code.push_back({I_CONST, 0, 0, 0, v});
+        push(1);
+    }
+    // a ScalarExpr node: one launch-time slot per node (the node may be inlined at
+    // several shifts); past SWEEP_MAX_CONSTS distinct nodes the value is baked in
+    std::vector<int> cslot;
+    std::vector<double> consts;
+    void ucst(int n, double v) {
+        if (cslot.empty()) cslot.assign(nnodes, -1);
+        if (cslot[n] < 0 && (int)consts.size() < SWEEP_MAX_CONSTS) {
+            cslot[n] = (int)consts.size();
+            consts.push_back(v);
+        }
+        if (cslot[n] < 0) return cst(v);
+        code.push_back({I_CONST, 1, cslot[n], 0, v});
         push(1);
     }
     void tab(int kind, int d, int shift) {
@@ -87,9 +101,36 @@ struct Lowerer {
                 case OPF_N_SUB:
                 case OPF_N_MUL:
                 case OPF_N_DIV:
+                case OPF_N_MIN:
+                case OPF_N_MAX:
+                case OPF_N_GT:
+                case OPF_N_LT:
+                case OPF_N_GE:
+                case OPF_N_LE:
                     if (!kid(e.a) || !kid(e.b)) return OPF_ERR_ARG;
                     locs[n] = locs[e.a][0] >= 0 ? locs[e.a] : locs[e.b];
                     break;
+                case OPF_N_COND:  // the location of the value it selects (Conditional.hpp: initPropsFrom(arg2))
+                    if (!kid(e.a) || !kid(e.b) || !kid(e.dim)) return OPF_ERR_ARG;
+                    locs[n] = locs[e.b][0] >= 0 ? locs[e.b] : locs[e.dim];
+                    break;
+                case OPF_N_ABS:
+                    if (!kid(e.a)) return OPF_ERR_ARG;
+                    locs[n] = locs[e.a];
+                    break;
+                case OPF_N_FLUXLIM: {
+                    // the interpolated argument b decides the direction (D1FluxLimiter.hpp prepare)
+                    if (!kid(e.a) || !kid(e.b) || e.dim < 0 || e.dim >= mesh->dim) return OPF_ERR_ARG;
+                    const int code = (int)e.value, lim = code % 32, dir = code / 32;
+                    if ((double)code != e.value || code < 0 || lim > OPF_LIM_BSOU || dir > 2) return OPF_ERR_ARG;
+                    // Cen2Cor needs a centre-located argument, Cor2Cen a corner-located one
+                    if (dir != 0 && locs[e.b][e.dim] >= 0 &&
+                        locs[e.b][e.dim] != (dir == 1 ? OPF_LOC_CENTER : OPF_LOC_CORNER))
+                        return OPF_ERR_ARG;
+                    locs[n] = locs[e.b];
+                    if (locs[n][e.dim] >= 0)
+                        locs[n][e.dim] = locs[n][e.dim] == OPF_LOC_CENTER ? OPF_LOC_CORNER : OPF_LOC_CENTER;
+                } break;
                 default:
                     if (!kid(e.a)) return OPF_ERR_ARG;
                     if (e.kind != OPF_N_NEG && (e.dim < 0 || e.dim >= mesh->dim)) return OPF_ERR_ARG;
@@ -102,6 +143,111 @@ struct Lowerer {
             }
         }
         return OPF_OK;
+    }
+
+    // D1FluxLimiterImpl<Kernel, d, dir>::eval (Interpolator/D1FluxLimiter.hpp:43-202) at shift sh:
+    //   u[i] > 0 ? upwind : downwind, each  y2 +- dx * 0.5 * K, in the reference's operation order
+    void emit_fluxlim(const opf_expr_node& e, const int sh[3]) {
+        const int d = e.dim, lim = (int)e.value % 32;
+        const bool c2n = locs[e.b][d] == OPF_LOC_CENTER;  // Cen2Cor: b centre-located in d
+        const int i = sh[d];
+        auto val = [&](int k) {  // b at index i + k along d
+            int t[3] = {sh[0], sh[1], sh[2]};
+            t[d] += k;
+            emit(e.b, t[0], t[1], t[2]);
+        };
+        // node coordinate (Cor2Cen) or cell centre x + 0.5 dx (Cen2Cor) at i + k
+        auto xp = [&](int k) {
+            x(d, i + k);
+            if (c2n) { cst(0.5); dx(d, i + k); op(I_MUL); op(I_ADD); }
+        };
+        // (yb - ya) / (xb - xa) for points i + ka, i + kb
+        auto slope = [&](int ka, int kb) {
+            val(kb); val(ka); op(I_SUB);
+            xp(kb); xp(ka); op(I_SUB);
+            op(I_DIV);
+        };
+        // one branch: points p1 < p2 < p3 (offsets), slop_u / slop_f pairs, the dx index and sign
+        auto branch = [&](bool up) {
+            int k1, k2, k3, kd;
+            if (c2n) { k1 = up ? -2 : -1; kd = up ? -1 : 0; }
+            else { k1 = up ? -1 : 0; kd = 0; }
+            k2 = k1 + 1;
+            k3 = k1 + 2;
+            // upwind: slop_u = (y2 - y1)/(x2 - x1), slop_f = (y3 - y2)/(x3 - x2); downwind swaps them
+            auto su = [&] { up ? slope(k1, k2) : slope(k2, k3); };
+            auto sf = [&] { up ? slope(k2, k3) : slope(k1, k2); };
+            val(k2);                            // y2
+            dx(d, i + kd); cst(0.5); op(I_MUL);  // dx * 0.5
+            if (lim <= OPF_LIM_LUI) {
+                // KappaKernel::eval(slop_u, slop_f) = (1 + kappa) / 2. * slop_f + (1 - kappa) / 2. * slop_u
+                const double kappa = lim == OPF_LIM_CDS ? 1.0 : lim == OPF_LIM_QUICK ? 0.5
+                                     : lim == OPF_LIM_CUI ? 1. / 3. : lim == OPF_LIM_FROMM ? 0.0 : -1.0;
+                cst((1 + kappa) / 2.); sf(); op(I_MUL);
+                cst((1 - kappa) / 2.); su(); op(I_MUL);
+                op(I_ADD);
+                op(I_MUL);
+            } else {
+                // r = slop_f / (slop_u + 1e-16); (dx * 0.5 * K(r)) * slop_u
+                auto r = [&] { sf(); su(); cst(1e-16); op(I_ADD); op(I_DIV); };
+                limiter(lim, r);
+                op(I_MUL);
+                su();
+                op(I_MUL);
+            }
+            op(up ? I_ADD : I_SUB);
+        };
+        emit(e.a, sh[0], sh[1], sh[2]);
+        cst(0.0);
+        op(I_GT);
+        branch(true);
+        branch(false);
+        op(I_SEL);
+    }
+    // K(r) of the ratio-form kernels (FluxLimiterKernels.hpp); std::min / std::max and their
+    // initializer-list forms fold left with the strict comparisons of I_MIN / I_MAX
+    template <typename R>
+    void limiter(int lim, R&& r) {
+        auto mr = [&](double m) { cst(m); r(); op(I_MUL); };
+        // 0.5 * (1 + kappa) * r + 0.5 * (1 - kappa)  and its mirror
+        auto lin = [&](double kappa, bool mirror) {
+            const double c1 = 0.5 * (1 + kappa), c2 = 0.5 * (1 - kappa);
+            cst(mirror ? c2 : c1); r(); op(I_MUL); cst(mirror ? c1 : c2); op(I_ADD);
+        };
+        switch (lim) {
+            case OPF_LIM_MINMOD:  // max(0., min(r, 1.))
+                cst(0.0); r(); cst(1.0); op(I_MIN); op(I_MAX);
+                break;
+            case OPF_LIM_SUPERBEE:  // max({0., min(2. * r, 1.), min(r, 2.)})
+                cst(0.0); mr(2.0); cst(1.0); op(I_MIN); op(I_MAX); r(); cst(2.0); op(I_MIN); op(I_MAX);
+                break;
+            case OPF_LIM_MUSCL:  // max(0., min({2 * r, (r + 1) / 2., 2.}))
+                cst(0.0); mr(2.0); r(); cst(1.0); op(I_ADD); cst(2.0); op(I_DIV); op(I_MIN); cst(2.0); op(I_MIN);
+                op(I_MAX);
+                break;
+            case OPF_LIM_HARMONIC:  // (r + fabs(r)) / (r + 1)
+                r(); r(); op(I_ABS); op(I_ADD); r(); cst(1.0); op(I_ADD); op(I_DIV);
+                break;
+            case OPF_LIM_VANALBADA:  // r * (r + 1) / (r * r + 1)
+                r(); r(); cst(1.0); op(I_ADD); op(I_MUL); r(); r(); op(I_MUL); cst(1.0); op(I_ADD); op(I_DIV);
+                break;
+            default: {
+                // SPL / GPL kernels; m, alpha and kappa are std::ratio num / den as integer divisions
+                // in the reference (FluxLimiterKernels.hpp:72-143), so the ratios < 1 count as 0
+                double m = 2, alpha = 0, kappa = 0;
+                if (lim == OPF_LIM_SMART) m = 4;
+                if (lim == OPF_LIM_BSOU) m = 1, kappa = -1;
+                cst(0.0);
+                if (lim == OPF_LIM_UMIST || lim == OPF_LIM_SPL13) {
+                    mr(m); lin(kappa, false); op(I_MIN); lin(kappa, true); op(I_MIN); cst(m); op(I_MIN);
+                } else if (lim == OPF_LIM_SPLMAX12 || lim == OPF_LIM_SPLMAX13) {
+                    mr(m); lin(kappa, false); lin(kappa, true); op(I_MAX); op(I_MIN); cst(m); op(I_MIN);
+                } else {  // GPLKappaKernel: Koren, SMART, BSOU
+                    mr(2 + alpha); lin(kappa, false); op(I_MIN); cst(m); op(I_MIN);
+                }
+                op(I_MAX);
+            }
+        }
     }
 
     void emit(int n, int s0, int s1, int s2) {
@@ -129,10 +275,32 @@ struct Lowerer {
                 if (ldu) {
                     // remember the offset in the instruction for pad mode slot lookup
                     code.back().c = (sh[0] + 8) | ((sh[1] + 8) << 5) | ((sh[2] + 8) << 10);
+                } else {
+                    // and for the compiled sweep's tile addressing (offsets within +-63)
+                    if (sh[0] < -63 || sh[0] > 63 || sh[1] < -63 || sh[1] > 63 || sh[2] < -63 || sh[2] > 63) {
+                        set_error("lowering: stencil offset beyond 63 cells");
+                        err = OPF_ERR_UNSUPPORTED;
+                        return;
+                    }
+                    code.back().c = (sh[0] + 64) | ((sh[1] + 64) << 8) | ((sh[2] + 64) << 16);
                 }
                 push(1);
             } break;
-            case OPF_N_CONST: cst(e.value); break;
+            case OPF_N_CONST: ucst(n, e.value); break;
+            case OPF_N_MIN: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_MIN); break;
+            case OPF_N_MAX: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_MAX); break;
+            case OPF_N_GT: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_GT); break;
+            case OPF_N_LT: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_LT); break;
+            case OPF_N_GE: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_GE); break;
+            case OPF_N_LE: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_LE); break;
+            case OPF_N_ABS: emit(e.a, s0, s1, s2); op(I_ABS); break;
+            case OPF_N_COND:
+                emit(e.a, s0, s1, s2);
+                emit(e.b, s0, s1, s2);
+                emit(e.dim, s0, s1, s2);
+                op(I_SEL);
+                break;
+            case OPF_N_FLUXLIM: emit_fluxlim(e, sh); break;
             case OPF_N_ADD: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_ADD); break;
             case OPF_N_SUB: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_SUB); break;
             case OPF_N_MUL: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_MUL); break;
@@ -246,6 +414,7 @@ int lower_tree_set(const opf_expr_node* nodes, int nnodes, int root, opf_field* 
               L.maxdepth, SWEEP_MAX_DEPTH);
     OPF_CHECK(L.code.size() < 100000, OPF_ERR_UNSUPPORTED, "expression too large after inlining");
     out->code = L.code;
+    out->consts = L.consts;
     out->maxdepth = L.maxdepth;
     for (int f = 0; f < SWEEP_MAX_FIELDS; ++f)
         for (int d = 0; d < 3; ++d) {
@@ -310,7 +479,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                 stk[sp++][tid] = tos;
                 tos = ((const T*)F.ptr)[base + in.b];
             } break;
-            case I_CONST: stk[sp++][tid] = tos; tos = (T)in.imm; break;
+            case I_CONST: stk[sp++][tid] = tos; tos = in.a ? (T)a.cst[in.b] : (T)in.imm; break;
             case I_DX: {
                 int c = in.a == 0 ? ci : in.a == 1 ? cj : k;
                 stk[sp++][tid] = tos;
@@ -326,6 +495,17 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
             case I_MUL: tos = stk[--sp][tid] * tos; break;
             case I_DIV: tos = stk[--sp][tid] / tos; break;
             case I_NEG: tos = -tos; break;
+            case I_MIN: { const T a_ = stk[--sp][tid]; tos = tos < a_ ? tos : a_; } break;
+            case I_MAX: { const T a_ = stk[--sp][tid]; tos = a_ < tos ? tos : a_; } break;
+            case I_ABS: tos = fabs(tos); break;
+            case I_GT: tos = stk[--sp][tid] > tos ? (T)1 : (T)0; break;
+            case I_LT: tos = stk[--sp][tid] < tos ? (T)1 : (T)0; break;
+            case I_GE: tos = stk[--sp][tid] >= tos ? (T)1 : (T)0; break;
+            case I_LE: tos = stk[--sp][tid] <= tos ? (T)1 : (T)0; break;
+            case I_SEL: {
+                sp -= 2;
+                tos = stk[sp][tid] != (T)0 ? stk[sp + 1][tid] : tos;
+            } break;
             case I_WENO_UP:
             case I_WENO_DOWN: {
                 sp -= 6;
@@ -356,10 +536,30 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
 
 namespace opf {
 
-int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P) {
+std::vector<Insn> structural_code(const LoweredProgram& P) {
+    std::vector<Insn> c = P.code;
+    for (Insn& in : c)
+        if (in.op == I_CONST && in.a) in.imm = 0.0;
+    return c;
+}
+
+int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a0, const LoweredProgram& P) {
     hipStream_t st = ctx->stream;
+    SweepArgs a = a0;
+    OPF_CHECK(P.consts.size() <= (size_t)SWEEP_MAX_CONSTS, OPF_ERR_ARG, "sweep: %zu constant slots",
+              P.consts.size());
+    for (size_t q = 0; q < P.consts.size(); ++q) a.cst[q] = P.consts[q];
     const long long cells = (long long)a.n[0] * a.n[1] * a.n[2];
-    if (ctx->jit_mode == 2 || (ctx->jit_mode == 1 && cells >= (1LL << 18))) return jit_launch_sweep(st, P, dtype, a);
+    const bool jit = (ctx->jit_mode == 2 || (ctx->jit_mode == 1 && cells >= (1LL << 18))) &&
+                     P.code.size() <= (size_t)SWEEP_JIT_MAX_INSNS;
+    if (jit) {
+        const int rc = jit_launch_sweep(st, P, dtype, a);
+        if (rc == OPF_OK) return OPF_OK;
+        // the compiled engine is an optimisation: a failed compile or module load
+        // runs the same program on the interpreter (same bits); argument errors stand
+        if (rc != OPF_ERR_HIP) return rc;
+        (void)hipGetLastError();
+    }
     dim3 grid((a.n[0] + 63) / 64, (a.n[1] + 3) / 4, a.n[2]);
     if (dtype == OPF_F64) hipLaunchKernelGGL(sweep_kernel<double>, grid, dim3(64, 4), 0, st, a);
     else hipLaunchKernelGGL(sweep_kernel<float>, grid, dim3(64, 4), 0, st, a);
@@ -391,6 +591,14 @@ int mesh_arrays(const opf_mesh* mc, int dtype, const void* x[3], const void* dx[
 }
 
 }  // namespace opf
+
+extern "C" int opf_cache_stats(opf_ctx* ctx, long long* stats) {
+    OPF_CHECK(ctx && stats, OPF_ERR_ARG, "opf_cache_stats: null arg");
+    jit_stats(&stats[0], &stats[1], &stats[2]);
+    stats[3] = ctx->prog_uploads;
+    stats[4] = (long long)ctx->progs.size();
+    return OPF_OK;
+}
 
 extern "C" int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_field* const* fields, int nfields,
                          opf_field* out, int op) {
@@ -446,7 +654,8 @@ extern "C" int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_f
         a.out = target;
         a.old = out->data;
         void* dev = nullptr;
-        OPF_TRY(upload_program(ctx, P.code.data(), P.code.size() * sizeof(Insn), &dev));
+        const std::vector<Insn> sc = structural_code(P);
+        OPF_TRY(upload_program(ctx, sc.data(), sc.size() * sizeof(Insn), &dev, true));
         a.code = dev;
         a.ncode = (int)P.code.size();
         OPF_TRY(launch_sweep(ctx, d.dtype, a, P));

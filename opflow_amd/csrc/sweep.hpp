@@ -8,12 +8,19 @@
 namespace opf {
 
 constexpr int SWEEP_MAX_FIELDS = 8;
-constexpr int SWEEP_MAX_DEPTH = 16;
+constexpr int SWEEP_MAX_DEPTH = 24;
+// scalar constants of an expression (ScalarExpr nodes) passed at launch, not
+// baked into the program: a new dt re-uses the program and the compiled kernel
+constexpr int SWEEP_MAX_CONSTS = 32;
+// programs longer than this run on the interpreter (compile time and code size
+// of the generated kernel grow with the program times the cells per thread)
+constexpr int SWEEP_JIT_MAX_INSNS = 16384;
 
 enum InsnOp {
-    I_LD = 0,     // push field[a] at linear offset b from the evaluation cell
+    I_LD = 0,     // push field[a] at linear offset b from the evaluation cell (c: the offset per
+                  // dimension, (dx + 64) | (dy + 64) << 8 | (dz + 64) << 16)
     I_LDU = 1,    // push unknown a (pad mode) at packed offset c
-    I_CONST = 2,  // push imm
+    I_CONST = 2,  // push imm; a = 1: push the launch-time constant slot b (imm keeps its value)
     I_DX = 3,     // push mesh dx of dim a at (cell index + b)
     I_X = 4,      // push mesh x of dim a at (cell index + b)
     I_ADD = 5,
@@ -22,7 +29,15 @@ enum InsnOp {
     I_DIV = 8,
     I_NEG = 9,
     I_WENO_UP = 10,
-    I_WENO_DOWN = 11
+    I_WENO_DOWN = 11,
+    I_MIN = 12,   // b < a ? b : a  (std::min(a, b))
+    I_MAX = 13,   // a < b ? b : a  (std::max(a, b))
+    I_ABS = 14,
+    I_GT = 15,    // a > b ? 1 : 0
+    I_LT = 16,
+    I_GE = 17,
+    I_LE = 18,
+    I_SEL = 19    // pops f, t, c: c != 0 ? t : f
 };
 
 struct Insn {
@@ -32,6 +47,7 @@ struct Insn {
 
 struct LoweredProgram {
     std::vector<Insn> code;
+    std::vector<double> consts;  // launch-time constants by slot (I_CONST with a = 1)
     int maxdepth = 0;
     int omin[SWEEP_MAX_FIELDS][3], omax[SWEEP_MAX_FIELDS][3];
     int tmin[3], tmax[3];
@@ -42,6 +58,10 @@ struct LoweredProgram {
 #include "sweep_args.inc"
 #undef OPF_SHARED_SRC
 static_assert(sizeof(((SweepArgs*)nullptr)->f) / sizeof(FieldArg) == SWEEP_MAX_FIELDS, "field slots");
+static_assert(sizeof(((SweepArgs*)nullptr)->cst) / sizeof(double) == SWEEP_MAX_CONSTS, "constant slots");
+// the program bytes a sweep uploads: launch-time constants zeroed, so the
+// device copy (and its cache key) depends on the expression's structure only
+std::vector<Insn> structural_code(const LoweredProgram& P);
 
 // unknown_slot: the field slot OPF_N_UNKNOWN reads; pad mode (unknown_as_field = false)
 // emits I_LDU for it, an ordinary sweep (true) reads that slot like any field
@@ -58,6 +78,8 @@ int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgr
 // the compiled kernel of P (hipRTC, cached) and its launch
 int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, hipFunction_t* fn);
 int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a);
+// process-wide: hipRTC compilations, compiled kernels loaded, kernels loaded now
+void jit_stats(long long* compiles, long long* loaded, long long* resident);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
                  const int n[3]);
 int mesh_arrays(const opf_mesh* m, int dtype, const void* x[3], const void* dx[3]);

@@ -22,6 +22,33 @@ class StructSolverParams:
     numPreRelax: Optional[int] = None
     numPostRelax: Optional[int] = None
     verbose: bool = False
+    # HYPRE knobs: honoured where they map onto the engine, refused where they would change
+    # the stopping test or the smoother kind, accepted without effect where they only select
+    # PFMG internals (rapType, skipRelax) -- as the C++ front-end does
+    useZeroGuess: Optional[bool] = None
+    relaxType: Optional[int] = None
+    rapType: Optional[int] = None
+    skipRelax: Optional[int] = None
+    twoNorm: Optional[int] = None
+    relChange: Optional[int] = None
+    absTol: Optional[float] = None
+
+
+def _check_params(q, which):
+    """Refuse (OPF_ERR_UNSUPPORTED) the HYPRE settings the engine cannot honour."""
+    if q is None:
+        return
+    msg = None
+    if q.absTol is not None and q.absTol > 0:
+        msg = "absTol > 0 is not supported (the engine stops on tol, the relative residual ||b-Ax||/||b||)"
+    elif q.twoNorm is not None and q.twoNorm == 0:
+        msg = "twoNorm = 0 is not supported (the stopping test uses the 2-norm)"
+    elif q.relChange is not None and q.relChange != 0:
+        msg = "relChange is not supported"
+    elif q.relaxType is not None and q.relaxType not in (0, 1):
+        msg = "relaxType >= 2 (Gauss-Seidel) is not supported: the smoother is weighted Jacobi"
+    if msg:
+        raise L.OpfError(L.OPF_ERR_UNSUPPORTED, f"{which}: {msg}")
 
 
 @dataclass
@@ -34,6 +61,8 @@ class EqnSolveState:  # Equation/EqnSolveHandler.hpp:17-25
 
 
 def _params(params, precParams):
+    _check_params(params, "solver params")
+    _check_params(precParams, "preconditioner params")
     p = L.SolveParams()
     p.solver = int(params.type if params else 6)
     p.precond = int(precParams.type if precParams else 0)
@@ -82,6 +111,8 @@ class _Handler:
         self._nodes = na(remap)
         fh = (C.c_void_p * len(self.fields))(*[f.h.value for f in self.fields])
         self.params = _params(solver.params, solver.precParams)
+        self.target = target
+        self.zero_guess = bool(solver.params and solver.params.useZeroGuess)
         h = C.c_void_p()
         lib = target.lib
         L.check(lib.opf_eqn_create(self._nodes, len(remap), lhs_root, rhs_root, fh, len(self.fields), target.h,
@@ -90,6 +121,8 @@ class _Handler:
 
     def solve(self):
         st = L.SolveState()
+        if self.zero_guess:
+            self.target.assign(0.0)
         L.check(self.lib.opf_eqn_solve(self.h, C.byref(st)))
         return EqnSolveState(st.niter, st.relerr, st.abserr, st.nrefine, bool(st.converged))
 

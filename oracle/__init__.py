@@ -31,6 +31,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_reduce.restype = C.c_double
         _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_ftcs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int]
         _lib.orc_update_padding.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.orc_desc_finalize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -111,6 +112,11 @@ def reduce(desc, arr, red, rs, re_):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    """OpenMP threads of later oracle calls; n <= 0 restores the process's initial count."""
+    lib().orc_set_threads(int(n))
 
 
 def csr_poisson(desc, mesh, c=1.0, pin_last=False):

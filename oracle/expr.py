@@ -11,7 +11,85 @@ import numpy as np
 
 FIELD, CONST, ADD, SUB, MUL, DIV, NEG = 0, 1, 2, 3, 4, 5, 6
 D2C, D1C, D1UP, D1DOWN, WENO_UP, WENO_DOWN, INTP_C2N, INTP_N2C, UNKNOWN = 7, 8, 9, 10, 11, 12, 13, 14, 15
+FLUXLIM, MIN, MAX, ABS, GT, LT, GE, LE, COND = 16, 17, 18, 19, 20, 21, 22, 23, 24
 CORNER, CENTER = 0, 1
+# limiter kernels of FLUXLIM (Interpolator/FluxLimiterKernels.hpp)
+(LIM_CDS, LIM_QUICK, LIM_CUI, LIM_FROMM, LIM_LUI, LIM_MINMOD, LIM_SUPERBEE, LIM_MUSCL, LIM_HARMONIC, LIM_VANALBADA,
+ LIM_UMIST, LIM_SPL13, LIM_SPLMAX12, LIM_SPLMAX13, LIM_KOREN, LIM_SMART, LIM_BSOU) = range(17)
+
+
+def _min(a, b):  # std::min(a, b): (b < a) ? b : a
+    return np.where(b < a, b, a)
+
+
+def _max(a, b):  # std::max(a, b): (a < b) ? b : a
+    return np.where(a < b, b, a)
+
+
+def _ratio(num, den):
+    """std::ratio<num, den> read as `num / den` with integer operands (FluxLimiterKernels.hpp:76-77)."""
+    return float(int(num / den) if num * den >= 0 else -(abs(num) // abs(den)))
+
+
+# (M, Alpha, Kappa) of the SPL / GPL kernels (FluxLimiterKernels.hpp:100-143)
+_SPL = {LIM_UMIST: ((2, 1), (1, 2)), LIM_SPL13: ((2, 1), (1, 3)),
+        LIM_SPLMAX12: ((2, 1), (1, 2)), LIM_SPLMAX13: ((2, 1), (1, 3))}
+_GPL = {LIM_KOREN: ((2, 1), (0, 1), (1, 3)), LIM_SMART: ((4, 1), (0, 1), (1, 2)), LIM_BSOU: ((1, 1), (0, 1), (-1, 1))}
+_KAPPA = {LIM_CDS: 1.0, LIM_QUICK: 0.5, LIM_CUI: 1. / 3., LIM_FROMM: 0.0, LIM_LUI: -1.0}
+
+
+def limiter(kind, r):
+    """psi(r) of the ratio-form kernels (FluxLimiterKernels.hpp:46-143), numpy, same operation order."""
+    z = np.zeros_like(r)
+    if kind == LIM_MINMOD:
+        return _max(z, _min(r, 1.0 + z))
+    if kind == LIM_SUPERBEE:
+        return _max(_max(z, _min(2. * r, 1.0 + z)), _min(r, 2.0 + z))
+    if kind == LIM_MUSCL:
+        return _max(z, _min(_min(2 * r, (r + 1) / 2.), 2.0 + z))
+    if kind == LIM_HARMONIC:
+        return (r + np.fabs(r)) / (r + 1)
+    if kind == LIM_VANALBADA:
+        return r * (r + 1) / (r * r + 1)
+    if kind in _SPL:
+        (mn, md), (kn, kd) = _SPL[kind]
+        m, kappa = _ratio(mn, md), _ratio(kn, kd)
+        a = 0.5 * (1 + kappa) * r + 0.5 * (1 - kappa)
+        b = 0.5 * (1 - kappa) * r + 0.5 * (1 + kappa)
+        if kind in (LIM_UMIST, LIM_SPL13):
+            inner = _min(_min(_min(m * r, a), b), m + z)
+        else:
+            inner = _min(_min(m * r, _max(a, b)), m + z)
+        return _max(z, inner)
+    (mn, md), (an, ad), (kn, kd) = _GPL[kind]
+    m, alpha, kappa = _ratio(mn, md), _ratio(an, ad), _ratio(kn, kd)
+    a = 0.5 * (1 + kappa) * r + 0.5 * (1 - kappa)
+    return _max(z, _min(_min((2 + alpha) * r, a), m + z))
+
+
+def fluxlim(kind, c2n, up, X, DXf, y, dxi):
+    """One branch of D1FluxLimiterImpl::eval (D1FluxLimiter.hpp:43-129): X(k) the node coordinate at
+    i + k, DXf(k) the spacing at i + k, y(k) the argument at i + k, dxi the spacing multiplying K."""
+    if c2n:
+        xc = lambda k: X(k) + 0.5 * DXf(k)  # noqa: E731  (cell centres)
+        k1 = -2 if up else -1
+    else:
+        xc = X
+        k1 = -1 if up else 0
+    x1, x2, x3 = xc(k1), xc(k1 + 1), xc(k1 + 2)
+    y1, y2, y3 = y(k1), y(k1 + 1), y(k1 + 2)
+    if up:
+        slop_u, slop_f = (y2 - y1) / (x2 - x1), (y3 - y2) / (x3 - x2)
+    else:
+        slop_u, slop_f = (y3 - y2) / (x3 - x2), (y2 - y1) / (x2 - x1)
+    if kind in _KAPPA:
+        kappa = _KAPPA[kind]
+        k = (1 + kappa) / 2. * slop_f + (1 - kappa) / 2. * slop_u
+        t = dxi * 0.5 * k
+    else:
+        r = slop_f / (slop_u + 1e-16)
+        t = dxi * 0.5 * limiter(kind, r) * slop_u
+    return y2 + t if up else y2 - t
 
 
 def _locs(nodes, fields, dim):
@@ -22,8 +100,17 @@ def _locs(nodes, fields, dim):
             locs.append([desc.loc[q] for q in range(dim)])
         elif k == CONST:
             locs.append(None)
-        elif k in (ADD, SUB, MUL, DIV):
+        elif k in (ADD, SUB, MUL, DIV, MIN, MAX, GT, LT, GE, LE):
             locs.append(list(locs[a]) if locs[a] is not None else (list(locs[b]) if locs[b] is not None else None))
+        elif k == COND:  # Conditional.hpp prepare: the properties of the selected value
+            locs.append(list(locs[b]) if locs[b] is not None else (list(locs[d]) if locs[d] is not None else None))
+        elif k == ABS:
+            locs.append(list(locs[a]) if locs[a] is not None else None)
+        elif k == FLUXLIM:  # D1FluxLimiter.hpp prepare: the location of the interpolated argument, flipped
+            l_ = list(locs[b]) if locs[b] is not None else None
+            if l_ is not None:
+                l_[d] = CORNER if l_[d] == CENTER else CENTER
+            locs.append(l_)
         else:
             l_ = list(locs[a]) if locs[a] is not None else None
             if l_ is not None:
@@ -118,6 +205,28 @@ def evaluate(nodes, root, fields, mesh, lo, hi):
             return ev(a, lo_, hi_) / ev(b, lo_, hi_)
         if k == NEG:
             return -ev(a, lo_, hi_)
+        if k in (MIN, MAX):
+            x_, y_ = np.asarray(ev(a, lo_, hi_), dtype=np.float64), np.asarray(ev(b, lo_, hi_), dtype=np.float64)
+            return _min(x_, y_) if k == MIN else _max(x_, y_)
+        if k == ABS:
+            return np.fabs(ev(a, lo_, hi_))
+        if k in (GT, LT, GE, LE):
+            x_, y_ = ev(a, lo_, hi_), ev(b, lo_, hi_)
+            c_ = {GT: np.greater, LT: np.less, GE: np.greater_equal, LE: np.less_equal}[k](x_, y_)
+            return c_.astype(np.float64)
+        if k == COND:  # CondOp::eval (Conditional.hpp:35-40): c ? t : f
+            return np.where(np.asarray(ev(a, lo_, hi_)) != 0, ev(b, lo_, hi_), ev(d, lo_, hi_))
+        if k == FLUXLIM:  # D1FluxLimiterImpl::eval: u[i] > 0. ? upwind : downwind
+            c2n = locs[b] is not None and locs[b][d] == CENTER
+            kind = int(v) % 32
+            y = lambda kk: np.asarray(ev(b, *sh(lo_, hi_, d, kk)), dtype=np.float64)  # noqa: E731
+            Xf = lambda kk: X(d, lo_, hi_, kk)  # noqa: E731
+            Df = lambda kk: DX(d, lo_, hi_, kk)  # noqa: E731
+            kd_up = -1 if c2n else 0
+            up = fluxlim(kind, c2n, True, Xf, Df, y, DX(d, lo_, hi_, kd_up))
+            dn = fluxlim(kind, c2n, False, Xf, Df, y, DX(d, lo_, hi_, 0))
+            uu = np.asarray(ev(a, lo_, hi_), dtype=np.float64)
+            return np.where(uu > 0., up, dn)
         corner = locs[a] is not None and locs[a][d] == CORNER
         if k == D2C:  # D2SecondOrderCentered::eval (D2SecondOrderCentered.hpp:160-171)
             l_ = ev(a, *sh(lo_, hi_, d, -1))

@@ -295,3 +295,15 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* Thread count of later parallel regions (the CPU-baseline timings at one
+ * thread and at all threads); n <= 0 restores the count the process started with. */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    static int initial = 0;
+    if (!initial) initial = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : initial);
+#else
+    (void)n;
+#endif
+}

@@ -24,6 +24,7 @@ void orc_update_padding(const opf_field_desc* f, const orc_mesh* m, double* u);
 void orc_ftcs(const opf_field_desc* f, const orc_mesh* m, double* u, double coef, int nsteps, int nthreads);
 double orc_reduce(const opf_field_desc* f, const double* u, int red, const int rs[3], const int re[3]);
 int orc_num_threads(void);
+void orc_set_threads(int n);
 long orc_csr_poisson(const opf_field_desc* f, const orc_mesh* m, double c, int pin_last, long* ptr, long* col,
                      double* val, double* rhs);
 void orc_poisson_apply(int dim, const int n[3], const double h[3], const int bc[3][2], double sigma, const double* x,

@@ -91,18 +91,39 @@ int main() {
     }
     {  // NeumEqnTest::ManualUnifiedSolve (pinValue, mean removed)
         Case c(BCType::Neum, 33, 2 * PI, 0.0, 0.0);
-        c.p_true.initBy([](auto&& x) { return std::cos(x[0]) * std::cos(x[1]); });
-        auto ave = rangeReduce(c.p.assignableRange, SumOp{}, c.p_true);
-        c.p_true -= ave / c.p.assignableRange.count();
+        auto& p = c.p;
+        auto& p_true = c.p_true;
+        // NeumEqnTest.cpp:60-64 and 118-124, spelled as the reference spells them
+        p_true.initBy([&](auto&& x) { return std::cos(x[0]) * std::cos(x[1]); });
+        auto ave_p = rangeReduce(
+                p.assignableRange, [](auto&& a, auto&& b) { return a + b; },
+                [&](auto&& idx) { return p_true[idx]; });
+        p_true -= ave_p / p.assignableRange.count();
         c.reset_case(0.5, 0.5, 1);
         StructSolverParams<StructSolverType::GMRES> params;
         params.tol = 1e-10;
-        params.pinValue = true;
         StructSolverParams<StructSolverType::PFMG> p_params;
-        auto st = Solve(c.poisson_eqn(), c.p, params, p_params);
-        ave = rangeReduce(c.p.assignableRange, SumOp{}, c.p);
-        c.p -= ave / c.p.assignableRange.count();
+        p = 0.;
+        params.pinValue = true;
+        auto st = Solve(c.poisson_eqn(), p, params, p_params);
+        ave_p = rangeReduce(
+                p.assignableRange, [](auto&& a, auto&& b) { return a + b; }, [&](auto&& idx) { return p[idx]; });
+        p -= ave_p / p.assignableRange.count();
         report("NeumEqnTest.ManualUnifiedSolve", c.max_rel_err(), 5e-8, st);
+        // the functor may return an expression of fields; the operator may be a max or a min
+        const double lo = rangeReduce(p.assignableRange, [](auto&& a, auto&& b) { return std::min(a, b); },
+                                      [&](auto&& i) { return p[i] - p_true[i]; });
+        const double hi = rangeReduce(p.assignableRange, [](auto&& a, auto&& b) { return a > b ? a : b; },
+                                      [&](auto&& i) { return p[i] - p_true[i]; });
+        double elo = 1e300, ehi = -1e300;
+        for (int j = p.assignableRange.start[1]; j < p.assignableRange.end[1]; ++j)
+            for (int i = p.assignableRange.start[0]; i < p.assignableRange.end[0]; ++i) {
+                const double v = p.evalAt(DS::MDIndex<2>{i, j}) - p_true.evalAt(DS::MDIndex<2>{i, j});
+                elo = std::min(elo, v);
+                ehi = std::max(ehi, v);
+            }
+        EqnSolveState none{};
+        report("rangeReduce.ExpressionMinMax", std::abs(lo - elo) + std::abs(hi - ehi), 0.0, none);
     }
     {  // PeriodicEqnTest::ManualUnifiedSolve
         Case c(BCType::Periodic, 33, 2 * PI, 0.0, 0.0);
@@ -215,6 +236,81 @@ int main() {
         std::printf("%s EqnSetTest.SimplePoisson%s_2Eqn: |p-p2| %.1e, err %.2e, iters %d, relres %.2e\n",
                     ok ? "PASS" : "FAIL", neum ? "_Neum" : "", diff, err, st.niter, st.relerr);
         failures += !ok;
+    }
+    {  // examples/EqnHandler/EqnHandlerAMGCL.cpp:44-48 on the 4x4 system of AMGCLTest.cpp /
+       // CSRMatrixGeneratorTest.cpp:57-89, whose solution has x[0] = -0.375, x[5] = -1.125
+        auto m = MeshBuilder<Mesh>().newMesh(5, 5).setMeshOfDim(0, 0., 4.).setMeshOfDim(1, 0., 4.).build();
+        auto u = ExprBuilder<Field>().setName("u").setMesh(m)
+                         .setBC(0, DimPos::start, BCType::Dirc, 0.).setBC(0, DimPos::end, BCType::Dirc, 0.)
+                         .setBC(1, DimPos::start, BCType::Dirc, 0.).setBC(1, DimPos::end, BCType::Dirc, 0.)
+                         .setLoc({LocOnMesh::Center, LocOnMesh::Center}).setExt(1).build();
+        u = 0.;
+        using Solver = DeviceIJSolver;
+        auto handler = makeEqnSolveHandler<Solver>(
+                [&](auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; }, u,
+                DS::MDRangeMapper<2> {u.assignableRange});
+        auto st = handler->solve();
+        const double e0 = std::abs(u[DS::MDIndex<2>{0, 0}] + 0.375), e5 = std::abs(u[DS::MDIndex<2>{1, 1}] + 1.125);
+        report("EqnHandlerAMGCL.MDRangeMapper", std::max(e0, e5), 1e-7, st);  // AMGCL's default tol 1e-8
+        // with IJSolverParams: a tighter tolerance
+        IJSolverParams<Solver> ip;
+        ip.p.solver.tol = 1e-12;
+        u = 0.;
+        auto h2 = makeEqnSolveHandler<Solver>(
+                [&](auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; }, u,
+                DS::MDRangeMapper<2> {u.assignableRange}, ip);
+        st = h2->solve();
+        report("EqnHandlerAMGCL.IJSolverParams",
+               std::max(std::abs(u[DS::MDIndex<2>{0, 0}] + 0.375), std::abs(u[DS::MDIndex<2>{1, 1}] + 1.125)), 1e-11,
+               st);
+        // the set form (AMGCLEqnSolveHandler.hpp:184-195): two equations, two targets
+        auto v = u;
+        v.name = "v";
+        u = 0.;
+        v = 0.;
+        auto g0 = [&](auto&& e, auto&&) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; };
+        auto g1 = [&](auto&&, auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; };
+        std::vector<IJSolverParams<Solver>> sp(2);
+        sp[0].p.solver.tol = 1e-12;
+        auto hs = makeEqnSolveHandler<Solver>(g0, g1, u, v, DS::ColoredMDRangeMapper<2> {u.assignableRange, v.assignableRange},
+                                              sp);
+        st = hs->solve();
+        double es = 0;
+        for (auto* f : {&u, &v})
+            es = std::max({es, std::abs((*f)[DS::MDIndex<2>{0, 0}] + 0.375), std::abs((*f)[DS::MDIndex<2>{1, 1}] + 1.125)});
+        report("EqnHandlerAMGCL.EquationSet", es, 1e-11, st);
+        // parameters the engine cannot honour are refused, not ignored
+        int refused = 0;
+        StructSolverParams<StructSolverType::PCG> bad;
+        bad.absTol = 1e-12;
+        try {
+            Solve([&](auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; }, u, bad);
+        } catch (const EngineError& x) {
+            refused += x.code == OPF_ERR_UNSUPPORTED;
+        }
+        StructSolverParams<StructSolverType::PCG> ok_p;
+        StructSolverParams<StructSolverType::PFMG> gs;
+        gs.relaxType = 2;
+        try {
+            Solve([&](auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; }, u, ok_p,
+                  gs);
+        } catch (const EngineError& x) {
+            refused += x.code == OPF_ERR_UNSUPPORTED;
+        }
+        // the knobs of the reference's examples (LidDriven2D.cpp:47-48) are accepted
+        StructSolverParams<StructSolverType::PFMG> pf;
+        pf.useZeroGuess = true;
+        pf.relaxType = 1;
+        pf.rapType = 0;
+        pf.numPreRelax = 1;
+        pf.numPostRelax = 1;
+        pf.skipRelax = 0;
+        ok_p.tol = 1e-12;
+        u = 0.;
+        st = Solve([&](auto&& e) { return d2x<D2SecondOrderCentered>(e) + d2y<D2SecondOrderCentered>(e) == 1.0; }, u, ok_p,
+                   pf);
+        report("SolverParams.RefusedAndAccepted",
+               (refused == 2 ? 0.0 : 1.0) + std::abs(u[DS::MDIndex<2>{0, 0}] + 0.375), 1e-11, st);
     }
     return failures ? 1 : 0;
 }

@@ -40,6 +40,8 @@ int main(int argc, char** argv) {
     auto t1 = std::chrono::steady_clock::now();
     std::printf("Elapsed time: %lld ms\n",
                 (long long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count());
+    std::printf("{\"n\": %d, \"steps\": %d, \"s_loop_and_read\": %.6e}\n", n, steps,
+                std::chrono::duration<double>(t1 - t0).count());
     std::printf("Center val: %.17g\n", u.evalAt(DS::MDIndex<2>{n / 2, n / 2}));
     // the same update through the generic interpreter path (no pattern match): u2 = u2 + (d2x + d2y) * c
     auto u2 = ExprBuilder<Field>().setName("u2").setMesh(mesh)
@@ -53,7 +55,7 @@ int main(int argc, char** argv) {
         Utils::RawBinaryOStream os(argv[4]);
         os << Utils::TimeStamp(steps * dt) << u;
     }
-    if (argc > 1) {
+    if (argc > 1 && argv[1][0]) {
         FILE* f = std::fopen(argv[1], "wb");
         std::fwrite(h.data(), sizeof(double), h.size(), f);
         const auto& h2 = u2.hostData();

@@ -89,11 +89,38 @@ def test_neum_eqn_pinned(ctx):
     st = of.Solve(eqn, p, of.StructSolverParams(tol=1e-10, pinValue=True))
     a = p.numpy()
     assert a[1, 1] == 0.0  # the first unknown is pinned at 0 (HYPREEqnSolveHandler.hpp:145-163)
-    ave = of.rangeReduce(p, "sum") / (32 * 32)
+    # the reference's spelling (NeumEqnTest.cpp:121-124): a binary operator and an index functor
+    ave = of.rangeReduce(p.assignableRange, lambda a, b: a + b, lambda idx: p[idx]) / (32 * 32)
+    assert ave == of.rangeReduce(p, "sum") / (32 * 32)
     p.assign(p - ave)
     err = _check(p, pt, 5e-8)
     print(f"Neum pinned: iters {st.niter} relerr {st.relerr:.2e} err {err:.2e}", flush=True)
     assert err <= 5e-8
+    # an expression functor and max / min operators, against numpy on the downloaded values
+    diff = p.numpy()[1:-1, 1:-1] - pt.numpy()[1:-1, 1:-1]
+    assert of.rangeReduce(p.assignableRange, max, lambda i: p[i] - pt[i]) == diff.max()
+    assert of.rangeReduce(p.assignableRange, min, lambda i: p[i] - pt[i]) == diff.min()
+    with pytest.raises(of._lib.OpfError):
+        of.rangeReduce(p.assignableRange, lambda a, b: a * b, lambda i: p[i])
+
+
+def test_solver_params_refused_or_honoured(ctx):
+    """HYPRE knobs that would change the stopping test or the smoother kind are refused;
+    the PFMG settings of the reference's examples (LidDriven2D.cpp:47-48) are accepted."""
+    mesh, p, pt, eqn = _setup(ctx, BC.Dirc, 1.0, lambda x: x[0] * (1 - x[0]) * x[1] * (1 - x[1]), None, 1000)
+    for bad in (of.StructSolverParams(tol=1e-10, absTol=1e-12), of.StructSolverParams(tol=1e-10, twoNorm=0),
+                of.StructSolverParams(tol=1e-10, relChange=1)):
+        with pytest.raises(of._lib.OpfError) as ei:
+            of.Solve(eqn, p, bad)
+        assert ei.value.code == of._lib.OPF_ERR_UNSUPPORTED
+    with pytest.raises(of._lib.OpfError):
+        of.Solve(eqn, p, of.StructSolverParams(tol=1e-10), of.StructSolverParams(type=3, relaxType=2))
+    pf = of.StructSolverParams(type=3, useZeroGuess=True, relaxType=1, rapType=0, numPreRelax=1, numPostRelax=1,
+                               skipRelax=0)
+    p.assign(5.0)  # useZeroGuess on the solver: starts from 0 whatever p holds
+    st = of.Solve(eqn, p, of.StructSolverParams(tol=1e-10, useZeroGuess=True, twoNorm=1), pf)
+    assert st.relerr <= 1e-10
+    assert _check(p, pt, 1e-8) <= 1e-8
 
 
 def test_periodic_eqn(ctx):

@@ -308,6 +308,56 @@ def test_sweep_3d_zmarch_bitexact(ctx, engine, op):
     np.testing.assert_array_equal(got, ref)
 
 
+def test_sweep_zextent_two_cells_per_thread(ctx, engine):
+    """A z extent that is even but not a multiple of 4 (18 cells): the compiled kernel computes
+    two cells along z per thread (plane-stride load offsets, lo + OP stores); bit-exact with the
+    oracle on both engines, in place and into another field."""
+    u_spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
+    w_spec = (1, [(DIRC, DIRC)] * 3, [(0.5, -0.5), (0.0, 1.0), (0.25, 0.75)], (1, 1))
+    D2, UP = of.D2SecondOrderCentered, of.D1FirstOrderBiasedUpwind
+
+    def fn(u, w):
+        return u + 0.01 * (of.d2x(D2, u) + of.d2z(D2, u)) - 0.02 * w * of.dz(UP, u)
+    for out in (None, w_spec):
+        got, ref = _sweep_case(ctx, fn, [u_spec, w_spec], [15, 11, 19], out, SEED + 13,
+                               ranges=[(0.0, 1.0), (0.0, 1.0), (0.0, 1.7)])
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("mode", [0, 2], ids=["interp", "jit"])
+def test_sweep_new_constant_reuses_program_and_kernel(ctx, mode):
+    """Scalar constants are launch arguments: the same expression with a different dt (as an
+    adaptive time step does) runs the program and kernel of the first sweep, with no new
+    compile, kernel load or program upload, and every result equals the oracle's."""
+    ctx.set_jit(mode)
+    try:
+        mesh, om = _mesh(ctx, [23, 17, 13])
+        spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
+        u = build_field(ctx, mesh, 3, spec[0], spec[1], spec[2], ext=spec[3])
+        a = oracle.update_padding(u.desc, om, random_storage(u, SEED + 14, 0.5, 1.5))
+        D2 = of.D2SecondOrderCentered
+        before = None
+        for k, dt in enumerate((1e-3, 2.5e-3, 7e-4, 1.9e-3, -3e-3)):
+            u.upload(a)
+            # a structure no other test sweeps (the 0.375 factor), so its first sweep builds it here
+            expr = u + dt * (0.375 * of.d2x(D2, u) + of.d2y(D2, u) + of.d2z(D2, u))
+            u.assign(expr)
+            nodes, flds, root = expr.flatten()
+            ref = oracle.assign(nodes, root, [(a, u.desc)], om, a, u.desc, 0,
+                                update_padding=lambda r: oracle.update_padding(u.desc, om, r))
+            np.testing.assert_array_equal(u.numpy(), ref)
+            if k == 0:
+                before = ctx.cache_stats()
+        after = ctx.cache_stats()
+        print("cache counters after the first sweep", before, "after five", after, flush=True)
+        assert after["programs_uploaded"] == before["programs_uploaded"]
+        assert after["jit_compiles"] == before["jit_compiles"]
+        assert after["jit_loaded"] == before["jit_loaded"]
+        u.close()
+    finally:
+        ctx.set_jit(1)
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_sweep_engines_agree_at_size(ctx, dtype):
     """A 3-D sweep of 96^3 cells (above the size from which the default engine compiles
@@ -387,3 +437,117 @@ def test_context_on_caller_stream(ctx):
         u.close()
     assert hip.hipStreamDestroy(raw) == 0
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+LIMITERS = [of.D1Central, of.D1QUICK, of.D1CUI, of.D1Fromm, of.D1LinearUpwind, of.D1Minmod, of.D1Superbee,
+            of.D1MUSCL, of.D1Harmonic, of.D1Albada, of.D1UMIST, of.D1SPL13, of.D1SPLMax12, of.D1SPLMax13,
+            of.D1KorenLimitedCUI, of.D1SMART, of.D1BSOU]
+
+
+def _fluxlim_case(ctx, fn, specs, nodes, out_spec, seed):
+    """Like _sweep_case, with the fields after the first velocities of both signs (zero at some cells)."""
+    mesh, om = _mesh(ctx, nodes)
+    dim = len(nodes)
+    fs, arrs = [], []
+    for k, (loc, bc, bcv, ext) in enumerate(specs):
+        f = build_field(ctx, mesh, dim, loc, bc, bcv, ext=ext, name=f"f{k}")
+        lo, hi = (0.5, 1.5) if k == 0 else (-1.0, 1.0)
+        raw = random_storage(f, seed + k, lo, hi)
+        if k >= 1:
+            raw.reshape(-1)[::11] = 0.0  # u = 0 takes the downwind branch (u > 0. fails)
+        a = oracle.update_padding(f.desc, om, raw)
+        f.upload(a)
+        fs.append(f)
+        arrs.append(a)
+    loc, bc, bcv, ext = out_spec
+    out = build_field(ctx, mesh, dim, loc, bc, bcv, ext=ext, name="out")
+    out_arr = oracle.update_padding(out.desc, om, random_storage(out, seed + 99, 0.5, 1.5))
+    out.upload(out_arr)
+    expr = fn(*fs)
+    nodes_, flds, root = expr.flatten()
+    slot_arrays = [(arrs[next(i for i, g in enumerate(fs) if g is f)], f.desc) for f in flds]
+    ref = oracle.assign(nodes_, root, slot_arrays, om, out_arr, out.desc, 0,
+                        update_padding=lambda r: oracle.update_padding(out.desc, om, r))
+    out.assign(expr)
+    return out.numpy(), ref
+
+
+@pytest.mark.parametrize("k", range(len(LIMITERS)))
+def test_sweep_fluxlim_cen2cor_bitexact(ctx, engine, k):
+    """Conservative advection step e - dt * (dx(u * intp(u, e)) + dy(...)) with the limiter's
+    Cen2Cor interpolation (D1FluxLimiter.hpp:43-158) in x and y, u of both signs; periodic,
+    two ghost layers; equal to the oracle bit for bit on both engines."""
+    K, D1 = LIMITERS[k], of.D1FirstOrderCentered
+    e_spec = (1, [(PER, PER), (PER, PER)], None, (2, 2))
+    ux_spec = ([0, 1], [(PER, PER), (PER, PER)], None, (2, 2))  # face velocities (MAC)
+    uy_spec = ([1, 0], [(PER, PER), (PER, PER)], None, (2, 2))
+
+    def fn(e, ux, uy):
+        fx = of.d1IntpCenterToCorner(0, ux, e, kernel=K)
+        fy = of.d1IntpCenterToCorner(1, uy, e, kernel=K)
+        return e - 0.001 * (of.dx(D1, ux * fx) + of.dy(D1, uy * fy))
+    got, ref = _fluxlim_case(ctx, fn, [e_spec, ux_spec, uy_spec], [41, 29], e_spec, SEED + 21 + k)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("k", [1, 5, 6, 9, 12, 16])
+def test_sweep_fluxlim_cor2cen_3d_bitexact(ctx, engine, k):
+    """Cor2Cen interpolation of a corner-located field to centres (the branch reads i-1..i+2),
+    in z on a 3-D grid (the compiled sweep's z ring), Dirichlet / Neumann ghosts."""
+    K = LIMITERS[k]
+    e_spec = ([1, 1, 0], [(DIRC, NEUM), (NEUM, DIRC), (DIRC, DIRC)], [(0.2, 0.1), (0.3, -0.1), (0.5, -0.5)], (2, 2))
+    u_spec = ([1, 1, 1], [(NEUM, NEUM)] * 3, [(0.0, 0.0)] * 3, (2, 2))
+    o_spec = ([1, 1, 1], [(NEUM, NEUM)] * 3, [(0.0, 0.0)] * 3, (1, 1))
+
+    def fn(e, u):
+        return of.d1IntpCornerToCenter(2, u, e, kernel=K) * 2.0 - u
+    got, ref = _fluxlim_case(ctx, fn, [e_spec, u_spec], [13, 11, 21], o_spec, SEED + 41 + k)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_sweep_conditional_minmax(ctx, engine):
+    """conditional / max / min / comparisons (Conditional.hpp, MinMax.hpp, Compare.hpp) in a
+    sweep, as the level-set example writes its upwind terms."""
+    spec_e = (1, [(NEUM, NEUM), (NEUM, NEUM)], [(0.0, 0.0), (0.0, 0.0)], (3, 3))
+    spec_u = (1, [(NEUM, NEUM), (NEUM, NEUM)], [(0.0, 0.0), (0.0, 0.0)], (3, 3))
+    DU, DD = of.D1WENO53Upwind, of.D1WENO53Downwind
+
+    def fn(p, u):
+        return (p - 0.01 * u * of.conditional(u > 0.0, of.dx(DD, p), of.dx(DU, p))
+                + of.max(-of.min(0.0, of.dy(DU, p)), of.max(of.dy(DD, p), 0.0)) * 1e-3
+                + of.conditional(p >= u, p, u) * of.conditional(u <= 0.25, 1.0, -1.0) * 1e-4
+                + of.conditional(u < p, 1.0, 0.0) * 1e-5)
+    got, ref = _fluxlim_case(ctx, fn, [spec_e, spec_u], [37, 23], spec_e, SEED + 60)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n,steps,bc", [
+    ((40, 36, 20), 13, [(DIRC, NEUM), (NEUM, DIRC), (DIRC, DIRC)]),
+    ((70, 33, 67), 14, [(NEUM, NEUM), (PER, PER), (DIRC, NEUM)]),
+    ((65, 97, 9), 8, [(PER, PER), (DIRC, DIRC), (NEUM, DIRC)]),
+    ((6, 5, 7), 7, [(DIRC, DIRC), (DIRC, NEUM), (PER, PER)]),
+    ((32, 32, 32), 11, [(DIRC, NEUM), (NEUM, NEUM), (DIRC, DIRC)]),
+    ((64, 32, 16), 9, [(NEUM, DIRC), (DIRC, DIRC), (NEUM, NEUM)]),
+], ids=["dirc-neum", "per-y", "per-x", "tiny", "pow2-equal", "pow2-unequal"])
+def test_ftcs3d_nonperiodic_blocked_bitexact(ctx, n, steps, bc):
+    """3-D FTCS with Dirichlet / Neumann sides runs 3 steps per launch, rebuilding the ghosts of
+    the intermediate levels inside the launch; equal to the oracle's step-by-step
+    FieldAssigner + updatePadding sequence bit for bit (ragged tiles, non-zero boundary values,
+    a z extent below the chunk, a grid smaller than the halo)."""
+    mesh, om = _mesh(ctx, [k + 1 for k in n], [(0.0, 1.0), (0.0, 1.0), (0.0, 1.0)])
+    bcv = [(0.4 if t0 == DIRC else -0.3 if t0 == NEUM else 0.0, -0.2 if t1 == DIRC else 0.6 if t1 == NEUM else 0.0)
+           for (t0, t1) in bc]
+    u = build_field(ctx, mesh, 3, 1, bc, bcv)
+    init = oracle.update_padding(u.desc, om, random_storage(u, SEED + 70 + steps))
+    u.upload(init)
+    launches, kmax = u.ftcs_plan(steps)
+    assert kmax == 3
+    coef = 0.1 / max(n) ** 2
+    u.ftcs(coef, steps)
+    ref = oracle.ftcs(u.desc, om, init, coef, steps)
+    np.testing.assert_array_equal(u.numpy(), ref)
+    # and equal to single steps on the device
+    u.upload(init)
+    u.ftcs_blocked(coef, steps, 1)
+    np.testing.assert_array_equal(u.numpy(), ref)
+    u.close()

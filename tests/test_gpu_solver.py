@@ -173,9 +173,10 @@ def test_full_size_fp64_vs_mixed(ctx, n):
         rel = np.linalg.norm(b - oracle.poisson_apply(n, h, bcs, x)) / nb
         print(f"{n} {name}: oracle-recomputed relres {rel:.3e}", flush=True)
         assert rel <= 1.01e-10
-    # (the oracle's FP64 residual stagnates near 1e-11..1e-12 at these sizes, so it solves to the
-    # configuration's 1e-10 too; the two solutions agree to ~3e-12)
-    ref, it, rel_ref = oracle.pcg_mg(n, h, bcs, b, tol=1e-10)
+    # (the oracle's FP64 residual stagnates near 1e-11..1e-12 at these sizes and its stopping test
+    # uses the recurrence residual, so it is asked for 1e-11 and checked at the configuration's
+    # 1e-10; the two solutions agree to ~6e-12)
+    ref, it, rel_ref = oracle.pcg_mg(n, h, bcs, b, tol=1e-11)
     err = np.max(np.abs(x64 - ref)) / np.max(np.abs(ref))
     print(f"{n}: oracle solve {it} iterations relres {rel_ref:.2e}; max |x64 - x_oracle| / max|x| {err:.3e}",
           flush=True)
