@@ -174,8 +174,11 @@ def cpp_programs(n):
                        timeout=300)
     if r.returncode == 0:
         j = json.loads([x for x in r.stdout.splitlines() if x.startswith("{")][0])
-        out["cpp_ftcs2d_256_fp64_1000steps"] = {"ms_loop_and_read": j["s_loop_and_read"] * 1e3,
-                                                "Mcell_updates_per_s": 256 * 256 * 1000 / j["s_loop_and_read"] / 1e6}
+        out["cpp_ftcs2d_256_fp64_1000steps"] = {
+            "ms_loop_and_read": j["s_loop_and_read"] * 1e3, "ms_first_step": j["s_first_step"] * 1e3,
+            "ms_steps_2_to_1000_and_read": j["s_steps_2_to_n_and_read"] * 1e3,
+            "Mcell_updates_per_s_steps_2_to_1000": 256 * 256 * 999 / j["s_steps_2_to_n_and_read"] / 1e6,
+            "note": "the first step also allocates the second time level and loads the kernels (first use)"}
     return out
 
 

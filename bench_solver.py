@@ -8,6 +8,8 @@ cfg5: served Poisson solves of mixed sizes at tolerance 1e-8 -- solves/s closed
 loop and p50/p99 latency open loop, next to the plain FP64 solver one at a time.
 eqn_sets: SolveEqns on ten equal-operator equations (batched) vs ten Solve()
 calls, and a coupled two-equation set.
+eqn_setup: per-solve setup of a non-static handler (pads and multigrid levels
+built on the device) on a variable-coefficient operator at 1024^2 and 256^3.
 """
 import time
 
@@ -291,10 +293,73 @@ def eqn_sets(of, ctx, n=256, neq=10, reps=3):
                                                      "relres": cst.relerr}}
 
 
-def solver_extra(of, ctx, ev, cpu=False):
+def eqn_setup(of, ctx, reps=3):
+    """Per-solve setup of a non-static handler (HYPREEqnSolveHandler::solve with staticMat
+    false: generateAb on every solve, HYPREEqnSolveHandler.hpp:125-166,190-209) on the
+    DircEqnTest operator d/dx(dp/dx / r) + d/dy(dp/dy / r) (r = 1 + 9 x y, a variable
+    coefficient) at 1024^2 and 256^3: a non-static solve (pads, levels built on the
+    device, solve) beside a static one (right-hand side only, same solve); the difference
+    is the setup. FP64, tol 1e-8, from the previous solution (the solve does the same
+    iterations both ways)."""
+    D1, C2N = of.D1FirstOrderCentered, of.d1IntpCenterToCorner
     out = {}
-    out.update(cfg3(of, ctx, ev, cpu))
-    out.update(cfg4(of, ctx, ev))
-    out.update(cfg5(of, ctx, ev))
-    out.update(eqn_sets(of, ctx))
+    for n in ([1024, 1024], [256, 256, 256]):
+        dim = len(n)
+        mb = of.MeshBuilder(ctx).newMesh(*[k + 1 for k in n])
+        for d in range(dim):
+            mb.setMeshOfDim(d, 0.0, 1.0)
+        mesh = mb.build()
+
+        def field(name, bc, g):
+            b = of.ExprBuilder().setMesh(mesh).setName(name).setLoc([of.LocOnMesh.Center] * dim).setExt(1)
+            for d in range(dim):
+                b.setBC(d, of.DimPos.start, bc, g).setBC(d, of.DimPos.end, bc, g)
+            return b.build()
+        p, r, f = field("p", of.BCType.Dirc, 0.0), field("r", of.BCType.Neum, 0.0), field("f", of.BCType.Dirc, 0.0)
+        r.initBy(lambda x: 1.0 + 9.0 * x[0] * x[1])
+        f.initBy(lambda x: np.sin(3 * x[0]) * np.cos(2 * x[1]) + x[0])
+        ops = [of.dx, of.dy, of.dz][:dim]
+
+        def eqn(e):
+            lhs = None
+            for d, op in enumerate(ops):
+                t = op(D1, op(D1, e) / C2N(d, r))
+                lhs = t if lhs is None else lhs + t
+            return f == lhs
+        res = {}
+        for static in (False, True):
+            h = of.makeEqnSolveHandler(eqn, p, of.PrecondStructSolver(of.StructSolverParams(tol=1e-8,
+                                                                                              staticMat=static)))
+            p.assign(0.0)
+            h.solve()  # first solve: full evaluation and build either way
+            ts = []
+            for _ in range(reps):
+                ctx.sync()
+                t0 = time.perf_counter()
+                st = h.solve()
+                ctx.sync()
+                ts.append(time.perf_counter() - t0)
+            res[static] = (float(np.median(ts)), st)
+            h.close()
+        key = "x".join(map(str, n))
+        out[f"eqn_setup_varcoef_{key}"] = {
+            "nonstatic_s_per_solve": res[False][0], "static_s_per_solve": res[True][0],
+            "setup_s": res[False][0] - res[True][0], "iters": res[False][1].niter, "relres": res[False][1].relerr,
+            "note": "non-static: pads + device level construction + solve; static: right-hand side + solve"}
+        for q in (p, r, f):
+            q.close()
+    return out
+
+
+def solver_extra(of, ctx, ev, cpu=False):
+    """Every leg above; a leg that raises is reported as its error (the bench line still prints)."""
+    out = {}
+    legs = (("cfg3", lambda: cfg3(of, ctx, ev, cpu)), ("cfg4", lambda: cfg4(of, ctx, ev)),
+            ("cfg5", lambda: cfg5(of, ctx, ev)), ("eqn_sets", lambda: eqn_sets(of, ctx)),
+            ("eqn_setup", lambda: eqn_setup(of, ctx)))
+    for name, leg in legs:
+        try:
+            out.update(leg())
+        except Exception as e:  # noqa: BLE001
+            out[f"{name}_error"] = repr(e)[:400]
     return out

@@ -122,7 +122,33 @@ enum {
     OPF_N_LE = 23,        /* a <= b                                                  */
     /* conditional(c, t, f) (Operator/Conditional.hpp:22-60): a = c, b = t, dim = the node index
      * of f; c != 0 selects t                                                          */
-    OPF_N_COND = 24
+    OPF_N_COND = 24,
+    /* f(a) for the <cmath> function dim = OPF_FN_* (Arithmetic/AMDS.hpp:52-90, DEFINE_UNIFUNC);
+     * OPF_FN_SMOOTH_DELTA / OPF_FN_SMOOTH_HEAVISIDE take value = eps (Math::smoothDelta /
+     * smoothHeviside, Math/Function/Numeric.hpp:86-96, the functors of UniOpAdaptor)       */
+    OPF_N_FUNC1 = 25,
+    /* f(a, b) for the <cmath> function dim = OPF_FN2_* (Arithmetic/AMDS.hpp:39-50, DEFINE_BINFUNC) */
+    OPF_N_FUNC2 = 26,
+    /* conv(a, kernel) (Convolution/Convolution.hpp:20-95): the kernel's K = n0 n1 n2 weights
+     * are K consecutive OPF_N_CONST nodes from index b, x-fastest; dim = n0 | n1 << 8 |
+     * n2 << 16 (odd extents, 1 in unused dims); the sum runs x-fastest over the window,
+     * weight * value, the first term alone and each next one added (rangeReduce_s)   */
+    OPF_N_CONV = 27
+};
+/* functions of OPF_N_FUNC1 */
+enum {
+    OPF_FN_EXP = 0, OPF_FN_EXP2 = 1, OPF_FN_EXPM1 = 2, OPF_FN_LOG = 3, OPF_FN_LOG10 = 4, OPF_FN_LOG2 = 5,
+    OPF_FN_LOG1P = 6, OPF_FN_SQRT = 7, OPF_FN_CBRT = 8, OPF_FN_SIN = 9, OPF_FN_COS = 10, OPF_FN_TAN = 11,
+    OPF_FN_ASIN = 12, OPF_FN_ACOS = 13, OPF_FN_ATAN = 14, OPF_FN_SINH = 15, OPF_FN_COSH = 16, OPF_FN_TANH = 17,
+    OPF_FN_ASINH = 18, OPF_FN_ACOSH = 19, OPF_FN_ATANH = 20, OPF_FN_ERF = 21, OPF_FN_ERFC = 22,
+    OPF_FN_TGAMMA = 23, OPF_FN_LGAMMA = 24, OPF_FN_CEIL = 25, OPF_FN_FLOOR = 26, OPF_FN_TRUNC = 27,
+    OPF_FN_ROUND = 28, OPF_FN_NEARBYINT = 29, OPF_FN_RINT = 30, OPF_FN_LOGB = 31,
+    OPF_FN_SMOOTH_DELTA = 32, OPF_FN_SMOOTH_HEAVISIDE = 33
+};
+/* functions of OPF_N_FUNC2 */
+enum {
+    OPF_FN2_POW = 0, OPF_FN2_FMOD = 1, OPF_FN2_REMAINDER = 2, OPF_FN2_FDIM = 3, OPF_FN2_HYPOT = 4,
+    OPF_FN2_ATAN2 = 5, OPF_FN2_COPYSIGN = 6, OPF_FN2_NEXTAFTER = 7
 };
 /* limiter kernels of OPF_N_FLUXLIM (Interpolator/FluxLimiterKernels.hpp); the kappa schemes use
  * the linear form of D1FluxLimiter.hpp, the others the ratio form                    */
@@ -158,7 +184,11 @@ int opf_ctx_set_stream(opf_ctx* ctx, void* stream);
  * ahead of time: 0 = the stack-program interpreter, 1 = (default) a kernel
  * generated from the expression and compiled at run time (hipRTC, cached in the
  * process and under $OPF_JIT_CACHE or ~/.cache/opflow_amd/jit) for sweeps of at
- * least 2^18 cells, 2 = that kernel for every sweep. Both give the same bits. */
+ * least 2^18 cells, 2 = that kernel for every sweep. The compiled kernel has
+ * two forms, chosen by grid size: fields staged in LDS while a workgroup marches
+ * z (grids of at least 512 64 x 16-cell tiles x 8-plane chunks) or loads straight
+ * from HBM; 3 / 4 = the compiled kernel for every sweep in the staged / direct
+ * form (tuning and tests). Every mode gives the same bits. */
 int opf_ctx_set_jit(opf_ctx* ctx, int mode);
 /* Counters of the engine's caches (no counterpart in the reference):
  * stats[0] hipRTC compilations in this process, [1] compiled sweep kernels

@@ -86,7 +86,8 @@ class Context:
 
     def set_jit(self, mode):
         """Engine of expression sweeps: 0 interpreter, 1 compiled kernels for >= 2^18 cells
-        (default), 2 compiled kernels always (same bits either way)."""
+        (default), 2 compiled kernels always, 3 / 4 compiled kernels always in the LDS-staged /
+        direct form (same bits in every mode)."""
         L.check(self.lib.opf_ctx_set_jit(self.h, int(mode)))
 
     def cache_stats(self):
@@ -186,6 +187,13 @@ _N_FIELD, _N_CONST, _N_ADD, _N_SUB, _N_MUL, _N_DIV, _N_NEG = 0, 1, 2, 3, 4, 5, 6
 _N_D2C, _N_D1C, _N_D1UP, _N_D1DOWN, _N_WENO_UP, _N_WENO_DOWN = 7, 8, 9, 10, 11, 12
 _N_INTP_C2N, _N_INTP_N2C, _N_UNKNOWN = 13, 14, 15
 _N_FLUXLIM, _N_MIN, _N_MAX, _N_ABS, _N_GT, _N_LT, _N_GE, _N_LE, _N_COND = 16, 17, 18, 19, 20, 21, 22, 23, 24
+_N_FUNC1, _N_FUNC2, _N_CONV = 25, 26, 27
+# OPF_FN_* / OPF_FN2_* (the <cmath> names of Arithmetic/AMDS.hpp:39-90)
+_FN1 = ["exp", "exp2", "expm1", "log", "log10", "log2", "log1p", "sqrt", "cbrt", "sin", "cos", "tan", "asin", "acos",
+        "atan", "sinh", "cosh", "tanh", "asinh", "acosh", "atanh", "erf", "erfc", "tgamma", "lgamma", "ceil", "floor",
+        "trunc", "round", "nearbyint", "rint", "logb"]
+_FN2 = ["pow", "fmod", "remainder", "fdim", "hypot", "atan2", "copysign", "nextafter"]
+_FN_SMOOTH_DELTA, _FN_SMOOTH_HEAVISIDE = 32, 33
 
 
 def _wrap(v):
@@ -243,11 +251,17 @@ class Expr:
                 node = (_N_FIELD, slot_of[key], 0, 0, 0.0)
             elif e.kind == _N_CONST:
                 node = (_N_CONST, 0, 0, 0, e.value)
+            elif e.kind == _N_CONV:  # the weights follow the argument as consecutive CONST nodes
+                ia = visit(e.a)
+                ib = len(nodes)
+                for w in e.value:
+                    nodes.append((_N_CONST, 0, 0, 0, float(w)))
+                node = (_N_CONV, ia, ib, e.dim, 0.0)
             else:
                 ia = visit(e.a)
                 ib = visit(e.b) if e.b is not None else 0
                 dim = visit(e.c) if e.kind == _N_COND else e.dim
-                node = (e.kind, ia, ib, dim, float(e.value) if e.kind == _N_FLUXLIM else 0.0)
+                node = (e.kind, ia, ib, dim, float(e.value) if e.kind in (_N_FLUXLIM, _N_FUNC1) else 0.0)
             nodes.append(node)
             memo[id(e)] = len(nodes) - 1
             return memo[id(e)]
@@ -378,6 +392,55 @@ def max(a, b):  # noqa: A001  OpFlow::max (Operator/Arithmetic/MinMax.hpp): std:
 
 def min(a, b):  # noqa: A001  OpFlow::min: std::min elementwise
     return Expr(_N_MIN, _wrap(a), _wrap(b))
+
+
+def _fn1_maker(fid, name):
+    def f(e):
+        return Expr(_N_FUNC1, _wrap(e), dim=fid)
+    f.__name__ = name
+    f.__doc__ = f"OpFlow::{name}(e): std::{name} of each value (Arithmetic/AMDS.hpp, DEFINE_UNIFUNC)."
+    return f
+
+
+def _fn2_maker(fid, name):
+    def f(a, b):
+        return Expr(_N_FUNC2, _wrap(a), _wrap(b), dim=fid)
+    f.__name__ = name
+    f.__doc__ = f"OpFlow::{name}(a, b): std::{name} of each pair of values (Arithmetic/AMDS.hpp, DEFINE_BINFUNC)."
+    return f
+
+
+for _i, _n in enumerate(_FN1):
+    globals()[_n] = _fn1_maker(_i, _n)
+for _i, _n in enumerate(_FN2):
+    globals()[_n] = _fn2_maker(_i, _n)
+
+
+def abs(e):  # noqa: A001  OpFlow::abs (Arithmetic/AMDS.hpp): std::abs of each value
+    return Expr(_N_ABS, _wrap(e))
+
+
+def smoothDelta(eps, e):
+    """Math::smoothDelta(eps, d) of each value (Math/Function/Numeric.hpp:94-96), the functor the
+    level-set example applies with UniOpAdaptor."""
+    return Expr(_N_FUNC1, _wrap(e), dim=_FN_SMOOTH_DELTA, value=float(eps))
+
+
+def smoothHeviside(eps, e):
+    """Math::smoothHeviside(eps, d) of each value (Math/Function/Numeric.hpp:86-92)."""
+    return Expr(_N_FUNC1, _wrap(e), dim=_FN_SMOOTH_HEAVISIDE, value=float(eps))
+
+
+def conv(e, kernel):
+    """conv(e, kernel) (Operator/Convolution/Convolution.hpp): sum over the window centred at
+    each cell of kernel[offset] * e(cell + offset); kernel: array of odd extents, indexed
+    [x, y(, z)] like DS::FixedSizeTensor<T, n0, n1(, n2)> (x-fastest values)."""
+    k = np.asarray(kernel, dtype=np.float64)
+    ext = list(k.shape) + [1] * (3 - k.ndim)
+    if k.ndim < 1 or k.ndim > 3 or any(n % 2 == 0 for n in ext):
+        raise ValueError("conv kernel: 1 to 3 dims of odd extents")
+    weights = tuple(float(w) for w in k.reshape(-1, order="F"))  # x-fastest
+    return Expr(_N_CONV, _wrap(e), dim=ext[0] | ext[1] << 8 | ext[2] << 16, value=weights)
 
 
 # ---------------------------------------------------------------- fields

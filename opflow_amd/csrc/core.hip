@@ -120,7 +120,7 @@ extern "C" int opf_ctx_sync(opf_ctx* c) {
 extern "C" void* opf_ctx_stream(opf_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 extern "C" int opf_ctx_set_jit(opf_ctx* c, int mode) {
-    OPF_CHECK(c && mode >= 0 && mode <= 2, OPF_ERR_ARG, "opf_ctx_set_jit: null ctx or mode outside 0..2");
+    OPF_CHECK(c && mode >= 0 && mode <= 4, OPF_ERR_ARG, "opf_ctx_set_jit: null ctx or mode outside 0..4");
     c->jit_mode = mode;
     return OPF_OK;
 }

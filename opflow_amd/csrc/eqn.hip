@@ -20,6 +20,10 @@
 #include "gmg.hpp"
 #include "sweep.hpp"
 
+#define OPF_SHARED_SRC(...) __VA_ARGS__
+#include "math_device.inc"
+#undef OPF_SHARED_SRC
+
 namespace opf {
 
 constexpr int PAD_K = 7;       // centre + 2 per dim
@@ -153,15 +157,38 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
                 bad |= 2;
                 sp -= 6;
                 break;
-            case I_SEL:  // selections and comparisons are not linear forms either
-                bad |= 2;
+            // the nonlinear operations apply to values that do not read the unknown
+            // (coefficients such as sqrt(r) or max(r, 0)); on a linear form they fail
+            case I_SEL: {
                 sp -= 2;
+                if (stk[sp][PAD_K + 1][tid] != 0.0 || stk[sp + 1][PAD_K + 1][tid] != 0.0 || t[PAD_K + 1] != 0.0) {
+                    bad |= 2;
+                    break;
+                }
+                t[PAD_K] = stk[sp][PAD_K][tid] != 0.0 ? stk[sp + 1][PAD_K][tid] : t[PAD_K];
+            } break;
+            case I_ABS:
+            case I_FN1:
+                if (t[PAD_K + 1] != 0.0) { bad |= 2; break; }
+                t[PAD_K] = in.op == I_ABS ? fabs(t[PAD_K]) : opf_fn1<double>(in.a, t[PAD_K]);
                 break;
-            case I_ABS: bad |= 2; break;
-            default:  // I_MIN, I_MAX and the comparisons: binary
-                bad |= 2;
+            default: {  // I_MIN, I_MAX, I_FN2 and the comparisons: binary
                 --sp;
-                break;
+                if (stk[sp][PAD_K + 1][tid] != 0.0 || t[PAD_K + 1] != 0.0) { bad |= 2; break; }
+                const double x = stk[sp][PAD_K][tid], y = t[PAD_K];
+                double v = 0;
+                switch (in.op) {
+                    case I_MIN: v = y < x ? y : x; break;
+                    case I_MAX: v = x < y ? y : x; break;
+                    case I_GT: v = x > y ? 1.0 : 0.0; break;
+                    case I_LT: v = x < y ? 1.0 : 0.0; break;
+                    case I_GE: v = x >= y ? 1.0 : 0.0; break;
+                    case I_LE: v = x <= y ? 1.0 : 0.0; break;
+                    case I_FN2: v = opf_fn2<double>(in.a, x, y); break;
+                    default: bad |= 2;
+                }
+                t[PAD_K] = v;
+            } break;
         }
     }
     if (!live) return;
@@ -277,6 +304,7 @@ struct opf_eqn {
     bool symmetric = true;  // false: BiCGStab with the symmetric part's multigrid as preconditioner
     bool built = false;
     bool first = true;
+    DevLevel lvl;  // the operator's fine level (device arrays, reused by every rebuild)
     Hierarchy H;
     int per[3] = {0, 0, 0};
     double alpha[3][2] = {{1, 1}, {1, 1}, {1, 1}};
@@ -355,6 +383,7 @@ static int eqn_eval_bias(opf_eqn* e) {
     const opf_field_desc& td = t->d;
     a.f[nf] = {e->zbuf, td.offset[0], td.offset[1], td.offset[2], td.extent[0], td.extent[1]};
     OPF_TRY(mesh_arrays(t->mesh, OPF_F64, a.x, a.dx));
+    uniform_spacing(t->mesh, OPF_F64, a);
     for (int q = 0; q < 3; ++q) {
         a.morg[q] = t->mesh->start[q] - t->mesh->pad;
         a.s[q] = e->s[q];
@@ -372,129 +401,17 @@ static int eqn_eval_bias(opf_eqn* e) {
     return launch_sweep(ctx, OPF_F64, a, e->Pb);
 }
 
-// The rows of one diagonal block (PAD_K coefficient arrays of N rows, host copy) as
-// a multigrid level: face coefficients from the neighbour slots, the rest of the
-// diagonal as sigma. *sign orients the diagonal positive; a uniform operator
-// becomes the matrix-free constant mode. A face whose two rows disagree makes the
-// operator non-symmetric: an error unless `symmetrize`, which takes the mean of
-// the two (the level is then the symmetric part, for use as a preconditioner)
-// and reports *symmetric = false.
-static int rows_to_level(const std::vector<double>& c, long long N, const int nn[3], int dim, const int per[3],
-                         double* sign, GLevelHost& L, bool symmetrize = false, bool* symmetric = nullptr) {
-    if (symmetric) *symmetric = true;
-    auto differ = [](double u, double v) { return std::fabs(u - v) > 1e-12 * std::max(std::fabs(u), std::fabs(v)); };
-    // orientation: diagonal must be positive after scaling by sign
-    double dsum = 0;
-    for (long long r = 0; r < N; ++r) dsum += c[r];
-    OPF_CHECK(dsum != 0.0, OPF_ERR_UNSUPPORTED, "equation has no diagonal");
-    *sign = dsum > 0 ? 1.0 : -1.0;
-    const double s = *sign;
-    L = GLevelHost{};
-    L.var = true;
-    for (int q = 0; q < 3; ++q) L.n[q] = nn[q];
-    L.N = N;
-    auto cidx = [&](int i, int j, int k) { return i + (long long)nn[0] * (j + (long long)nn[1] * k); };
-    double maxa = 0;
-    for (int d = 0; d < dim; ++d) {
-        int fn[3] = {nn[0] + (d == 0), nn[1] + (d == 1), nn[2] + (d == 2)};
-        L.af[d].assign((size_t)fn[0] * fn[1] * fn[2], 0.0);
-        for (int k = 0; k < fn[2]; ++k)
-            for (int j = 0; j < fn[1]; ++j)
-                for (int i = 0; i < fn[0]; ++i) {
-                    int p[3] = {i, j, k};
-                    double alo = NAN, ahi = NAN;  // from the row above (its lo face) and below (its hi face)
-                    if (p[d] < nn[d]) alo = -s * c[(size_t)(1 + 2 * d) * N + cidx(i, j, k)];
-                    if (p[d] > 0) {
-                        int q[3] = {i, j, k};
-                        q[d] -= 1;
-                        ahi = -s * c[(size_t)(2 + 2 * d) * N + cidx(q[0], q[1], q[2])];
-                    }
-                    double a = std::isnan(alo) ? ahi : alo;
-                    if (!std::isnan(alo) && !std::isnan(ahi) && differ(alo, ahi)) {
-                        OPF_CHECK(symmetrize, OPF_ERR_UNSUPPORTED,
-                                  "operator is not symmetric (face %d of dim %d): %g vs %g", i, d, alo, ahi);
-                        a = 0.5 * (alo + ahi);
-                        if (symmetric) *symmetric = false;
-                    }
-                    L.af[d][(size_t)i + (size_t)fn[0] * (j + (size_t)fn[1] * k)] = a;
-                    maxa = std::max(maxa, std::fabs(a));
-                }
-        if (per[d]) {  // face 0 and face n are the same physical face
-            for (int k = 0; k < fn[2]; ++k)
-                for (int j = 0; j < fn[1]; ++j)
-                    for (int i = 0; i < fn[0]; ++i) {
-                        int p[3] = {i, j, k};
-                        if (p[d] != 0) continue;
-                        int q[3] = {i, j, k};
-                        q[d] = nn[d];
-                        double& f0 = L.af[d][(size_t)p[0] + (size_t)fn[0] * (p[1] + (size_t)fn[1] * p[2])];
-                        double& fnv = L.af[d][(size_t)q[0] + (size_t)fn[0] * (q[1] + (size_t)fn[1] * q[2])];
-                        if (differ(f0, fnv)) {
-                            OPF_CHECK(symmetrize, OPF_ERR_UNSUPPORTED, "periodic faces disagree in dim %d", d);
-                            f0 = fnv = 0.5 * (f0 + fnv);
-                            if (symmetric) *symmetric = false;
-                        }
-                    }
-        }
-    }
-    L.sg.assign(N, 0.0);
-    for (int k = 0; k < nn[2]; ++k)
-        for (int j = 0; j < nn[1]; ++j)
-            for (int i = 0; i < nn[0]; ++i) {
-                long long r = cidx(i, j, k);
-                double sum = 0;
-                int p[3] = {i, j, k};
-                for (int d = 0; d < dim; ++d) {
-                    int fn0 = nn[0] + (d == 0), fn1 = nn[1] + (d == 1);
-                    int lo[3] = {i, j, k}, hi[3] = {i, j, k};
-                    hi[d] += 1;
-                    sum += L.af[d][(size_t)lo[0] + (size_t)fn0 * (lo[1] + (size_t)fn1 * lo[2])];
-                    sum += L.af[d][(size_t)hi[0] + (size_t)fn0 * (hi[1] + (size_t)fn1 * hi[2])];
-                    (void)p;
-                }
-                double sg = s * c[r] - sum;
-                if (std::fabs(sg) <= 1e-12 * std::max(1.0, sum)) sg = 0.0;
-                L.sg[r] = sg;
-            }
-    // constant-coefficient detection (uniform operator => matrix-free constant mode)
-    bool cst = true;
-    double a0[3] = {0, 0, 0};
-    for (int d = 0; d < dim && cst; ++d) {
-        a0[d] = L.af[d][0];
-        for (double v : L.af[d]) cst &= v == a0[d];
-    }
-    for (double v : L.sg) cst &= v == L.sg[0];
-    if (cst) {
-        GLevelHost C;
-        C.var = false;
-        for (int q = 0; q < 3; ++q) {
-            C.n[q] = L.n[q];
-            C.a[q] = a0[q];
-        }
-        C.N = N;
-        C.sigma = L.sg[0];
-        L = C;
-    }
-    return OPF_OK;
-}
-
-// rows -> multigrid hierarchy of a single equation
+// rows -> multigrid hierarchy of a single equation, on the device (the rows are
+// never copied to the host); a rebuild over an operator of the same layout keeps
+// the hierarchy's buffers and recomputes only the coefficients
 static int eqn_build_operator(opf_eqn* e) {
-    const long long N = e->N;
-    std::vector<double> c((size_t)PAD_K * N);
-    OPF_HIP(hipMemcpyAsync(c.data(), e->coef, sizeof(double) * c.size(), hipMemcpyDeviceToHost, e->ctx->stream));
-    OPF_HIP(hipStreamSynchronize(e->ctx->stream));
-    GLevelHost L;
-    OPF_TRY(rows_to_level(c, N, e->n, e->target->d.dim, e->per, &e->sign, L, true, &e->symmetric));
-    if (e->built) gmg_free(e->H);
-    e->H = Hierarchy{};
+    OPF_TRY(rows_to_level_dev(e->ctx->stream, e->coef, e->N, e->n, e->target->d.dim, e->per, &e->sign, e->lvl,
+                              &e->symmetric));
     if (e->params.pre_smooth > 0) e->H.pre = e->params.pre_smooth;
     if (e->params.post_smooth > 0) e->H.post = e->params.post_smooth;
     const bool mixed = e->params.precision == OPF_PREC_MIXED && e->symmetric;
-    int rc = gmg_build(e->ctx, e->H, L, e->per, e->alpha, mixed ? OPF_PREC_MIXED : OPF_PREC_FP64);
-    if (rc != OPF_OK) return rc;
-    e->built = true;
-    return OPF_OK;
+    e->built = true;  // gmg_free is safe on a partly built hierarchy
+    return gmg_build_dev(e->ctx, e->H, e->lvl, e->per, e->alpha, mixed ? OPF_PREC_MIXED : OPF_PREC_FP64);
 }
 
 // non-symmetric operators: BiCGStab on the pinned system (defined with the set solvers)
@@ -680,6 +597,7 @@ extern "C" int opf_eqn_destroy(opf_eqn* e) {
     if (!e) return OPF_OK;
     hipStreamSynchronize(e->ctx->stream);
     if (e->built) gmg_free(e->H);
+    dev_level_free(e->lvl);
     void* p[] = {e->coef, e->bias, e->b, e->x, e->shift, e->err, e->zbuf};
     for (void* q : p)
         if (q) hipFree(q);
@@ -948,14 +866,6 @@ static bool same_diagonal(const EqsSystem& S, int i, int j) {
     return true;
 }
 
-static bool same_level(const GLevelHost& a, const GLevelHost& b) {
-    if (a.var != b.var || a.N != b.N) return false;
-    for (int q = 0; q < 3; ++q)
-        if (a.n[q] != b.n[q]) return false;
-    if (!a.var) return a.a[0] == b.a[0] && a.a[1] == b.a[1] && a.a[2] == b.a[2] && a.sigma == b.sigma;
-    return a.af[0] == b.af[0] && a.af[1] == b.af[1] && a.af[2] == b.af[2] && a.sg == b.sg;
-}
-
 }  // namespace opf
 
 extern "C" int opf_eqns_csr(int neq, const opf_eqn_desc* eqs, opf_field* const* targets, const int* pin,
@@ -1190,31 +1100,35 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
     for (int i = 0; i < neq; ++i) coupled |= !(S.refs[i].size() == 1 && S.refs[i][0] == i);
     std::vector<int> pin(neq);
     for (int i = 0; i < neq; ++i) pin[i] = params[i].pin_value != 0;
-    // diagonal blocks -> multigrid levels
-    std::vector<GLevelHost> L(neq);
+    // diagonal blocks -> multigrid levels, built on the device; a twin (an earlier
+    // equation whose diagonal block has the same program) shares its level's arrays
+    struct Levels {
+        std::vector<DevLevel> L;
+        std::vector<char> own;
+        ~Levels() {
+            for (size_t i = 0; i < L.size(); ++i)
+                if (own[i]) dev_level_free(L[i]);
+        }
+    } lv{std::vector<DevLevel>(neq), std::vector<char>(neq, 0)};
+    std::vector<DevLevel>& L = lv.L;
     std::vector<double> sign(neq, 1.0);
     std::vector<char> symm(neq, 1);
-    {
-        std::vector<double> c((size_t)PAD_K * N);
-        for (int i = 0; i < neq; ++i) {
-            const int b = S.blk[(size_t)i * neq + i];
-            OPF_CHECK(b >= 0, OPF_ERR_UNSUPPORTED, "equation %d does not read its own target (no diagonal block)", i);
-            int twin = -1;  // an earlier equation whose diagonal block has the same coefficients
-            for (int j = 0; j < i && twin < 0; ++j)
-                if (same_diagonal(S, i, j)) twin = j;
-            if (twin >= 0) {
-                L[i] = L[twin];
-                sign[i] = sign[twin];
-                symm[i] = symm[twin];
-                continue;
-            }
-            OPF_HIP(hipMemcpyAsync(c.data(), S.coef + (size_t)b * PAD_K * N, sizeof(double) * c.size(),
-                                   hipMemcpyDeviceToHost, st));
-            OPF_HIP(hipStreamSynchronize(st));
-            bool sy = true;
-            OPF_TRY(rows_to_level(c, N, S.n, S.dim, S.per[i], &sign[i], L[i], true, &sy));
-            symm[i] = sy;
+    for (int i = 0; i < neq; ++i) {
+        const int b = S.blk[(size_t)i * neq + i];
+        OPF_CHECK(b >= 0, OPF_ERR_UNSUPPORTED, "equation %d does not read its own target (no diagonal block)", i);
+        int twin = -1;
+        for (int j = 0; j < i && twin < 0; ++j)
+            if (same_diagonal(S, i, j)) twin = j;
+        if (twin >= 0) {
+            L[i] = L[twin];
+            sign[i] = sign[twin];
+            symm[i] = symm[twin];
+            continue;
         }
+        lv.own[i] = 1;
+        bool sy = true;
+        OPF_TRY(rows_to_level_dev(st, S.coef + (size_t)b * PAD_K * N, N, S.n, S.dim, S.per[i], &sign[i], L[i], &sy));
+        symm[i] = sy;
     }
     // a non-symmetric diagonal block is solved by the Krylov method of coupled sets,
     // its multigrid level (the symmetric part) as preconditioner
@@ -1222,7 +1136,7 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
     auto hier = [&](Hierarchy& H, int i, int slots, int precision) {
         if (params[0].pre_smooth > 0) H.pre = params[0].pre_smooth;
         if (params[0].post_smooth > 0) H.post = params[0].post_smooth;
-        return gmg_build(ctx, H, L[i], S.per[i], S.alpha[i], precision, slots);
+        return gmg_build_dev(ctx, H, L[i], S.per[i], S.alpha[i], precision, slots);
     };
     double* vec = nullptr;  // x, b over the whole system (block i at i * N)
     OPF_HIP(hipMalloc(&vec, sizeof(double) * 2 * N * neq + 64));
@@ -1250,13 +1164,15 @@ extern "C" int opf_eqns_solve(int neq, const opf_eqn_desc* eqs, opf_field* const
             for (int i = 0; i < neq && !rc; ++i) {
                 if (grp[i] >= 0) continue;
                 std::vector<int> mem{i};
-                for (int j = i + 1; j < neq; ++j) {
-                    bool same = grp[j] < 0 && same_level(L[i], L[j]);
+                for (int j = i + 1; j < neq && !rc; ++j) {
+                    bool same = false;
+                    if (grp[j] < 0) rc = dev_levels_equal(st, L[i], L[j], S.dim, &same);
                     for (int d = 0; d < 3 && same; ++d)
                         same &= S.per[i][d] == S.per[j][d] && S.alpha[i][d][0] == S.alpha[j][d][0] &&
                                 S.alpha[i][d][1] == S.alpha[j][d][1];
                     if (same) mem.push_back(j);
                 }
+                if (rc) break;
                 for (int m : mem) grp[m] = i;
                 H.emplace_back();
                 Hierarchy& G = H.back();

@@ -21,12 +21,6 @@
 namespace opf {
 
 // ------------------------------------------------------------------ helpers
-__host__ __device__ __forceinline__ long long face_index(int d, int i, int j, int k, int n0, int n1) {
-    // face arrays of dim d have extents n + e_d
-    int f0 = n0 + (d == 0), f1 = n1 + (d == 1);
-    return i + (long long)f0 * (j + (long long)f1 * k);
-}
-
 template <typename T>
 __device__ __forceinline__ T face_coef(const GOp<T>& o, bool var, int d, int i, int j, int k) {
     return var ? o.af[d][face_index(d, i, j, k, o.n0, o.n1)] : o.a[d];
@@ -751,7 +745,22 @@ __global__ __launch_bounds__(256) void k_pjacobi(GOp<T> o, const T* __restrict__
     xo[t] = yc + w * (b[t] - ay) / D;
 }
 
-// coarsest level: x = Cinv b (FP64 inverse), one block
+// x = Cinv b in FP64 for the coarsest level: 16 lanes per row, each summing every 16th
+// column, the partial sums combined in a fixed order, so a row's reads are coalesced
+// and every thread of the block works (bv(c): b at column c)
+template <typename T, typename BV>
+__device__ __forceinline__ void dense_apply(const double* __restrict__ cinv, int n, BV bv, T* x) {
+    const int lane = threadIdx.x & 15, groups = blockDim.x >> 4;
+    for (int row = threadIdx.x >> 4; row < n; row += groups) {  // the 16 lanes of a group share the row
+        const double* a = cinv + (long long)row * n;
+        double s = 0;
+        for (int c = lane; c < n; c += 16) s += a[c] * bv(c);
+        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+        if (lane == 0) x[row] = (T)s;
+    }
+}
+
+// coarsest level: x = Cinv b (FP64 inverse), one block per slot
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, int n, const T* __restrict__ b,
                                                T* __restrict__ x, Bat bt) {
@@ -761,12 +770,7 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
     x += (long long)slot * n;
     for (int q = threadIdx.x; q < n; q += blockDim.x) sb[q] = (double)b[q];
     __syncthreads();
-    for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        double s = 0;
-        const double* a = cinv + (long long)row * n;
-        for (int c = 0; c < n; ++c) s += a[c] * sb[c];
-        x[row] = (T)s;
-    }
+    dense_apply<T>(cinv, n, [&](int c) { return sb[c]; }, x);
 }
 
 
@@ -908,12 +912,8 @@ __global__ __launch_bounds__(GMG_TAIL_NT) void k_vtail(TailArgs<T> ta, Bat bt) {
     }
     {  // coarsest: x = Cinv b in FP64
         const TailLevel<T>& L = ta.lv[ta.nl - 1];
-        for (int row = threadIdx.x; row < ta.cN; row += blockDim.x) {
-            double s = 0;
-            const double* a = ta.cinv + (long long)row * ta.cN;
-            for (int c = 0; c < ta.cN; ++c) s += a[c] * (double)Bv(L)[c];
-            X(L)[row] = (T)s;
-        }
+        const T* bl = Bv(L);
+        dense_apply<T>(ta.cinv, ta.cN, [&](int c) { return (double)bl[c]; }, X(L));
         __syncthreads();
     }
     for (int l = ta.nl - 2; l >= 0; --l) {  // up: prolongation, post-smoothing
@@ -931,8 +931,12 @@ __global__ __launch_bounds__(GMG_TAIL_NT) void k_vtail(TailArgs<T> ta, Bat bt) {
 }
 
 __global__ void k_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ s, double invN,
-                         int singular, Bat bt) {
+                         int singular, Bat bt, int* __restrict__ mk_it, int* __restrict__ mk_rep, int maxit) {
     BAT_SLOT(bt);
+    if (mode == FIN_RR_DECIDE && !mk_it[slot]) {  // launched for every slot: a stopped one is not updated
+        if (threadIdx.x == 0) mk_rep[slot] = 0;
+        return;
+    }
     part += (long long)slot * bt.ps;
     s += (long long)slot * bt.ss;
     // up to 16384 partials per value: independent loads, four accumulators per
@@ -967,6 +971,39 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int mode, doub
             s[S_RR] = v[0];
             s[S_RREF] = v[0];
             break;
+        case FIN_RR_DECIDE: {
+            // count the iteration; fold e into x and recompute the true residual when the
+            // recurrence residual fell by delta2 since the last update, looks converged, or
+            // the cap is reached (!(a >= b) also catches NaN)
+            const double rr = v[0];
+            s[S_RR] = rr;
+            const double its = s[S_ITS] + 1.0;
+            s[S_ITS] = its;
+            const int rep = !(rr >= s[S_DLT2] * s[S_RREF]) || rr <= s[S_TGT] || its >= (double)maxit;
+            mk_rep[slot] = rep;
+            if (rep) {
+                s[S_NREP] += 1.0;
+                s[S_RREC] = rr;
+            }
+        } break;
+        case FIN_REP_DONE: {
+            // after the update's true residual: stop a slot that meets tol (or hit the cap /
+            // broke down); else adapt the update threshold to the drift of the recurrence
+            const double rr = v[0];
+            s[S_RR] = rr;
+            s[S_RREF] = rr;
+            if (rr <= s[S_TGT] || s[S_ITS] >= (double)maxit || !isfinite(rr)) {
+                mk_it[slot] = 0;
+                s[S_DONE] = 1.0;
+                break;
+            }
+            const double d2 = s[S_DLT2];
+            if (d2 > 0.0) {
+                const double drift = rr / s[S_RREC];
+                if (drift < 4.0) s[S_DLT2] = fmax(d2 * 1e-2, 1e-12);
+                else if (drift > 100.0) s[S_DLT2] = fmin(d2 * 1e2, 1e-2);
+            }
+        } break;
         case FIN_MEAN: s[S_MEAN] = v[0]; break;
         case FIN_DOT3_FIRST:
             s[S_RZ] = v[0];
@@ -1119,58 +1156,16 @@ __global__ void k_rel_init(int B, const int* __restrict__ m_new, int* __restrict
     m_start[k] = go;
 }
 
-// after an iteration's (r, r): count it; fold e into x and recompute the true residual
-// when the recurrence residual fell by delta2 since the last update, looks converged,
-// or the cap is reached (!(a >= b) also catches NaN)
-__global__ void k_rel_decide(int B, const int* __restrict__ m_it, int* __restrict__ m_rep, double* __restrict__ scal,
-                             int maxit) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= B) return;
-    if (!m_it[k]) {
-        m_rep[k] = 0;
-        return;
-    }
-    double* s = scal + (size_t)k * GMG_SSTRIDE;
-    const double its = s[S_ITS] + 1.0;
-    s[S_ITS] = its;
-    const double rr = s[S_RR];
-    const int rep = !(rr >= s[S_DLT2] * s[S_RREF]) || rr <= s[S_TGT] || its >= (double)maxit;
-    m_rep[k] = rep;
-    if (rep) {
-        s[S_NREP] += 1.0;
-        s[S_RREC] = rr;
-    }
-}
-
-// after the update's true residual: stop the slots that meet tol (or hit the cap / break down)
-__global__ void k_rel_done(int B, int* __restrict__ m_it, const int* __restrict__ m_rep, double* __restrict__ scal,
-                           int maxit) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= B || !m_rep[k]) return;
-    double* s = scal + (size_t)k * GMG_SSTRIDE;
-    const double rr = s[S_RR];
-    if (rr <= s[S_TGT] || s[S_ITS] >= (double)maxit || !isfinite(rr)) {
-        m_it[k] = 0;
-        s[S_DONE] = 1.0;
-        return;
-    }
-    // drift of the recurrence residual since the last update -> next update threshold
-    const double d2 = s[S_DLT2];
-    if (d2 > 0.0) {
-        const double drift = rr / s[S_RREC];
-        if (drift < 4.0) s[S_DLT2] = fmax(d2 * 1e-2, 1e-12);
-        else if (drift > 100.0) s[S_DLT2] = fmin(d2 * 1e2, 1e-2);
-    }
-}
-
 // x (FP64) += e ; e = 0
 template <typename TE>
 __global__ void k_add_zero(long long N, double* __restrict__ x, TE* __restrict__ e, Bat bt) {
     BAT_SLOT(bt);
     x += vo;
     e += vo;
-    const uint32_t t = cell_id();
-    if (t < (uint32_t)N) {
+    // a capped grid striding over the cells: the launch sits in every iteration and is
+    // masked off in most, where fewer blocks cost less to dispatch
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; t < (uint32_t)N; t += stride) {
         x[t] += (double)e[t];
         e[t] = (TE)0;
     }
@@ -1210,6 +1205,9 @@ static bool use_z(const Hierarchy& H, int l) {
     const GLevelHost& L = H.host[l];
     return !H.var && L.n[0] >= 64 && L.N >= (H.dim == 3 ? (1 << 18) : (1 << 16));
 }
+// blocks of a z-marching launch: every tile item, at most GMG_RBLK blocks striding over
+// them (fewer partial sums to finish, cheaper dispatch of the masked launches)
+static inline int zgrid(const ZTile& z) { return std::min(z.items, GMG_RBLK); }
 static ZTile ztile(const GLevelHost& L) {
     ZTile z;
     z.zc = L.n[2] == 1 ? 1 : L.n[2] >= 256 ? 8 : 4;
@@ -1251,7 +1249,7 @@ static bool use_zp_restrict(const Hierarchy& H, int l) {
 template <typename T>
 static int red_count(const Hierarchy& H) {
     if (use_zp(H, 0)) return zp_grid<T>(H.host[0]).blocks;
-    if (use_z(H, 0)) return std::min(GMG_MAXB, ztile(H.host[0]).items);
+    if (use_z(H, 0)) return zgrid(ztile(H.host[0]));
     return rblk(H.host[0].N);
 }
 
@@ -1359,13 +1357,15 @@ static bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
     return true;
 }
 
-static void coarsen(const Hierarchy& H, const GLevelHost& F, GLevelHost& C) {
+// the next coarser level's shape and constant-mode coefficients; its variable-mode
+// arrays too when `arrays` (host master copy)
+static void coarsen(const Hierarchy& H, const GLevelHost& F, GLevelHost& C, bool arrays = true) {
     for (int d = 0; d < 3; ++d) C.n[d] = d < H.dim ? F.n[d] / 2 : 1;
     C.N = (long long)C.n[0] * C.n[1] * C.n[2];
     C.var = F.var;
     for (int d = 0; d < 3; ++d) C.a[d] = F.a[d] * 0.25;
     C.sigma = F.sigma;
-    if (!F.var) return;
+    if (!F.var || !arrays) return;
     const int tw = H.dim == 3 ? 4 : 2;  // fine faces per coarse face
     for (int d = 0; d < H.dim; ++d) {
         C.af[d].assign(face_count(C.n, d), 0.0);
@@ -1408,8 +1408,11 @@ static int alloc(void** p, long long n, size_t& bytes) {
     return OPF_OK;
 }
 
-int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
-              int precision, int slots) {
+// parameters and singularity of H and the host description of its levels (the
+// coarse levels' variable-mode arrays only when `arrays`); shift: some cell of
+// the fine level has a nonzero sigma
+static int hier_levels(Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
+                       int precision, int slots, bool shift, bool arrays) {
     H.precision = precision;
     H.B = std::max(1, slots);
     H.var = fine.var;
@@ -1423,9 +1426,6 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
               fine.N);
     H.omega = H.dim == 3 ? 6.0 / 7.0 : 0.8;
     // singular: no Dirichlet-like face and no diagonal shift
-    bool shift = fine.var ? false : fine.sigma != 0.0;
-    if (fine.var)
-        for (double v : fine.sg) shift |= v != 0.0;
     bool anchored = false;
     for (int d = 0; d < H.dim; ++d)
         for (int s = 0; s < 2; ++s) anchored |= !per[d] && alpha[d][s] != 1.0;
@@ -1438,13 +1438,19 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
         for (int d = 0; d < H.dim; ++d) can &= (F.n[d] % 2 == 0) && F.n[d] >= 4;
         if (!can) break;
         GLevelHost C;
-        coarsen(H, F, C);
+        coarsen(H, F, C, arrays);
         H.host.push_back(C);
     }
     const GLevelHost& Lc = H.host.back();
     OPF_CHECK(Lc.N <= 1024, OPF_ERR_UNSUPPORTED,
               "gmg: coarsest grid has %lld cells (> 1024); use sizes divisible by a power of two", Lc.N);
-    // coarsest dense inverse
+    return OPF_OK;
+}
+
+// dense inverse of the coarsest level (its arrays on the host in variable mode)
+// into H.cinv (allocated on first use)
+static int hier_coarsest(Hierarchy& H) {
+    const GLevelHost& Lc = H.host.back();
     std::vector<double> A, inv;
     dense_matrix(H, Lc, A);
     if (H.singular) {
@@ -1454,15 +1460,63 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
         for (auto& v : A) v += dm;
     }
     OPF_CHECK(invert(A, (int)Lc.N, inv), OPF_ERR_ARG, "gmg: singular coarsest operator");
-    H.cN = (int)Lc.N;
-    size_t& bytes = H.bytes;
-    bytes = 0;
-    OPF_TRY(alloc<double>((void**)&H.cinv, (long long)H.cN * H.cN, bytes));
+    if (!H.cinv) {
+        H.cN = (int)Lc.N;
+        OPF_TRY(alloc<double>((void**)&H.cinv, (long long)H.cN * H.cN, H.bytes));
+    }
     OPF_HIP(hipMemcpy(H.cinv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    return OPF_OK;
+}
+
+// work vectors of every level (x, b below the finest, t), the Krylov vectors and
+// the scalars of the B slots
+static int hier_vectors(Hierarchy& H) {
+    const bool mixed = H.precision == OPF_PREC_MIXED;
+    const size_t ts = mixed ? sizeof(float) : sizeof(double);
+    size_t& bytes = H.bytes;
+    for (size_t l = 0; l < H.host.size(); ++l) {
+        GLevelDev& D = H.dev[l];
+        const size_t BN = (size_t)H.B * H.host[l].N;
+        if (l > 0) {
+            OPF_HIP(hipMalloc(&D.x, ts * BN));
+            OPF_HIP(hipMalloc(&D.b, ts * BN));
+            bytes += 2 * ts * BN;
+        }
+        OPF_HIP(hipMalloc(&D.t, ts * BN));
+        bytes += ts * BN;
+    }
+    const size_t BN = (size_t)H.B * H.host[0].N;
+    OPF_HIP(hipMalloc(&H.r, ts * BN));
+    OPF_HIP(hipMalloc(&H.z, ts * BN));
+    OPF_HIP(hipMalloc(&H.p[0], ts * BN));
+    OPF_HIP(hipMalloc(&H.p[1], ts * BN));
+    OPF_HIP(hipMalloc(&H.q, ts * BN));
+    bytes += 5 * ts * BN;
+    OPF_HIP(hipMalloc(&H.e, ts * BN));  // inner correction (mixed) / serving correction
+    bytes += ts * BN;
+    OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * BN));
+    bytes += sizeof(double) * BN;
+    OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * GMG_PSTRIDE * H.B));
+    OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * GMG_SSTRIDE * H.B));
+    OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * GMG_SSTRIDE * H.B));
+    OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * GMG_SSTRIDE * H.B, hipHostMallocDefault));
+    OPF_HIP(hipMalloc((void**)&H.mk, sizeof(int) * MK_COUNT * H.B));
+    OPF_HIP(hipMemset(H.mk, 0, sizeof(int) * MK_COUNT * H.B));
+    return OPF_OK;
+}
+
+int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
+              int precision, int slots) {
+    bool shift = fine.var ? false : fine.sigma != 0.0;
+    if (fine.var)
+        for (double v : fine.sg) shift |= v != 0.0;
+    OPF_TRY(hier_levels(H, fine, per, alpha, precision, slots, shift, true));
+    H.bytes = 0;
+    OPF_TRY(hier_coarsest(H));
     // device levels
     H.dev.assign(H.host.size(), GLevelDev{});
     const bool mixed = precision == OPF_PREC_MIXED;
-    const size_t ts = mixed ? sizeof(float) : sizeof(double);
+    size_t& bytes = H.bytes;
     for (size_t l = 0; l < H.host.size(); ++l) {
         GLevelHost& L = H.host[l];
         GLevelDev& D = H.dev[l];
@@ -1490,43 +1544,110 @@ int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[
                 L.af[0].clear(); L.af[1].clear(); L.af[2].clear(); L.sg.clear();
             }
         }
-        const size_t BN = (size_t)H.B * L.N;
-        if (l > 0) {
-            OPF_HIP(hipMalloc(&D.x, ts * BN));
-            OPF_HIP(hipMalloc(&D.b, ts * BN));
-            bytes += 2 * ts * BN;
-        }
-        OPF_HIP(hipMalloc(&D.t, ts * BN));
-        bytes += ts * BN;
     }
-    const size_t BN = (size_t)H.B * fine.N;
-    OPF_HIP(hipMalloc(&H.r, ts * BN));
-    OPF_HIP(hipMalloc(&H.z, ts * BN));
-    OPF_HIP(hipMalloc(&H.p[0], ts * BN));
-    OPF_HIP(hipMalloc(&H.p[1], ts * BN));
-    OPF_HIP(hipMalloc(&H.q, ts * BN));
-    bytes += 5 * ts * BN;
-    OPF_HIP(hipMalloc(&H.e, ts * BN));  // inner correction (mixed) / serving correction
-    bytes += ts * BN;
-    OPF_HIP(hipMalloc((void**)&H.r64, sizeof(double) * BN));
-    bytes += sizeof(double) * BN;
-    OPF_HIP(hipMalloc((void**)&H.part, sizeof(double) * GMG_PSTRIDE * H.B));
-    OPF_HIP(hipMalloc((void**)&H.scal, sizeof(double) * GMG_SSTRIDE * H.B));
-    OPF_HIP(hipMemset(H.scal, 0, sizeof(double) * GMG_SSTRIDE * H.B));
-    OPF_HIP(hipHostMalloc((void**)&H.hscal, sizeof(double) * GMG_SSTRIDE * H.B, hipHostMallocDefault));
-    OPF_HIP(hipMalloc((void**)&H.mk, sizeof(int) * MK_COUNT * H.B));
-    OPF_HIP(hipMemset(H.mk, 0, sizeof(int) * MK_COUNT * H.B));
+    OPF_TRY(hier_vectors(H));
     (void)ctx;
     return OPF_OK;
+}
+
+int gmg_build_dev(opf_ctx* ctx, Hierarchy& H, const DevLevel& fine, const int per[3], const double alpha[3][2],
+                  int precision, int slots) {
+    hipStream_t st = ctx->stream;
+    GLevelHost F{};
+    for (int d = 0; d < 3; ++d) {
+        F.n[d] = fine.n[d];
+        F.a[d] = fine.a[d];
+    }
+    F.N = fine.N;
+    F.var = fine.var;
+    F.sigma = fine.sigma;
+    // an existing hierarchy of the same layout keeps its buffers (and graph)
+    bool same = !H.dev.empty() && H.var == fine.var && H.precision == precision && H.B == std::max(1, slots);
+    for (int d = 0; d < 3 && same; ++d)
+        same &= H.host[0].n[d] == fine.n[d] && H.per[d] == per[d] && H.alpha[d][0] == alpha[d][0] &&
+                H.alpha[d][1] == alpha[d][1];
+    if (same && fine.var) {
+        same = H.dev[0].borrowed && H.dev[0].sg64 == fine.sg;
+        for (int d = 0; d < 3; ++d) same &= H.dev[0].af64[d] == fine.af[d];
+        bool anchored = false;
+        for (int d = 0; d < H.dim; ++d)
+            for (int s = 0; s < 2; ++s) anchored |= !per[d] && alpha[d][s] != 1.0;
+        same &= H.singular == (!fine.shift && !anchored);
+    }
+    if (same && !fine.var) {
+        same = H.host[0].sigma == fine.sigma;
+        for (int d = 0; d < 3; ++d) same &= H.host[0].a[d] == fine.a[d];
+        if (same) return OPF_OK;  // constant mode: the same operator
+    }
+    if (!same) {
+        const int pre = H.pre, post = H.post;
+        if (!H.dev.empty()) gmg_free(H);
+        H = Hierarchy{};
+        H.pre = pre;
+        H.post = post;
+    }
+    if (!fine.var) return gmg_build(ctx, H, F, per, alpha, precision, slots);  // two numbers per level
+    const bool mixed = precision == OPF_PREC_MIXED;
+    if (!same) {
+        OPF_TRY(hier_levels(H, F, per, alpha, precision, slots, fine.shift, false));
+        H.bytes = 0;
+        H.dev.assign(H.host.size(), GLevelDev{});
+        for (size_t l = 0; l < H.host.size(); ++l) {
+            const GLevelHost& L = H.host[l];
+            GLevelDev& D = H.dev[l];
+            for (int d = 0; d < 3; ++d) D.n[d] = L.n[d];
+            D.N = L.N;
+            if (l == 0) {
+                D.borrowed = true;
+                for (int d = 0; d < 3; ++d) D.af64[d] = fine.af[d];
+                D.sg64 = fine.sg;
+            } else {
+                for (int d = 0; d < H.dim; ++d) OPF_TRY(alloc<double>((void**)&D.af64[d], face_count(L.n, d), H.bytes));
+                OPF_TRY(alloc<double>((void**)&D.sg64, L.N, H.bytes));
+            }
+            if (mixed) {
+                for (int d = 0; d < H.dim; ++d) OPF_TRY(alloc<float>((void**)&D.af32[d], face_count(L.n, d), H.bytes));
+                OPF_TRY(alloc<float>((void**)&D.sg32, L.N, H.bytes));
+            }
+        }
+        OPF_TRY(hier_vectors(H));
+    }
+    // coefficients of the coarse levels, FP32 copies, coarsest inverse
+    for (size_t l = 1; l < H.host.size(); ++l) {
+        const GLevelDev &P = H.dev[l - 1], &D = H.dev[l];
+        const double* faf[3] = {P.af64[0], P.af64[1], P.af64[2]};
+        double* const caf[3] = {D.af64[0], D.af64[1], D.af64[2]};
+        coarsen_dev(st, H.dim, P.n, faf, P.sg64, D.n, caf, D.sg64);
+    }
+    if (mixed)
+        for (size_t l = 0; l < H.host.size(); ++l) {
+            const GLevelDev& D = H.dev[l];
+            for (int d = 0; d < H.dim; ++d) to_f32_dev(st, face_count(D.n, d), D.af64[d], D.af32[d]);
+            to_f32_dev(st, D.N, D.sg64, D.sg32);
+        }
+    OPF_HIP(hipGetLastError());
+    GLevelHost& Lc = H.host.back();
+    const GLevelDev& Dc = H.dev.back();
+    for (int d = 0; d < H.dim; ++d) {
+        Lc.af[d].resize(face_count(Lc.n, d));
+        OPF_HIP(hipMemcpyAsync(Lc.af[d].data(), Dc.af64[d], sizeof(double) * Lc.af[d].size(), hipMemcpyDeviceToHost, st));
+    }
+    Lc.sg.resize(Lc.N);
+    OPF_HIP(hipMemcpyAsync(Lc.sg.data(), Dc.sg64, sizeof(double) * Lc.N, hipMemcpyDeviceToHost, st));
+    OPF_HIP(hipStreamSynchronize(st));
+    int rc = hier_coarsest(H);
+    for (int d = 0; d < 3; ++d) std::vector<double>().swap(Lc.af[d]);
+    std::vector<double>().swap(Lc.sg);
+    return rc;
 }
 
 void gmg_free(Hierarchy& H) {
     for (auto& D : H.dev) {
         for (int d = 0; d < 3; ++d) {
-            if (D.af64[d]) hipFree(D.af64[d]);
+            if (D.af64[d] && !D.borrowed) hipFree(D.af64[d]);
             if (D.af32[d]) hipFree(D.af32[d]);
         }
-        if (D.sg64) hipFree(D.sg64);
+        if (D.sg64 && !D.borrowed) hipFree(D.sg64);
         if (D.sg32) hipFree(D.sg32);
         if (D.x) hipFree(D.x);
         if (D.b) hipFree(D.b);
@@ -1648,9 +1769,9 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         if (zp)
             zp_launch<T, ZP_JAC2Z>(H, l, st, bt, rhs, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
         else if (zm && H.dim == 3)
-            hipLaunchKernelGGL((zk_jacobi2z<T, 3>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
+            hipLaunchKernelGGL((zk_jacobi2z<T, 3>), dim3(zgrid(zt), B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
         else if (zm)
-            hipLaunchKernelGGL((zk_jacobi2z<T, 2>), dim3(zt.items, B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
+            hipLaunchKernelGGL((zk_jacobi2z<T, 2>), dim3(zgrid(zt), B), dim3(256), 0, st, o, rhs, tmp, w, zt, bt);
         else if (H.dim == 3)
             hipLaunchKernelGGL((k_jacobi2z<T, 3>), dim3(nblk(N), B), dim3(256), 0, st, o, rhs, tmp, w, bt);
         else
@@ -1665,7 +1786,7 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         if (zp)
             zp_launch<T, ZP_JAC>(H, l, st, bt, (const T*)cur, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
         else if (zm)
-            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
+            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zgrid(zt), B), dim3(256), 0, st, o,
                                                           (const T*)cur, rhs, tmp, w, zt, bt));
         else
             GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256),
@@ -1723,7 +1844,7 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         } else if (zp) {
             zp_launch<T, ZP_JAC>(H, l, st, bt, (const T*)cur, nullptr, rhs, nullptr, tmp, (T*)nullptr, nullptr);
         } else if (zm) {
-            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zt.items, B), dim3(256), 0, st, o,
+            GMG_DISPATCH(false, H.dim, hipLaunchKernelGGL((zk_jacobi<T, D_>), dim3(zgrid(zt), B), dim3(256), 0, st, o,
                                                           (const T*)cur, rhs, tmp, w, zt, bt));
         } else {
             GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_jacobi<T, V_, D_, false>), dim3(nblk(N), B), dim3(256),
@@ -1735,10 +1856,11 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
 
 // finish a reduction whose partials came from a finest-level launch in precision T
 template <typename T>
-static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular) {
+static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular,
+                      int* m_it = nullptr, int* m_rep = nullptr, int maxit = 0) {
     const long long N = H.host[0].N;
     hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, red_count<T>(H), mode, H.scal,
-                       1.0 / N, singular, bat(H, mask, N));
+                       1.0 / N, singular, bat(H, mask, N), m_it, m_rep, maxit);
 }
 
 // start of an inner FCG solve on r: p_old = 0, z = M r, rz = (r, z)
@@ -1797,7 +1919,7 @@ void ph_fcg_iter(const Hierarchy& H, hipStream_t st, const int* mask, T* e, int 
 // r_T = b - A x in FP64 (x may be null), partial sums of r^2 -> finish `mode`
 template <typename TO>
 void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const double* x, const double* b, TO* r,
-                int mode) {
+                int mode, int* m_it = nullptr, int maxit = 0) {
     GOp<double> o = make_op<double>(H, 0);
     const long long N = H.host[0].N;
     if (x && use_zp(H, 0))
@@ -1810,7 +1932,7 @@ void ph_resid64(const Hierarchy& H, hipStream_t st, const int* mask, const doubl
     else
         GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_resid<double, TO, V_, D_>), dim3(red_count<double>(H), (unsigned)H.B),
                                                       dim3(256), 0, st, o, x, b, r, H.part, bat(H, mask, N)));
-    ph_finish<double>(H, st, mask, mode, H.singular);
+    ph_finish<double>(H, st, mask, mode, H.singular, m_it, nullptr, maxit);
 }
 
 template <typename T>
@@ -1876,13 +1998,14 @@ void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* 
     ph_finish<T>(H, st, m_it, FIN_PQ, H.singular);
     hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
                        bt);
-    ph_finish<T>(H, st, m_it, FIN_RR, H.singular);
-    // reliable update: x += e, e = 0, r = b - A x in FP64 (the direction p is kept)
-    hipLaunchKernelGGL(k_rel_decide, dim3(slot_blocks(H)), dim3(64), 0, st, B, (const int*)m_it, m_rep, H.scal, maxit);
-    hipLaunchKernelGGL(k_add_zero<T>, dim3(nblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e, bat(H, m_rep, N));
-    ph_resid64<T>(H, st, m_rep, x64, b64, r, FIN_REP);
+    // (r, r), the iteration counted and the reliable-update decision (sets m_rep)
+    ph_finish<T>(H, st, nullptr, FIN_RR_DECIDE, H.singular, m_it, m_rep, maxit);
+    // reliable update: x += e, e = 0, r = b - A x in FP64 (the direction p is kept); its
+    // finish stops the slots that meet the tolerance (clears m_it)
+    hipLaunchKernelGGL(k_add_zero<T>, dim3((unsigned)rblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e,
+                       bat(H, m_rep, N));
+    ph_resid64<T>(H, st, m_rep, x64, b64, r, FIN_REP_DONE, m_it, maxit);
     if (H.singular) ph_project<T>(H, st, m_rep, r);
-    hipLaunchKernelGGL(k_rel_done, dim3(slot_blocks(H)), dim3(64), 0, st, B, m_it, (const int*)m_rep, H.scal, maxit);
     // next preconditioned residual for the slots still iterating
     if (H.post > 0) {
         ph_vcycle<T>(H, st, m_it, 0, (const T*)r, z, (const T*)q);

@@ -49,10 +49,24 @@ struct GLevelHost {
     std::vector<double> af[3], sg;  // variable mode
 };
 
+// A level built on the device (rows_to_level_dev): constant mode (a, sigma) or
+// variable mode (FP64 device arrays af over the faces of each dim, sg per cell).
+struct DevLevel {
+    int n[3] = {1, 1, 1};
+    long long N = 0;
+    bool var = false;
+    bool shift = false;  // some cell has a nonzero sigma
+    double a[3] = {0, 0, 0}, sigma = 0;
+    double* af[3] = {nullptr, nullptr, nullptr};
+    double* sg = nullptr;
+};
+
 struct GLevelDev {
     int n[3];
     long long N;
-    // coefficients in both precisions (variable mode arrays)
+    // coefficients in both precisions (variable mode arrays); borrowed: af64 / sg64
+    // belong to the caller's DevLevel and are not freed with the hierarchy
+    bool borrowed = false;
     double* af64[3] = {nullptr, nullptr, nullptr};
     float* af32[3] = {nullptr, nullptr, nullptr};
     double* sg64 = nullptr;
@@ -101,6 +115,34 @@ struct Hierarchy {
 
 int gmg_build(opf_ctx* ctx, Hierarchy& H, const GLevelHost& fine, const int per[3], const double alpha[3][2],
               int precision, int slots = 1);
+// Hierarchy over a level built on the device: the coarse levels are coarsened by
+// kernels and a variable-mode fine level's arrays are borrowed (they must outlive
+// H). When H already holds a variable-mode hierarchy of the same shape, precision,
+// slots, boundaries and singularity over the same arrays, its buffers (and a
+// captured iteration graph) are kept and only the coefficients are recomputed.
+int gmg_build_dev(opf_ctx* ctx, Hierarchy& H, const DevLevel& fine, const int per[3], const double alpha[3][2],
+                  int precision, int slots = 1);
+
+// ---- device level construction (level.hip)
+// face arrays of dim d have extents n + e_d
+__host__ __device__ __forceinline__ long long face_index(int d, int i, int j, int k, int n0, int n1) {
+    int f0 = n0 + (d == 0), f1 = n1 + (d == 1);
+    return i + (long long)f0 * (j + (long long)f1 * k);
+}
+long long level_face_count(const int n[3], int d);
+// The rows of one diagonal block (PAD_K device arrays of N rows) as a level:
+// face coefficients from the neighbour slots, the rest of the oriented diagonal as
+// sigma. *sign orients the diagonal positive. A face whose two rows disagree is
+// given their mean (the level is then the operator's symmetric part) and
+// *symmetric is false. L.af / L.sg are allocated unless already set (reused); a
+// uniform operator comes back in constant mode with its arrays still allocated.
+int rows_to_level_dev(hipStream_t st, const double* coef, long long N, const int nn[3], int dim, const int per[3],
+                      double* sign, DevLevel& L, bool* symmetric);
+int dev_levels_equal(hipStream_t st, const DevLevel& a, const DevLevel& b, int dim, bool* eq);
+void dev_level_free(DevLevel& L);
+void coarsen_dev(hipStream_t st, int dim, const int fn[3], const double* const faf[3], const double* fsg,
+                 const int cn[3], double* const caf[3], double* csg);
+void to_f32_dev(hipStream_t st, long long n, const double* src, float* dst);
 int read_scal(const Hierarchy& H, hipStream_t st);
 // Krylov solve with reliable updates, batched over the slots of H (vectors slot-major).
 // The Krylov vectors and the V-cycle are in precision T; x64 / b64 are the FP64
@@ -143,7 +185,10 @@ enum {
     S_N
 };
 static_assert(S_N <= GMG_SSTRIDE, "scalars per slot");
-enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0, FIN_REP, FIN_MEAN };
+// FIN_RR_DECIDE: FIN_RR plus the reliable-update decision of the iteration (it counts
+// the iteration and sets the slot's MK_REP flag); FIN_REP_DONE: FIN_REP plus the stop
+// decision after the update's true residual (clears MK_IT of a finished slot)
+enum { FIN_PQ = 0, FIN_RR, FIN_DOT3, FIN_DOT3_FIRST, FIN_BB, FIN_RR0, FIN_REP, FIN_MEAN, FIN_RR_DECIDE, FIN_REP_DONE };
 // device masks per slot, rows of B ints (Hierarchy::mk)
 enum { MK_NEW = 0, MK_IT, MK_REP, MK_START, MK_COUNT };
 void gmg_free(Hierarchy& H);

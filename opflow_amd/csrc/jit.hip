@@ -43,6 +43,9 @@ const char* const kArgsSrc =
 const char* const kWenoSrc =
 #include "weno_device.inc"
     ;
+const char* const kMathSrc =
+#include "math_device.inc"
+    ;
 #undef OPF_SHARED_SRC
 
 const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
@@ -61,6 +64,40 @@ std::string hexkey(uint64_t h) {
     return buf;
 }
 
+// Values a generator knows to be s * h_d, a power-of-two multiple s of the uniform
+// power-of-two spacing h_d of dim d (a dx of that dim, sums of two such, scalings by
+// a power-of-two literal), and literal constants. A division by s * h_d is emitted
+// as the multiplication by 1 / (s * h_d): both round the same real number, so the
+// bits are the reference's.
+struct Known {
+    std::vector<signed char> d;  // dim of a known multiple, -1 none
+    std::vector<double> s;       // its multiple
+    std::vector<char> lit;       // a literal constant
+    std::vector<double> c;       // its value
+    explicit Known(size_t n) : d(n, -1), s(n, 0.0), lit(n, 0), c(n, 0.0) {}
+    static bool pow2(double v, int lo, int hi) {
+        int e;
+        return v > 0 && std::frexp(v, &e) == 0.5 && e - 1 >= lo && e - 1 <= hi;
+    }
+    void dx(int id, int dim) { d[id] = (signed char)dim, s[id] = 1.0; }
+    void literal(int id, double v) { lit[id] = 1, c[id] = v; }
+    // result id of x op y
+    void arith(int op, int id, int x, int y) {
+        if (op == I_ADD && d[x] >= 0 && d[x] == d[y] && pow2(s[x] + s[y], -2, 2)) d[id] = d[x], s[id] = s[x] + s[y];
+        if (op == I_MUL) {
+            if (d[x] >= 0 && lit[y] && pow2(s[x] * c[y], -2, 2)) d[id] = d[x], s[id] = s[x] * c[y];
+            if (d[y] >= 0 && lit[x] && pow2(s[y] * c[x], -2, 2)) d[id] = d[y], s[id] = s[y] * c[x];
+        }
+    }
+    // the multiplication replacing x / y, or an empty string
+    std::string div(int x, int y) const {
+        if (d[y] < 0) return "";
+        char b[96];
+        snprintf(b, sizeof b, "v%d * ((T)a.rdx[%d] * (T)%a)", x, d[y], 1.0 / s[y]);
+        return b;
+    }
+};
+
 // the straight-line kernel of program P in precision T, writing out (op) value
 int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, std::string& s) {
     s.clear();
@@ -68,6 +105,7 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
     s += kArgsSrc;
     s += "\n";
     s += kWenoSrc;
+    s += kMathSrc;
     {
         char hdr[512];
         snprintf(hdr, sizeof hdr,
@@ -115,6 +153,7 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
     std::vector<int> st;  // stack of value ids
     std::vector<int> res;  // result value of each of the R cells
     const int NV = (int)P.code.size();
+    Known kn((size_t)R * NV);
     auto pop = [&](int& v) -> bool {
         if (st.empty()) return false;
         v = st.back();
@@ -142,9 +181,15 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
                     long long bits;
                     std::memcpy(&bits, &in.imm, 8);
                     snprintf(line, sizeof line, "  const T v%d = (T)__longlong_as_double(%lldLL);\n", id, bits);
+                    kn.literal(id, in.imm);
                 } break;
                 case I_DX:
-                    snprintf(line, sizeof line, "  const T v%d = DX%d[%d];\n", id, in.a, in.b + (in.a == 2 ? r : 0));
+                    if (a.umask >> in.a & 1) {
+                        snprintf(line, sizeof line, "  const T v%d = (T)a.hdx[%d];\n", id, in.a);
+                        kn.dx(id, in.a);
+                    } else {
+                        snprintf(line, sizeof line, "  const T v%d = DX%d[%d];\n", id, in.a, in.b + (in.a == 2 ? r : 0));
+                    }
                     break;
                 case I_X:
                     snprintf(line, sizeof line, "  const T v%d = X%d[%d];\n", id, in.a, in.b + (in.a == 2 ? r : 0));
@@ -156,7 +201,12 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
                     int x, y;
                     if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
                     const char o = in.op == I_ADD ? '+' : in.op == I_SUB ? '-' : in.op == I_MUL ? '*' : '/';
-                    snprintf(line, sizeof line, "  const T v%d = v%d %c v%d;\n", id, x, o, y);
+                    const std::string mul = in.op == I_DIV ? kn.div(x, y) : std::string();
+                    if (!mul.empty())
+                        snprintf(line, sizeof line, "  const T v%d = %s;\n", id, mul.c_str());
+                    else
+                        snprintf(line, sizeof line, "  const T v%d = v%d %c v%d;\n", id, x, o, y);
+                    kn.arith(in.op, id, x, y);
                 } break;
                 case I_NEG: {
                     int x;
@@ -183,6 +233,16 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
                     int x;
                     if (!pop(x)) return OPF_ERR_ARG;
                     snprintf(line, sizeof line, "  const T v%d = (T)fabs((double)v%d);\n", id, x);
+                } break;
+                case I_FN1: {
+                    int x;
+                    if (!pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "  const T v%d = opf_fn1<T>(%d, v%d);\n", id, in.a, x);
+                } break;
+                case I_FN2: {
+                    int x, y;
+                    if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "  const T v%d = opf_fn2<T>(%d, v%d, v%d);\n", id, in.a, x, y);
                 } break;
                 case I_SEL: {
                     int c, t_, f_;
@@ -233,7 +293,7 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
 // ---------------------------------------------------------------------------
 // Tiled plane-marching form. A 64 x NTY workgroup owns a 64 x TY tile of the
 // x-y plane (TY = NTY * RY: each thread RY rows of one column) and marches a
-// chunk of ZC planes in z. Every field the program reads is staged in LDS as a
+// chunk of a.zc planes in z (a launch argument). Every field the program reads is staged in LDS as a
 // ring of planes covering the tile plus the program's x/y read offsets; the
 // ring holds the planes at the z offsets the program reads plus one, so one
 // barrier per plane separates a plane's store from its reads (the slot written
@@ -279,24 +339,25 @@ bool tile_layout(const LoweredProgram& P, bool plane, int RY, TileField (&tf)[SW
     return true;
 }
 
-int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, bool plane,
+int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, bool plane, int umask,
                     const TileField (&tf)[SWEEP_MAX_FIELDS], std::string& s) {
     s.clear();
     s += dtype == OPF_F64 ? "typedef double T;\n" : "typedef float T;\n";
     s += kArgsSrc;
     s += "\n";
     s += kWenoSrc;
+    s += kMathSrc;
     const int TY = kTileNTY * RY;
     char line[640];
     snprintf(line, sizeof line,
              "\nextern \"C\" __global__ __launch_bounds__(256) void opf_sweep_jit(SweepArgs a) {\n"
              "  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;\n"
              "  const int x0 = a.s[0] + (int)blockIdx.x * 64, y0 = a.s[1] + (int)blockIdx.y * %d;\n"
-             "  const int kb = a.s[2] + (int)blockIdx.z * %d;\n"
-             "  const int ke = min(kb + %d, a.s[2] + a.n[2]);\n"
+             "  const int kb = a.s[2] + (int)blockIdx.z * a.zc;\n"
+             "  const int ke = min(kb + a.zc, a.s[2] + a.n[2]);\n"
              "  const int xe = a.s[0] + a.n[0], ye = a.s[1] + a.n[1];\n"
              "  const int i = x0 + tx, jb = y0 + ty * %d;\n",
-             TY, ZC, ZC, RY);
+             TY, RY);
     s += line;
     bool dxu[3] = {}, xu[3] = {};
     for (const Insn& in : P.code) {
@@ -422,6 +483,7 @@ int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, 
     std::vector<int> st;
     std::vector<int> res;
     const int NV = (int)P.code.size();
+    Known kn((size_t)RY * NV);
     auto pop = [&](int& v) -> bool {
         if (st.empty()) return false;
         v = st.back();
@@ -448,9 +510,15 @@ int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, 
                     long long bits;
                     std::memcpy(&bits, &in.imm, 8);
                     snprintf(line, sizeof line, "    const T v%d = (T)__longlong_as_double(%lldLL);\n", id, bits);
+                    kn.literal(id, in.imm);
                 } break;
                 case I_DX:
                 case I_X: {
+                    if (in.op == I_DX && (umask >> in.a & 1)) {
+                        snprintf(line, sizeof line, "    const T v%d = (T)a.hdx[%d];\n", id, in.a);
+                        kn.dx(id, in.a);
+                        break;
+                    }
                     const char* tb = in.op == I_DX ? "DX" : "X";
                     if (in.a == 1) snprintf(line, sizeof line, "    const T v%d = %s1_%d[%d];\n", id, tb, r, in.b);
                     else snprintf(line, sizeof line, "    const T v%d = %s%d[%d];\n", id, tb, in.a, in.b);
@@ -462,7 +530,12 @@ int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, 
                     int x, y;
                     if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
                     const char o = in.op == I_ADD ? '+' : in.op == I_SUB ? '-' : in.op == I_MUL ? '*' : '/';
-                    snprintf(line, sizeof line, "    const T v%d = v%d %c v%d;\n", id, x, o, y);
+                    const std::string mul = in.op == I_DIV ? kn.div(x, y) : std::string();
+                    if (!mul.empty())
+                        snprintf(line, sizeof line, "    const T v%d = %s;\n", id, mul.c_str());
+                    else
+                        snprintf(line, sizeof line, "    const T v%d = v%d %c v%d;\n", id, x, o, y);
+                    kn.arith(in.op, id, x, y);
                 } break;
                 case I_NEG: {
                     int x;
@@ -489,6 +562,16 @@ int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, int ZC, 
                     int x;
                     if (!pop(x)) return OPF_ERR_ARG;
                     snprintf(line, sizeof line, "      const T v%d = (T)fabs((double)v%d);\n", id, x);
+                } break;
+                case I_FN1: {
+                    int x;
+                    if (!pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "      const T v%d = opf_fn1<T>(%d, v%d);\n", id, in.a, x);
+                } break;
+                case I_FN2: {
+                    int x, y;
+                    if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                    snprintf(line, sizeof line, "      const T v%d = opf_fn2<T>(%d, v%d, v%d);\n", id, in.a, x, y);
                 } break;
                 case I_SEL: {
                     int c, t_, f_;
@@ -661,18 +744,12 @@ void jit_stats(long long* compiles, long long* loaded, long long* resident) {
 }
 
 namespace {
-int function_for_source(const std::string& src, hipFunction_t* fn);
+int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out = nullptr);
 }  // namespace
-
-int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, hipFunction_t* fn) {
-    std::string src;
-    OPF_CHECK(gen_source(P, dtype, op, R, a, src) == OPF_OK, OPF_ERR_ARG, "jit: malformed sweep program");
-    return function_for_source(src, fn);
-}
 
 namespace {
 // the loaded kernel of a generated source: in-process cache, disk cache, then hipRTC
-int function_for_source(const std::string& src, hipFunction_t* fn) {
+int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out) {
     int dev = 0;
     OPF_HIP(hipGetDevice(&dev));
     uint64_t h = fnv1a64(src.data(), src.size());
@@ -680,6 +757,7 @@ int function_for_source(const std::string& src, hipFunction_t* fn) {
     int ver[2] = {0, 0};  // a compiler update invalidates the disk cache
     if (hiprtcVersion(&ver[0], &ver[1]) == HIPRTC_SUCCESS) h = fnv1a64(ver, sizeof ver, h);
     const uint64_t key = fnv1a64(&dev, sizeof dev, h);
+    if (key_out) *key_out = key;
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_fns.find(key);
     if (it != g_fns.end()) {
@@ -712,13 +790,85 @@ int function_for_source(const std::string& src, hipFunction_t* fn) {
 }
 }  // namespace
 
-int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a) {
+namespace {
+// What a launch needs once its source has been generated and loaded: the module
+// (by its key in g_fns) and the form's rows per thread. Keyed by everything the
+// generated source depends on, so a repeated sweep skips generating and hashing it.
+struct Plan {
+    uint64_t mod;
+    bool tile;
+    int rows;
+};
+std::unordered_map<uint64_t, Plan> g_plans;
+
+uint64_t plan_key(const LoweredProgram& P, int dtype, const SweepArgs& a, int dev, int form) {
+    uint64_t h = 1469598103934665603ULL;
+    for (const Insn& in : P.code) {
+        const int w[4] = {in.op, in.a, in.b, in.c};
+        h = fnv1a64(w, sizeof w, h);
+        if (!(in.op == I_CONST && in.a)) h = fnv1a64(&in.imm, sizeof in.imm, h);
+    }
+    const int q[6] = {dtype, a.op, a.umask, a.n[2] == 1, a.n[2] % 4, dev * 4 + form};
+    h = fnv1a64(q, sizeof q, h);
+    for (const FieldArg& f : a.f) {  // the direct form bakes plane strides into its loads
+        const long long ps = (long long)f.e0 * f.e1;
+        h = fnv1a64(&ps, sizeof ps, h);
+    }
+    return h;
+}
+
+}  // namespace
+
+int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a, int form) {
     hipFunction_t fn;
-    // tiled form: fields staged in LDS rings, 4 / 2 / 1 rows per thread by LDS need
     const bool plane = a.n[2] == 1;
-    constexpr int kZC = 32;
+    int dev = 0;
+    OPF_HIP(hipGetDevice(&dev));
+    // the tiled form needs enough workgroups to fill the device: tiles of 64 x 16 cells
+    // and chunks of 32 .. 8 planes, at least 512 workgroups (else the direct form)
+    const long long tiles = (long long)((a.n[0] + 63) / 64) * ((a.n[1] + 15) / 16);
+    if (form == 0 && tiles * (plane ? 1 : (a.n[2] + 7) / 8) < 512) form = 1;
+    const uint64_t pk = plan_key(P, dtype, a, dev, form);
+    auto launch = [&](const Plan& pl) -> int {
+        SweepArgs args = a;
+        void* params[] = {&args};
+        if (pl.tile) {
+            const int TY = kTileNTY * pl.rows;
+            const long long t2 = (long long)((a.n[0] + 63) / 64) * ((a.n[1] + TY - 1) / TY);
+            int zc = 32;
+            while (zc > 8 && t2 * ((a.n[2] + zc - 1) / zc) < 2048) zc /= 2;
+            args.zc = plane ? 1 : zc;
+            OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((a.n[0] + 63) / 64), (unsigned)((a.n[1] + TY - 1) / TY),
+                                          (unsigned)(plane ? 1 : (a.n[2] + zc - 1) / zc), 64, kTileNTY, 1, 0, st,
+                                          params, nullptr));
+        } else {
+            OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((a.n[0] + 63) / 64), (unsigned)((a.n[1] + 3) / 4),
+                                          (unsigned)(a.n[2] / pl.rows), 64, 4, 1, 0, st, params, nullptr));
+        }
+        return OPF_OK;
+    };
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_plans.find(pk);
+        if (it != g_plans.end()) {
+            auto m = g_fns.find(it->second.mod);
+            if (m != g_fns.end()) {
+                m->second.used = ++g_tick;
+                fn = m->second.fn;
+                const Plan pl = it->second;
+                return launch(pl);
+            }
+            g_plans.erase(it);  // its module was unloaded
+        }
+        if (g_plans.size() > 4 * kMaxModules) g_plans.clear();
+    }
+    auto remember = [&](const Plan& pl) {
+        std::lock_guard<std::mutex> lock(g_mu);
+        g_plans[pk] = pl;
+    };
+    // tiled form: fields staged in LDS rings, 4 / 2 / 1 rows per thread by LDS need
     const int tsize = dtype == OPF_F64 ? 8 : 4;
-    for (int RY = 4; RY >= 1; RY /= 2) {
+    for (int RY = 4; RY >= 1 && form != 1; RY /= 2) {
         TileField tf[SWEEP_MAX_FIELDS];
         size_t lds = 0;
         if (!tile_layout(P, plane, RY, tf, &lds, tsize)) break;
@@ -726,16 +876,13 @@ int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const S
         for (auto& f : tf) nq += f.used ? f.NQ : 0;
         if (lds > (size_t)64 * 1024 || nq > 24 || P.code.size() * RY > (size_t)SWEEP_JIT_MAX_INSNS) continue;
         std::string src;
-        OPF_CHECK(gen_source_tile(P, dtype, a.op, RY, kZC, plane, tf, src) == OPF_OK, OPF_ERR_ARG,
+        OPF_CHECK(gen_source_tile(P, dtype, a.op, RY, plane, a.umask, tf, src) == OPF_OK, OPF_ERR_ARG,
                   "jit: malformed sweep program");
-        OPF_TRY(function_for_source(src, &fn));
-        SweepArgs args = a;
-        void* params[] = {&args};
-        const int TY = kTileNTY * RY;
-        OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((a.n[0] + 63) / 64), (unsigned)((a.n[1] + TY - 1) / TY),
-                                      (unsigned)(plane ? 1 : (a.n[2] + kZC - 1) / kZC), 64, kTileNTY, 1, 0, st, params,
-                                      nullptr));
-        return OPF_OK;
+        uint64_t mk = 0;
+        OPF_TRY(function_for_source(src, &fn, &mk));
+        const Plan pl{mk, true, RY};
+        remember(pl);
+        return launch(pl);
     }
     // direct form: loads from global memory
     // each thread computes R cells along z (R = 4, 2 or 1, dividing the z extent):
@@ -743,12 +890,13 @@ int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const S
     int R = 1;
     for (int c = 4; c > 1; c /= 2)
         if (a.n[2] % c == 0 && P.code.size() * c <= (size_t)SWEEP_JIT_MAX_INSNS) { R = c; break; }
-    OPF_TRY(jit_sweep_function(P, dtype, a.op, R, a, &fn));
-    SweepArgs args = a;
-    void* params[] = {&args};
-    OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((a.n[0] + 63) / 64), (unsigned)((a.n[1] + 3) / 4), (unsigned)(a.n[2] / R),
-                                  64, 4, 1, 0, st, params, nullptr));
-    return OPF_OK;
+    std::string src;
+    OPF_CHECK(gen_source(P, dtype, a.op, R, a, src) == OPF_OK, OPF_ERR_ARG, "jit: malformed sweep program");
+    uint64_t mk = 0;
+    OPF_TRY(function_for_source(src, &fn, &mk));
+    const Plan pl{mk, false, R};
+    remember(pl);
+    return launch(pl);
 }
 
 }  // namespace opf

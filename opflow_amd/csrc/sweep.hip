@@ -18,6 +18,9 @@ using namespace opf;
 
 namespace opf {
 
+// OpFlow's PI (Core/Constants.hpp:37)
+constexpr double kPi = 3.141592653589793;
+
 struct Lowerer {
     const opf_expr_node* nodes;
     int nnodes;
@@ -58,6 +61,7 @@ struct Lowerer {
         code.push_back({I_CONST, 0, 0, 0, v});
         push(1);
     }
+    void fn1(int f) { code.push_back({I_FN1, f, 0, 0, 0.0}); }
     // a ScalarExpr node: one launch-time slot per node (the node may be inlined at
     // several shifts); past SWEEP_MAX_CONSTS distinct nodes the value is baked in
     std::vector<int> cslot;
@@ -118,6 +122,25 @@ struct Lowerer {
                     if (!kid(e.a)) return OPF_ERR_ARG;
                     locs[n] = locs[e.a];
                     break;
+                case OPF_N_FUNC1:
+                    if (!kid(e.a) || e.dim < 0 || e.dim > OPF_FN_SMOOTH_HEAVISIDE) return OPF_ERR_ARG;
+                    if (e.dim >= OPF_FN_SMOOTH_DELTA && !(e.value > 0)) return OPF_ERR_ARG;
+                    locs[n] = locs[e.a];
+                    break;
+                case OPF_N_FUNC2:
+                    if (!kid(e.a) || !kid(e.b) || e.dim < 0 || e.dim > OPF_FN2_NEXTAFTER) return OPF_ERR_ARG;
+                    locs[n] = locs[e.a][0] >= 0 ? locs[e.a] : locs[e.b];
+                    break;
+                case OPF_N_CONV: {  // Convolution::prepare: the argument's location
+                    const int k[3] = {e.dim & 255, (e.dim >> 8) & 255, (e.dim >> 16) & 255};
+                    const int K = k[0] * k[1] * k[2];
+                    if (!kid(e.a) || K < 1 || e.b < 0 || e.b + K > n) return OPF_ERR_ARG;
+                    for (int d = 0; d < 3; ++d)
+                        if (k[d] % 2 == 0 || (d >= mesh->dim && k[d] != 1)) return OPF_ERR_ARG;
+                    for (int q = 0; q < K; ++q)
+                        if (nodes[e.b + q].kind != OPF_N_CONST) return OPF_ERR_ARG;
+                    locs[n] = locs[e.a];
+                } break;
                 case OPF_N_FLUXLIM: {
                     // the interpolated argument b decides the direction (D1FluxLimiter.hpp prepare)
                     if (!kid(e.a) || !kid(e.b) || e.dim < 0 || e.dim >= mesh->dim) return OPF_ERR_ARG;
@@ -294,6 +317,56 @@ struct Lowerer {
             case OPF_N_GE: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_GE); break;
             case OPF_N_LE: emit(e.a, s0, s1, s2); emit(e.b, s0, s1, s2); op(I_LE); break;
             case OPF_N_ABS: emit(e.a, s0, s1, s2); op(I_ABS); break;
+            case OPF_N_FUNC1:
+                if (e.dim == OPF_FN_SMOOTH_DELTA) {
+                    // Math::smoothDelta (Numeric.hpp:94-96):
+                    //   std::max(d, -d) > eps ? 0. : 1. / 2. / eps * (1 + std::cos(d * PI / eps))
+                    const double eps = e.value;
+                    emit(e.a, s0, s1, s2); emit(e.a, s0, s1, s2); op(I_NEG); op(I_MAX); cst(eps); op(I_GT);
+                    cst(0.0);
+                    cst(1. / 2. / eps);
+                    cst(1.0); emit(e.a, s0, s1, s2); cst(kPi); op(I_MUL); cst(eps); op(I_DIV); fn1(OPF_FN_COS);
+                    op(I_ADD); op(I_MUL);
+                    op(I_SEL);
+                } else if (e.dim == OPF_FN_SMOOTH_HEAVISIDE) {
+                    // Math::smoothHeviside (Numeric.hpp:86-92): d < -eps ? 0 : d > eps ? 1 :
+                    //   0.5 * (1 + d / eps + 1 / PI * std::sin(d * PI / eps))
+                    const double eps = e.value;
+                    emit(e.a, s0, s1, s2); cst(-eps); op(I_LT);
+                    cst(0.0);
+                    emit(e.a, s0, s1, s2); cst(eps); op(I_GT);
+                    cst(1.0);
+                    cst(0.5);
+                    cst(1.0); emit(e.a, s0, s1, s2); cst(eps); op(I_DIV); op(I_ADD);
+                    cst(1 / kPi); emit(e.a, s0, s1, s2); cst(kPi); op(I_MUL); cst(eps); op(I_DIV); fn1(OPF_FN_SIN);
+                    op(I_MUL); op(I_ADD); op(I_MUL);
+                    op(I_SEL);
+                    op(I_SEL);
+                } else {
+                    emit(e.a, s0, s1, s2);
+                    fn1(e.dim);
+                }
+                break;
+            case OPF_N_FUNC2:
+                emit(e.a, s0, s1, s2);
+                emit(e.b, s0, s1, s2);
+                code.push_back({I_FN2, e.dim, 0, 0, 0.0});
+                push(-1);
+                break;
+            case OPF_N_CONV: {
+                // Convolution::eval (Convolution.hpp:52-68): rangeReduce_s over the window
+                // x-fastest of kernel[idx - i + n/2] * e(idx): the first term, then += each
+                const int k[3] = {e.dim & 255, (e.dim >> 8) & 255, (e.dim >> 16) & 255};
+                int q = 0;
+                for (int c = 0; c < k[2]; ++c)
+                    for (int b = 0; b < k[1]; ++b)
+                        for (int a = 0; a < k[0]; ++a, ++q) {
+                            ucst(e.b + q, nodes[e.b + q].value);
+                            emit(e.a, s0 + a - k[0] / 2, s1 + b - k[1] / 2, s2 + c - k[2] / 2);
+                            op(I_MUL);
+                            if (q > 0) op(I_ADD);
+                        }
+            } break;
             case OPF_N_COND:
                 emit(e.a, s0, s1, s2);
                 emit(e.b, s0, s1, s2);
@@ -455,6 +528,7 @@ int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields,
 // -------------------------------------------------------------- device side
 #define OPF_SHARED_SRC(...) __VA_ARGS__
 #include "weno_device.inc"
+#include "math_device.inc"
 #undef OPF_SHARED_SRC
 
 template <typename T>
@@ -506,6 +580,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                 sp -= 2;
                 tos = stk[sp][tid] != (T)0 ? stk[sp + 1][tid] : tos;
             } break;
+            case I_FN1: tos = opf_fn1<T>(in.a, tos); break;
+            case I_FN2: tos = opf_fn2<T>(in.a, stk[--sp][tid], tos); break;
             case I_WENO_UP:
             case I_WENO_DOWN: {
                 sp -= 6;
@@ -543,6 +619,22 @@ std::vector<Insn> structural_code(const LoweredProgram& P) {
     return c;
 }
 
+void uniform_spacing(const opf_mesh* m, int dtype, SweepArgs& a) {
+    a.umask = 0;
+    for (int d = 0; d < m->dim; ++d) {
+        a.hdx[d] = a.rdx[d] = 0.0;
+        int e;
+        const double h = m->h[d];
+        // a power of two whose reciprocal is a normal number of the sweep's precision
+        if (!m->uniform[d] || !(h > 0) || std::frexp(h, &e) != 0.5) continue;
+        const int lim = dtype == OPF_F64 ? 1016 : 120;  // room for multiples 1/4 .. 4 of h
+        if (e - 1 > lim || e - 1 < -lim) continue;
+        a.umask |= 1 << d;
+        a.hdx[d] = h;
+        a.rdx[d] = 1.0 / h;
+    }
+}
+
 int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a0, const LoweredProgram& P) {
     hipStream_t st = ctx->stream;
     SweepArgs a = a0;
@@ -550,10 +642,10 @@ int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a0, const LoweredProg
               P.consts.size());
     for (size_t q = 0; q < P.consts.size(); ++q) a.cst[q] = P.consts[q];
     const long long cells = (long long)a.n[0] * a.n[1] * a.n[2];
-    const bool jit = (ctx->jit_mode == 2 || (ctx->jit_mode == 1 && cells >= (1LL << 18))) &&
+    const bool jit = (ctx->jit_mode >= 2 || (ctx->jit_mode == 1 && cells >= (1LL << 18))) &&
                      P.code.size() <= (size_t)SWEEP_JIT_MAX_INSNS;
     if (jit) {
-        const int rc = jit_launch_sweep(st, P, dtype, a);
+        const int rc = jit_launch_sweep(st, P, dtype, a, ctx->jit_mode >= 3 ? ctx->jit_mode - 1 : 0);
         if (rc == OPF_OK) return OPF_OK;
         // the compiled engine is an optimisation: a failed compile or module load
         // runs the same program on the interpreter (same bits); argument errors stand
@@ -630,6 +722,7 @@ extern "C" int opf_sweep(const opf_expr_node* nodes, int nnodes, int root, opf_f
             a.f[f] = {fields[f]->data, fd.offset[0], fd.offset[1], fd.offset[2], fd.extent[0], fd.extent[1]};
         }
         OPF_TRY(mesh_arrays(mesh, d.dtype, a.x, a.dx));
+        uniform_spacing(mesh, d.dtype, a);
         for (int q = 0; q < 3; ++q) {
             a.morg[q] = mesh->start[q] - mesh->pad;
             a.s[q] = s[q];

@@ -37,7 +37,9 @@ enum InsnOp {
     I_LT = 16,
     I_GE = 17,
     I_LE = 18,
-    I_SEL = 19    // pops f, t, c: c != 0 ? t : f
+    I_SEL = 19,   // pops f, t, c: c != 0 ? t : f
+    I_FN1 = 20,   // f(a) for OPF_FN_* a (opf_fn1)
+    I_FN2 = 21    // f(a, b) for OPF_FN2_* a (opf_fn2)
 };
 
 struct Insn {
@@ -75,13 +77,17 @@ int lower_tree_set(const opf_expr_node* nodes, int nnodes, int root, opf_field* 
 // the sweep of program P over the box [a.s, a.s + a.n) on the context's stream, by the
 // engine the context's jit mode selects (interpreter or the compiled kernel of P)
 int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgram& P);
-// the compiled kernel of P (hipRTC, cached) and its launch
-int jit_sweep_function(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, hipFunction_t* fn);
-int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a);
+// the compiled kernel of P (hipRTC, cached) and its launch; form 0 = by grid size,
+// 1 = direct (loads from HBM), 2 = staged in LDS where the program allows
+int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a, int form = 0);
 // process-wide: hipRTC compilations, compiled kernels loaded, kernels loaded now
 void jit_stats(long long* compiles, long long* loaded, long long* resident);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],
                  const int n[3]);
 int mesh_arrays(const opf_mesh* m, int dtype, const void* x[3], const void* dx[3]);
+// dims of m whose spacing is uniform and a power of two with a normal reciprocal in
+// precision dtype: a.umask, a.hdx, a.rdx (compiled sweeps divide by h as a
+// multiplication by 1 / h, which gives the same bits)
+void uniform_spacing(const opf_mesh* m, int dtype, SweepArgs& a);
 
 }  // namespace opf

@@ -7,12 +7,81 @@ indices and combines them with the mesh spacings in the reference's operation
 order. numpy float64 elementwise ops round exactly like the reference's
 scalar C++ (no FMA contraction), so results are comparable bit for bit.
 """
+import math
+
 import numpy as np
 
 FIELD, CONST, ADD, SUB, MUL, DIV, NEG = 0, 1, 2, 3, 4, 5, 6
 D2C, D1C, D1UP, D1DOWN, WENO_UP, WENO_DOWN, INTP_C2N, INTP_N2C, UNKNOWN = 7, 8, 9, 10, 11, 12, 13, 14, 15
 FLUXLIM, MIN, MAX, ABS, GT, LT, GE, LE, COND = 16, 17, 18, 19, 20, 21, 22, 23, 24
+FUNC1, FUNC2, CONV = 25, 26, 27
 CORNER, CENTER = 0, 1
+PI = 3.141592653589793  # Core/Constants.hpp:37
+# functions of FUNC1 / FUNC2 (the <cmath> names of Arithmetic/AMDS.hpp:39-90)
+FN1_NAMES = ["exp", "exp2", "expm1", "log", "log10", "log2", "log1p", "sqrt", "cbrt", "sin", "cos", "tan", "asin",
+             "acos", "atan", "sinh", "cosh", "tanh", "asinh", "acosh", "atanh", "erf", "erfc", "tgamma", "lgamma",
+             "ceil", "floor", "trunc", "round", "nearbyint", "rint", "logb", "smoothDelta", "smoothHeviside"]
+FN2_NAMES = ["pow", "fmod", "remainder", "fdim", "hypot", "atan2", "copysign", "nextafter"]
+FN_SMOOTH_DELTA, FN_SMOOTH_HEAVISIDE = 32, 33
+# results the C library specifies exactly (the others are accurate to a few ulp, differently
+# rounded by each library)
+FN1_EXACT = {"sqrt", "ceil", "floor", "trunc", "round", "nearbyint", "rint", "logb"}
+FN2_EXACT = {"fmod", "remainder", "fdim", "copysign", "nextafter"}
+
+
+def _cround(x):  # std::round: halfway cases away from zero
+    f = math.floor(x)
+    if x >= 0:
+        return f + 1.0 if x - f >= 0.5 else f
+    c = math.ceil(x)
+    return c - 1.0 if c - x >= 0.5 else c
+
+
+def _logb(x):
+    if x == 0:
+        return -math.inf
+    if math.isinf(x):
+        return math.inf
+    if math.isnan(x):
+        return x
+    return float(math.frexp(x)[1] - 1)
+
+
+def _fdim(x, y):
+    if math.isnan(x) or math.isnan(y):
+        return math.nan
+    return x - y if x > y else 0.0
+
+
+_FN1 = {"exp": math.exp, "exp2": lambda x: 2.0 ** x, "expm1": math.expm1, "log": math.log, "log10": math.log10,
+        "log2": math.log2, "log1p": math.log1p, "sqrt": math.sqrt, "cbrt": lambda x: float(np.cbrt(x)),
+        "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin, "acos": math.acos, "atan": math.atan,
+        "sinh": math.sinh, "cosh": math.cosh, "tanh": math.tanh, "asinh": math.asinh, "acosh": math.acosh,
+        "atanh": math.atanh, "erf": math.erf, "erfc": math.erfc, "tgamma": math.gamma, "lgamma": math.lgamma,
+        "ceil": lambda x: float(math.ceil(x)), "floor": lambda x: float(math.floor(x)),
+        "trunc": lambda x: float(math.trunc(x)), "round": _cround, "nearbyint": lambda x: float(np.rint(x)),
+        "rint": lambda x: float(np.rint(x)), "logb": _logb}
+_FN2 = {"pow": math.pow, "fmod": math.fmod, "remainder": math.remainder, "fdim": _fdim, "hypot": math.hypot,
+        "atan2": math.atan2, "copysign": math.copysign, "nextafter": math.nextafter}
+
+
+def fn1(fid, x, eps=0.0):
+    """The <cmath> function fid of FUNC1 elementwise through Python's math module (the C
+    library's functions); smoothDelta / smoothHeviside restate Math/Function/Numeric.hpp:86-96."""
+    x = np.asarray(x, dtype=np.float64)
+    if fid == FN_SMOOTH_DELTA:  # std::max(d, -d) > eps ? 0. : 1. / 2. / eps * (1 + std::cos(d * PI / eps))
+        c = np.vectorize(math.cos, otypes=[np.float64])(x * PI / eps) if x.size else x
+        return np.where(_max(x, -x) > eps, 0.0, 1. / 2. / eps * (1 + c))
+    if fid == FN_SMOOTH_HEAVISIDE:  # d < -eps ? 0 : d > eps ? 1 : 0.5 * (1 + d / eps + 1 / PI * sin(d * PI / eps))
+        sn = np.vectorize(math.sin, otypes=[np.float64])(x * PI / eps) if x.size else x
+        return np.where(x < -eps, 0.0, np.where(x > eps, 1.0, 0.5 * (1 + x / eps + 1 / PI * sn)))
+    f = _FN1[FN1_NAMES[fid]]
+    return np.vectorize(f, otypes=[np.float64])(x)
+
+
+def fn2(fid, x, y):
+    x, y = np.broadcast_arrays(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
+    return np.vectorize(_FN2[FN2_NAMES[fid]], otypes=[np.float64])(x, y)
 # limiter kernels of FLUXLIM (Interpolator/FluxLimiterKernels.hpp)
 (LIM_CDS, LIM_QUICK, LIM_CUI, LIM_FROMM, LIM_LUI, LIM_MINMOD, LIM_SUPERBEE, LIM_MUSCL, LIM_HARMONIC, LIM_VANALBADA,
  LIM_UMIST, LIM_SPL13, LIM_SPLMAX12, LIM_SPLMAX13, LIM_KOREN, LIM_SMART, LIM_BSOU) = range(17)
@@ -104,8 +173,10 @@ def _locs(nodes, fields, dim):
             locs.append(list(locs[a]) if locs[a] is not None else (list(locs[b]) if locs[b] is not None else None))
         elif k == COND:  # Conditional.hpp prepare: the properties of the selected value
             locs.append(list(locs[b]) if locs[b] is not None else (list(locs[d]) if locs[d] is not None else None))
-        elif k == ABS:
+        elif k in (ABS, FUNC1, CONV):
             locs.append(list(locs[a]) if locs[a] is not None else None)
+        elif k == FUNC2:
+            locs.append(list(locs[a]) if locs[a] is not None else (list(locs[b]) if locs[b] is not None else None))
         elif k == FLUXLIM:  # D1FluxLimiter.hpp prepare: the location of the interpolated argument, flipped
             l_ = list(locs[b]) if locs[b] is not None else None
             if l_ is not None:
@@ -210,6 +281,23 @@ def evaluate(nodes, root, fields, mesh, lo, hi):
             return _min(x_, y_) if k == MIN else _max(x_, y_)
         if k == ABS:
             return np.fabs(ev(a, lo_, hi_))
+        if k == FUNC1:
+            return fn1(d, ev(a, lo_, hi_), v)
+        if k == FUNC2:
+            return fn2(d, ev(a, lo_, hi_), ev(b, lo_, hi_))
+        if k == CONV:  # Convolution::eval: rangeReduce_s x-fastest of kernel[idx - i + n/2] * e(idx)
+            ks = [d & 255, (d >> 8) & 255, (d >> 16) & 255]
+            res, q = None, 0
+            for c2 in range(ks[2]):
+                for c1 in range(ks[1]):
+                    for c0 in range(ks[0]):
+                        off = [c0 - ks[0] // 2, c1 - ks[1] // 2, c2 - ks[2] // 2]
+                        l2 = [lo_[t] + off[t] for t in range(dim)]
+                        h2 = [hi_[t] + off[t] for t in range(dim)]
+                        term = np.float64(nodes[b + q][4]) * ev(a, l2, h2)
+                        res = term if res is None else res + term
+                        q += 1
+            return res
         if k in (GT, LT, GE, LE):
             x_, y_ = ev(a, lo_, hi_), ev(b, lo_, hi_)
             c_ = {GT: np.greater, LT: np.less, GE: np.greater_equal, LE: np.less_equal}[k](x_, y_)

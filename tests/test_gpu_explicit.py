@@ -228,11 +228,12 @@ OPS = {
 }
 
 
-@pytest.fixture(params=["interp", "jit"])
+@pytest.fixture(params=["interp", "jit", "jit-tile"])
 def engine(ctx, request):
     """Run a sweep test with the stack-program interpreter and with the kernel compiled
-    from the expression at run time; both must equal the oracle bit for bit."""
-    ctx.set_jit(0 if request.param == "interp" else 2)
+    from the expression at run time, in its direct form and in its LDS-staged form;
+    all must equal the oracle bit for bit."""
+    ctx.set_jit({"interp": 0, "jit": 4, "jit-tile": 3}[request.param])
     yield request.param
     ctx.set_jit(1)
 
@@ -252,9 +253,11 @@ def test_sweep_weno_periodic(ctx, engine, kind):
     np.testing.assert_array_equal(got, ref)
 
 
-def test_sweep_lid_driven_terms(ctx, engine):
-    """Convective terms of examples/LidDriven/LidDriven2D.cpp:30-41 on a MAC grid."""
-    nodes = [25, 25]
+@pytest.mark.parametrize("nodes", [[25, 25], [33, 17]], ids=["h1/24", "pow2"])
+def test_sweep_lid_driven_terms(ctx, engine, nodes):
+    """Convective terms of examples/LidDriven/LidDriven2D.cpp:30-41 on a MAC grid; on a mesh
+    of power-of-two spacings (1/32, 1/16) the compiled sweep divides by multiples of h as
+    multiplications by their reciprocals, which must give the same bits."""
     u_spec = ([0, 1], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 1.0)], (1, 1))
     v_spec = ([1, 0], [(DIRC, DIRC), (DIRC, DIRC)], [(0, 0), (0, 0)], (1, 1))
     p_spec = ([1, 1], [(NEUM, NEUM), (NEUM, NEUM)], [(0, 0), (0, 0)], (1, 1))
@@ -306,6 +309,24 @@ def test_sweep_3d_zmarch_bitexact(ctx, engine, op):
     got, ref = _sweep_case(ctx, fn, [u_spec, w_spec], [21, 13, 41], None if op == 0 else w_spec, SEED + 12, op=op,
                            ranges=[(0.0, 1.0), (0.0, 1.0), (0.0, 2.5)])
     np.testing.assert_array_equal(got, ref)
+
+
+def test_sweep_pow2_spacing_bitexact(ctx, engine):
+    """Uniform power-of-two spacings in every dim (1/16, 1/4, 1/128): divisions by h and by
+    its multiples (D2, centred D1 at centres and corners, upwind, the linear interpolations,
+    WENO's h) on both engines equal the oracle bit for bit, FP64 in place and into a field."""
+    u_spec = (1, [(DIRC, NEUM), (NEUM, DIRC), (PER, PER)], [(0.2, 0.1), (0.3, -0.1), (0, 0)], (3, 3))
+    w_spec = ([0, 1, 1], [(DIRC, DIRC)] * 3, [(0.5, -0.5), (0.0, 1.0), (0.25, 0.75)], (1, 1))
+    D2, W, UP, D1 = of.D2SecondOrderCentered, of.D1WENO53Upwind, of.D1FirstOrderBiasedUpwind, of.D1FirstOrderCentered
+
+    def fn(u, w):
+        return (u + 0.01 * (of.d2x(D2, u) + of.d2y(D2, u) + of.d2z(D2, u)) - 0.001 * of.dz(W, u)
+                - 0.02 * of.dx(D1, w) * of.dy(UP, u)
+                + of.d1IntpCornerToCenter(0, w) * of.dy(D1, of.d1IntpCenterToCorner(1, u)))
+    for out in (None, u_spec):
+        got, ref = _sweep_case(ctx, fn, [u_spec, w_spec], [17, 9, 33], out, SEED + 14,
+                               ranges=[(0.0, 1.0), (0.0, 2.0), (0.0, 0.25)])
+        np.testing.assert_array_equal(got, ref)
 
 
 def test_sweep_zextent_two_cells_per_thread(ctx, engine):
@@ -551,3 +572,70 @@ def test_ftcs3d_nonperiodic_blocked_bitexact(ctx, n, steps, bc):
     u.ftcs_blocked(coef, steps, 1)
     np.testing.assert_array_equal(u.numpy(), ref)
     u.close()
+
+
+# <cmath> functions of expressions (Arithmetic/AMDS.hpp:39-90): the argument mapped into each
+# function's domain from the test fields' values in [0.5, 1.5]
+FN1_ARGS = {"acos": lambda u: u - 1.0, "asin": lambda u: u - 1.0, "atanh": lambda u: u - 1.0,
+            "acosh": lambda u: u + 1.0, "log1p": lambda u: u - 1.0, "expm1": lambda u: u - 1.0,
+            "round": lambda u: u * 4.0 - 4.0, "nearbyint": lambda u: u * 4.0 - 4.0, "rint": lambda u: u * 4.0 - 4.0,
+            "ceil": lambda u: u * 4.0 - 4.0, "floor": lambda u: u * 4.0 - 4.0, "trunc": lambda u: u * 4.0 - 4.0,
+            "logb": lambda u: u * 1e3 - 900.0, "tan": lambda u: u - 1.0}
+FN2_ARGS = {"fmod": lambda u, w: (u * 7.0, w), "remainder": lambda u, w: (u * 7.0, w),
+            "copysign": lambda u, w: (u, w - 1.0), "atan2": lambda u, w: (u - 1.0, w - 1.0)}
+SPEC_C2 = (1, [(DIRC, NEUM), (PER, PER)], [(0.2, 0.1), (0, 0)], (2, 2))
+
+
+def _ulp_close(got, ref, name, exact):
+    if exact:
+        np.testing.assert_array_equal(got, ref, err_msg=name)
+    else:  # the device's function and the C library's round differently in the last bits
+        np.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-15, err_msg=name)
+
+
+@pytest.mark.parametrize("name", oracle.expr.FN1_NAMES[:32])
+def test_sweep_math_fn1(ctx, engine, name):
+    """Each unary <cmath> function of an expression on all engines against the oracle (the C
+    library's function through Python's math module): bit for bit where the C standard fixes
+    the result (sqrt, rounding functions, logb), within a few ulp otherwise."""
+    f = getattr(of, name)
+    arg = FN1_ARGS.get(name, lambda u: u)
+    got, ref = _sweep_case(ctx, lambda u, w: f(arg(u)) * 0.5 + w, [SPEC_C2, SPEC_C2], [21, 14], SPEC_C2,
+                           SEED + 60 + oracle.expr.FN1_NAMES.index(name))
+    _ulp_close(got, ref, name, name in oracle.expr.FN1_EXACT)
+
+
+@pytest.mark.parametrize("name", oracle.expr.FN2_NAMES)
+def test_sweep_math_fn2(ctx, engine, name):
+    f = getattr(of, name)
+    args = FN2_ARGS.get(name, lambda u, w: (u, w))
+    got, ref = _sweep_case(ctx, lambda u, w: f(*args(u, w)) - u, [SPEC_C2, SPEC_C2], [17, 12], SPEC_C2,
+                           SEED + 100 + oracle.expr.FN2_NAMES.index(name))
+    _ulp_close(got, ref, name, name in oracle.expr.FN2_EXACT)
+
+
+def test_sweep_smooth_delta_heaviside(ctx, engine):
+    """Math::smoothDelta / smoothHeviside (Math/Function/Numeric.hpp:86-96) of an expression,
+    the functors of the level-set examples' UniOpAdaptor, eps = 0.3 against values in
+    [-0.5, 0.5] (both branches and the cut-offs)."""
+    got, ref = _sweep_case(ctx, lambda u, w: of.smoothDelta(0.3, u - 1.0) + of.smoothHeviside(0.3, w - 1.0) * 2.0,
+                           [SPEC_C2, SPEC_C2], [23, 15], SPEC_C2, SEED + 120)
+    _ulp_close(got, ref, "smooth", False)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sweep_conv_bitexact(ctx, engine, dim):
+    """conv(expr, kernel) (Convolution.hpp:52-68): the level-set example's 3 x 3 kernel
+    (16/24 centre, 1/24 around) in 2-D and an asymmetric 3 x 3 x 3 kernel in 3-D, over an
+    expression of two fields, equal to the oracle bit for bit (x-fastest accumulation)."""
+    if dim == 2:
+        c_, o_ = 16. / 24., 1. / 24.
+        ker = np.array([[o_, o_, o_], [o_, c_, o_], [o_, o_, o_]])
+        spec, nodes = SPEC_C2, [19, 13]
+    else:
+        ker = np.arange(27, dtype=np.float64).reshape(3, 3, 3) / 17.0 - 0.5
+        spec = (1, [(DIRC, NEUM), (PER, PER), (NEUM, DIRC)], [(0.2, 0.1), (0, 0), (-0.3, 0.7)], (1, 1))
+        nodes = [11, 9, 13]
+    got, ref = _sweep_case(ctx, lambda u, w: 0.25 * of.conv(u * w - 1.0, ker) + u, [spec, spec], nodes,
+                           spec, SEED + 130 + dim)
+    np.testing.assert_array_equal(got, ref)
