@@ -23,35 +23,23 @@
 #define OPF_SHARED_SRC(...) __VA_ARGS__
 #include "math_device.inc"
 #undef OPF_SHARED_SRC
+// out of line in the pad interpreter (see sweep.hip)
+__device__ __noinline__ double pad_fn1(int f, double x) { return opf_fn1<double>(f, x); }
+__device__ __noinline__ double pad_fn2(int f, double x, double y) { return opf_fn2<double>(f, x, y); }
 
 namespace opf {
 
-constexpr int PAD_K = 7;       // centre + 2 per dim
+// PadArgs: the row box s, n (target interior), the target's local range [tl, te) (every
+// unknown has this layout), per unknown the periodic dims and the ghost rule's affine
+// constants (PadBC), `live` (the unknown whose coefficients are kept; -1: every unknown,
+// in shared slots), the program, coef [PAD_K][N], bias [N], error flags, and the
+// launch-time constants of the program
+#define OPF_SHARED_SRC(...) __VA_ARGS__
+#include "pad_args.inc"
+#undef OPF_SHARED_SRC
+static_assert(sizeof(((PadArgs*)nullptr)->f) / sizeof(FieldArg) == SWEEP_MAX_FIELDS, "field slots");
 constexpr int PAD_W = PAD_K + 2;  // coefficients, bias, linear flag
-constexpr int PAD_BLOCK = 32;  // 24-deep stack of 9 doubles per lane: 54 KB of LDS
-constexpr int EQS_MAX = 16;    // unknowns of an equation set
-
-// boundary handling of one unknown: periodic dims and the ghost rule's affine constant
-struct PadBC {
-    int per[3];
-    double beta[3][2];
-};
-
-struct PadArgs {
-    FieldArg f[SWEEP_MAX_FIELDS];
-    const double* x[3];
-    const double* dx[3];
-    int morg[3];
-    int s[3], n[3];      // row box (target interior)
-    int tl[3], te[3];    // target local range [tl, te) (every unknown has this layout)
-    PadBC bc[EQS_MAX];   // per unknown
-    int live;            // unknown whose coefficients are kept; -1: every unknown, in shared slots
-    const void* code;
-    int ncode;
-    double* coef;        // [PAD_K][N]
-    double* bias;        // [N]
-    int* err;            // error flags
-};
+constexpr int PAD_BLOCK = 32;     // 24-deep stack of 9 doubles per lane: 54 KB of LDS
 
 __device__ __forceinline__ int slot_of(int ox, int oy, int oz) {
     if (ox == 0 && oy == 0 && oz == 0) return 0;
@@ -170,7 +158,7 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
             case I_ABS:
             case I_FN1:
                 if (t[PAD_K + 1] != 0.0) { bad |= 2; break; }
-                t[PAD_K] = in.op == I_ABS ? fabs(t[PAD_K]) : opf_fn1<double>(in.a, t[PAD_K]);
+                t[PAD_K] = in.op == I_ABS ? fabs(t[PAD_K]) : pad_fn1(in.a, t[PAD_K]);
                 break;
             default: {  // I_MIN, I_MAX, I_FN2 and the comparisons: binary
                 --sp;
@@ -184,7 +172,7 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
                     case I_LT: v = x < y ? 1.0 : 0.0; break;
                     case I_GE: v = x >= y ? 1.0 : 0.0; break;
                     case I_LE: v = x <= y ? 1.0 : 0.0; break;
-                    case I_FN2: v = opf_fn2<double>(in.a, x, y); break;
+                    case I_FN2: v = pad_fn2(in.a, x, y); break;
                     default: bad |= 2;
                 }
                 t[PAD_K] = v;
@@ -195,6 +183,22 @@ __global__ __launch_bounds__(PAD_BLOCK) void pad_kernel(PadArgs a) {
     if (bad) atomicOr(a.err, bad);
     for (int q = 0; q < PAD_K; ++q) a.coef[(long long)q * N + row] = t[q];
     a.bias[row] = t[PAD_K];
+}
+
+// the pads of program P over the rows of a: the compiled kernel (jit.hip) where the
+// context compiles sweeps (every size in modes 2-4, >= 2^16 rows by default) and the
+// program is a linear form it covers; else (and on a failed compile) the interpreter
+static int launch_pads(opf_ctx* ctx, hipStream_t st, const LoweredProgram& P, PadArgs& a, long long N) {
+    for (size_t q = 0; q < P.consts.size() && q < 32; ++q) a.cst[q] = P.consts[q];
+    if (ctx->jit_mode >= 2 || (ctx->jit_mode == 1 && N >= (1LL << 16))) {
+        const int rc = jit_launch_pad(st, P, a);
+        if (rc == OPF_OK) return OPF_OK;
+        if (rc != OPF_ERR_UNSUPPORTED && rc != OPF_ERR_HIP) return rc;
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((N + PAD_BLOCK - 1) / PAD_BLOCK)), dim3(PAD_BLOCK), 0, st, a);
+    OPF_HIP(hipGetLastError());
+    return OPF_OK;
 }
 
 // b = -s * bias, gathered over rows
@@ -341,9 +345,7 @@ static int eqn_eval_pads(opf_eqn* e) {
     a.bias = e->bias;
     a.err = e->err;
     OPF_HIP(hipMemsetAsync(e->err, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((e->N + PAD_BLOCK - 1) / PAD_BLOCK)), dim3(PAD_BLOCK), 0,
-                       ctx->stream, a);
-    OPF_HIP(hipGetLastError());
+    OPF_TRY(launch_pads(ctx, ctx->stream, e->P, a, e->N));
     int herr = 0;
     OPF_HIP(hipMemcpyAsync(&herr, e->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     OPF_HIP(hipStreamSynchronize(ctx->stream));
@@ -834,8 +836,7 @@ static int eqs_eval(EqsSystem& S) {
         for (int j : S.refs[i]) {
             a.live = j;
             a.coef = S.coef + (size_t)S.blk[(size_t)i * S.neq + j] * PAD_K * S.N;
-            hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((S.N + PAD_BLOCK - 1) / PAD_BLOCK)), dim3(PAD_BLOCK), 0, st,
-                               a);
+            OPF_TRY(launch_pads(S.ctx, st, S.P[i], a, S.N));
         }
     }
     OPF_HIP(hipGetLastError());

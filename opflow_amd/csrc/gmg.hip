@@ -774,6 +774,209 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
 }
 
 
+// ---- 2-D levels with constant coefficients, V(2,2): a level in two launches.
+// down: x1 = w b / D, x2 = x1 + w (b - A x1) / D, r = b - A x2, coarse b = the mean of r
+//       over each 2 x 2 block; x2 is stored.
+// up:   x3 = x2 + P x_c, x4 = x3 + w (b - A x3) / D, x5 = x4 + w (b - A x4) / D -> out,
+//       on the finest level with the partials of (b, x5), (x5, q), sum(x5).
+// A 256-thread workgroup owns a 32 x 32 tile; the intermediate vectors live in LDS on
+// the tile plus a halo of 2 (x1, x3), 1 (x2, x4) and are recomputed by neighbouring
+// tiles instead of going through memory, so a level costs two launches instead of
+// four or five and each fine vector crosses HBM once per phase. Halo positions of a
+// periodic dim hold the wrapped cells; those outside a non-periodic boundary are
+// never read (the boundary cell folds its ghost into D, as apply_op does). Blocks
+// stride over the tiles (a fixed grid: the partial sums are reproducible).
+constexpr int F2_T = 32;                // tile side
+constexpr int F2_W2 = F2_T + 4;         // region with halo 2
+constexpr int F2_W1 = F2_T + 2;         // region with halo 1
+
+struct F2Tile {
+    int tx, ty, tiles;
+};
+
+// global coordinate g of a position, wrapped into [0, n) on a periodic dim; false when
+// it lies outside a non-periodic dim
+__device__ __forceinline__ bool f2_map(int g, int n, int per, int& w) {
+    if (g >= 0 && g < n) {
+        w = g;
+        return true;
+    }
+    if (!per) return false;
+    w = ((g % n) + n) % n;
+    return true;
+}
+
+// D and (A v)_c at cell (i, j) of a constant-coefficient 2-D level, the neighbours of v
+// read at LDS offsets +-1 / +-W from c (apply_op's order and boundary folding)
+template <typename T>
+__device__ __forceinline__ T f2_apply(const GOp<T>& o, int i, int j, const T* v, int c, int W, T& D) {
+    T acc = 0;
+    D = o.sigma;
+    const int cc[2] = {i, j}, nn[2] = {o.n0, o.n1}, st[2] = {1, W};
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        if (cc[d] > 0 || o.per[d]) {
+            D += o.a[d];
+            acc += o.a[d] * v[c - st[d]];
+        } else {
+            D += o.a[d] * o.bfac[d][0];
+        }
+        if (cc[d] < nn[d] - 1 || o.per[d]) {
+            D += o.a[d];
+            acc += o.a[d] * v[c + st[d]];
+        } else {
+            D += o.a[d] * o.bfac[d][1];
+        }
+    }
+    return D * v[c] - acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k2_down(GOp<T> o, const T* __restrict__ b, T* __restrict__ x2o,
+                                               T* __restrict__ bc, int c0, int c1, T w, F2Tile ft, Bat bt) {
+    __shared__ T sb[F2_W2 * F2_W2], s1[F2_W2 * F2_W2], s2[F2_W1 * F2_W1], sr[F2_T * F2_T];
+    BAT_SLOT(bt);
+    b += vo;
+    x2o += vo;
+    bc += (long long)slot * ((long long)c0 * c1);
+    for (int tile = (int)xcd_block(blockIdx.x, gridDim.x); tile < ft.tiles; tile += gridDim.x) {
+        const int x0 = (tile % ft.tx) * F2_T, y0 = (tile / ft.tx) * F2_T;
+        // b and x1 = w b / D on the tile + 2
+        for (int p = threadIdx.x; p < F2_W2 * F2_W2; p += blockDim.x) {
+            int i, j;
+            const bool ok = f2_map(x0 - 2 + p % F2_W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / F2_W2, o.n1, o.per[1], j);
+            T bv = 0, x1 = 0;
+            if (ok) {
+                bv = b[(uint32_t)i + o.s1 * (uint32_t)j];
+                x1 = w * bv / diag_const<T, 2>(o, i, j, 0);
+            }
+            sb[p] = bv;
+            s1[p] = x1;
+        }
+        __syncthreads();
+        // x2 on the tile + 1
+        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
+            const int li = p % F2_W1, lj = p / F2_W1, c = (lj + 1) * F2_W2 + li + 1;
+            int i, j;
+            T x2 = 0;
+            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
+                T D;
+                const T ax = f2_apply<T>(o, i, j, s1, c, F2_W2, D);
+                x2 = s1[c] + w * (sb[c] - ax) / D;
+            }
+            s2[p] = x2;
+        }
+        __syncthreads();
+        // r = b - A x2 on the tile; x2 stored
+        for (int p = threadIdx.x; p < F2_T * F2_T; p += blockDim.x) {
+            const int li = p % F2_T, lj = p / F2_T, i = x0 + li, j = y0 + lj;
+            const int c = (lj + 1) * F2_W1 + li + 1;
+            T r = 0;
+            if (i < o.n0 && j < o.n1) {
+                T D;
+                r = sb[(lj + 2) * F2_W2 + li + 2] - f2_apply<T>(o, i, j, s2, c, F2_W1, D);
+                x2o[(uint32_t)i + o.s1 * (uint32_t)j] = s2[c];
+            }
+            sr[p] = r;
+        }
+        __syncthreads();
+        // coarse b: the children summed as k_restrict sums them
+        {
+            const int q = threadIdx.x, I = x0 / 2 + q % (F2_T / 2), J = y0 / 2 + q / (F2_T / 2);
+            if (I < c0 && J < c1) {
+                const int li = 2 * (q % (F2_T / 2)), lj = 2 * (q / (F2_T / 2));
+                T s = 0;
+                s += sr[lj * F2_T + li];
+                s += sr[lj * F2_T + li + 1];
+                s += sr[(lj + 1) * F2_T + li];
+                s += sr[(lj + 1) * F2_T + li + 1];
+                bc[(uint32_t)I + (uint32_t)c0 * (uint32_t)J] = s * (T)0.25;
+            }
+        }
+        __syncthreads();  // the LDS tiles are reused by the next tile
+    }
+}
+
+template <typename T, bool DOTS>
+__global__ __launch_bounds__(256) void k2_up(GOp<T> o, const T* __restrict__ x2, const T* __restrict__ xc, int c0,
+                                             int c1, const T* __restrict__ b, T* __restrict__ out, T w,
+                                             const T* __restrict__ q, double* __restrict__ part, F2Tile ft, Bat bt) {
+    __shared__ T s3[F2_W2 * F2_W2], sb[F2_W1 * F2_W1], s4[F2_W1 * F2_W1];
+    BAT_SLOT(bt);
+    x2 += vo;
+    b += vo;
+    out += vo;
+    xc += (long long)slot * ((long long)c0 * c1);
+    if (DOTS) {
+        q += vo;
+        part += (long long)slot * bt.ps;
+    }
+    const int cn[3] = {c0, c1, 1};
+    double d0 = 0, d1 = 0, d2 = 0;
+    for (int tile = (int)xcd_block(blockIdx.x, gridDim.x); tile < ft.tiles; tile += gridDim.x) {
+        const int x0 = (tile % ft.tx) * F2_T, y0 = (tile / ft.tx) * F2_T;
+        // x3 = x2 + P x_c on the tile + 2
+        for (int p = threadIdx.x; p < F2_W2 * F2_W2; p += blockDim.x) {
+            int i, j;
+            T x3 = 0;
+            if (f2_map(x0 - 2 + p % F2_W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / F2_W2, o.n1, o.per[1], j))
+                x3 = x2[(uint32_t)i + o.s1 * (uint32_t)j] + prolong_at<T, 2>(o, xc, cn, i, j, 0);
+            s3[p] = x3;
+        }
+        // b on the tile + 1
+        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
+            int i, j;
+            T bv = 0;
+            if (f2_map(x0 - 1 + p % F2_W1, o.n0, o.per[0], i) && f2_map(y0 - 1 + p / F2_W1, o.n1, o.per[1], j))
+                bv = b[(uint32_t)i + o.s1 * (uint32_t)j];
+            sb[p] = bv;
+        }
+        __syncthreads();
+        // x4 on the tile + 1
+        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
+            const int li = p % F2_W1, lj = p / F2_W1, c = (lj + 1) * F2_W2 + li + 1;
+            int i, j;
+            T x4 = 0;
+            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
+                T D;
+                const T ax = f2_apply<T>(o, i, j, s3, c, F2_W2, D);
+                x4 = s3[c] + w * (sb[p] - ax) / D;
+            }
+            s4[p] = x4;
+        }
+        __syncthreads();
+        // x5 on the tile -> out (+ the dot products)
+        for (int p = threadIdx.x; p < F2_T * F2_T; p += blockDim.x) {
+            const int li = p % F2_T, lj = p / F2_T, i = x0 + li, j = y0 + lj;
+            if (i >= o.n0 || j >= o.n1) continue;
+            const int c = (lj + 1) * F2_W1 + li + 1;
+            T D;
+            const T ax = f2_apply<T>(o, i, j, s4, c, F2_W1, D);
+            const T bv = sb[c];
+            const T x5 = s4[c] + w * (bv - ax) / D;
+            const uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j;
+            out[t] = x5;
+            if (DOTS) {
+                const double zv = (double)x5;
+                d0 += (double)bv * zv;
+                d1 += zv * (double)q[t];
+                d2 += zv;
+            }
+        }
+        __syncthreads();
+    }
+    if (DOTS) {
+        d0 = gmg_block_sum(d0);
+        d1 = gmg_block_sum(d1);
+        d2 = gmg_block_sum(d2);
+        if (threadIdx.x == 0) {
+            part[blockIdx.x] = d0;
+            part[GMG_MAXB + blockIdx.x] = d1;
+            part[2 * GMG_MAXB + blockIdx.x] = d2;
+        }
+    }
+}
+
 // ---- coarse tail: every level of at most GMG_TAIL_N cells in one workgroup per slot.
 // A V-cycle's coarse levels are tiny and launch-bound (a handful of cells per
 // CU); here one 1024-thread workgroup runs their smoothing sweeps, residual
@@ -1205,9 +1408,9 @@ static bool use_z(const Hierarchy& H, int l) {
     const GLevelHost& L = H.host[l];
     return !H.var && L.n[0] >= 64 && L.N >= (H.dim == 3 ? (1 << 18) : (1 << 16));
 }
-// blocks of a z-marching launch: every tile item, at most GMG_RBLK blocks striding over
-// them (fewer partial sums to finish, cheaper dispatch of the masked launches)
-static inline int zgrid(const ZTile& z) { return std::min(z.items, GMG_RBLK); }
+// blocks of a z-marching launch: one per tile item (each block's tiles stay in its XCD's
+// L2; striding fewer blocks over the items measured slower), at most GMG_MAXB
+static inline int zgrid(const ZTile& z) { return std::min(z.items, GMG_MAXB); }
 static ZTile ztile(const GLevelHost& L) {
     ZTile z;
     z.zc = L.n[2] == 1 ? 1 : L.n[2] >= 256 ? 8 : 4;
@@ -1733,6 +1936,24 @@ static void ph_vtail(const Hierarchy& H, hipStream_t st, const int* mask, int l0
                                                   st, ta, bt));
 }
 
+// 2-D constant-coefficient V(2,2) levels run as k2_down / k2_up: one block per 32 x 32
+// tile (at most GMG_MAXB, then striding); on the finest level k2_up writes the partials
+// of the three dot products, one per block
+static bool fused2d(const Hierarchy& H) { return H.dim == 2 && !H.var && H.pre == 2 && H.post == 2; }
+static F2Tile f2tile(const GLevelHost& L) {
+    F2Tile ft;
+    ft.tx = (L.n[0] + F2_T - 1) / F2_T;
+    ft.ty = (L.n[1] + F2_T - 1) / F2_T;
+    ft.tiles = ft.tx * ft.ty;
+    return ft;
+}
+// partial sums per value that the finest level's V-cycle leaves for the DOT3 finish
+template <typename T>
+static int dot3_parts(const Hierarchy& H) {
+    if (fused2d(H) && H.dev.size() > 1) return std::min(f2tile(H.host[0]).tiles, GMG_MAXB);
+    return red_count<T>(H);
+}
+
 template <typename T>
 void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const T* rhs, T* out,
                const T* dot_q = nullptr) {
@@ -1753,6 +1974,23 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
     }
     GOp<T> o = make_op<T>(H, l);
     const T w = (T)H.omega;
+    if (fused2d(H)) {
+        // the level in two launches (k2_down / k2_up): x2 kept in the level temp
+        const GLevelDev& C = H.dev[l + 1];
+        const F2Tile ft = f2tile(L);
+        const unsigned G = (unsigned)std::min(ft.tiles, GMG_MAXB);
+        const Bat bt = bat(H, mask, N);
+        hipLaunchKernelGGL(k2_down<T>, dim3(G, B), dim3(256), 0, st, o, rhs, (T*)D.t, (T*)C.b, C.n[0], C.n[1], w, ft,
+                           bt);
+        ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
+        if (dot_q)  // one partial per block (dot3_parts)
+            hipLaunchKernelGGL((k2_up<T, true>), dim3(G, B), dim3(256), 0, st, o, (const T*)D.t, (const T*)C.x,
+                               C.n[0], C.n[1], rhs, out, w, dot_q, H.part, ft, bt);
+        else
+            hipLaunchKernelGGL((k2_up<T, false>), dim3(G, B), dim3(256), 0, st, o, (const T*)D.t, (const T*)C.x,
+                               C.n[0], C.n[1], rhs, out, w, (const T*)nullptr, (double*)nullptr, ft, bt);
+        return;
+    }
     // Jacobi sweeps ping-pong between `out` and the level temp; start on the
     // buffer that makes the last sweep land in `out`
     T* cur = out;
@@ -1859,8 +2097,9 @@ template <typename T>
 static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int mode, int singular,
                       int* m_it = nullptr, int* m_rep = nullptr, int maxit = 0) {
     const long long N = H.host[0].N;
-    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, red_count<T>(H), mode, H.scal,
-                       1.0 / N, singular, bat(H, mask, N), m_it, m_rep, maxit);
+    const int nb = mode == FIN_DOT3 || mode == FIN_DOT3_FIRST ? dot3_parts<T>(H) : red_count<T>(H);
+    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, nb, mode, H.scal, 1.0 / N,
+                       singular, bat(H, mask, N), m_it, m_rep, maxit);
 }
 
 // start of an inner FCG solve on r: p_old = 0, z = M r, rz = (r, z)

@@ -33,6 +33,12 @@
 #include "sweep.hpp"
 
 namespace opf {
+#define OPF_SHARED_SRC(...) __VA_ARGS__
+#include "pad_args.inc"
+#undef OPF_SHARED_SRC
+}  // namespace opf
+
+namespace opf {
 
 namespace {
 
@@ -45,6 +51,9 @@ const char* const kWenoSrc =
     ;
 const char* const kMathSrc =
 #include "math_device.inc"
+    ;
+const char* const kPadArgsSrc =
+#include "pad_args.inc"
     ;
 #undef OPF_SHARED_SRC
 
@@ -744,12 +753,13 @@ void jit_stats(long long* compiles, long long* loaded, long long* resident) {
 }
 
 namespace {
-int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out = nullptr);
+int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out = nullptr,
+                        const char* kname = "opf_sweep_jit");
 }  // namespace
 
 namespace {
 // the loaded kernel of a generated source: in-process cache, disk cache, then hipRTC
-int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out) {
+int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key_out, const char* kname) {
     int dev = 0;
     OPF_HIP(hipGetDevice(&dev));
     uint64_t h = fnv1a64(src.data(), src.size());
@@ -778,9 +788,9 @@ int function_for_source(const std::string& src, hipFunction_t* fn, uint64_t* key
     hipModule_t mod;
     OPF_HIP(hipModuleLoadData(&mod, code.data()));
     hipFunction_t f;
-    if (hipModuleGetFunction(&f, mod, "opf_sweep_jit") != hipSuccess) {
+    if (hipModuleGetFunction(&f, mod, kname) != hipSuccess) {
         hipModuleUnload(mod);
-        set_error("jit: no opf_sweep_jit in the code object");
+        set_error("jit: no %s in the code object", kname);
         return OPF_ERR_HIP;
     }
     g_fns.emplace(key, Loaded{f, mod, dev, ++g_tick});
@@ -897,6 +907,266 @@ int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const S
     const Plan pl{mk, false, R};
     remember(pl);
     return launch(pl);
+}
+
+
+// ---------------------------------------------------------------------------
+// StencilPads of an equation, compiled: the pad-mode program (the interpreter in
+// eqn.hip carries every value as PAD_K coefficients + bias + a linear flag on an LDS
+// stack) becomes straight-line code in which each value is its bias and only the
+// coefficients of the slots it can reach, all in registers. Which slots a value
+// holds and whether it is linear in the unknowns are known from the program, so the
+// generated code does per row only the arithmetic the interpreter does on those
+// entries, in the same order (same bits). Forms that are not linear in the unknowns
+// (a product of two linear values, a division by one, WENO or a nonlinear function
+// of one, a read off the 5/7-point stencil) are not compiled: the interpreter runs
+// them and reports the error.
+namespace {
+struct PadVal {
+    bool lin = false;
+    unsigned mask = 0;  // slots with a coefficient
+};
+
+int gen_pad_source(const LoweredProgram& P, std::string& s) {
+    s.clear();
+    s += "typedef double T;\n";
+    s += kArgsSrc;
+    s += "\n";
+    s += kPadArgsSrc;
+    s += "\n";
+    s += kMathSrc;
+    s += "\nextern \"C\" __global__ __launch_bounds__(256) void opf_pad_jit(PadArgs a) {\n"
+         "  const long long N = (long long)a.n[0] * a.n[1] * a.n[2];\n"
+         "  const long long row = (long long)blockIdx.x * 256 + threadIdx.x;\n"
+         "  if (row >= N) return;\n"
+         "  const int ci = a.s[0] + (int)(row % a.n[0]);\n"
+         "  const int cj = a.s[1] + (int)((row / a.n[0]) % a.n[1]);\n"
+         "  const int ck = a.s[2] + (int)(row / ((long long)a.n[0] * a.n[1]));\n";
+    bool fu[SWEEP_MAX_FIELDS] = {};
+    for (const Insn& in : P.code)
+        if (in.op == I_LD) {
+            if (in.a < 0 || in.a >= SWEEP_MAX_FIELDS) return OPF_ERR_ARG;
+            fu[in.a] = true;
+        }
+    char line[512];
+    for (int f = 0; f < SWEEP_MAX_FIELDS; ++f) {
+        if (!fu[f]) continue;
+        snprintf(line, sizeof line,
+                 "  const double* __restrict__ F%d = (const double*)a.f[%d].ptr + ((ci - a.f[%d].off0) + "
+                 "(long long)a.f[%d].e0 * ((cj - a.f[%d].off1) + (long long)a.f[%d].e1 * (ck - a.f[%d].off2)));\n",
+                 f, f, f, f, f, f, f);
+        s += line;
+    }
+    std::vector<PadVal> val(P.code.size());
+    std::vector<int> st;
+    auto pop = [&](int& v) -> bool {
+        if (st.empty()) return false;
+        v = st.back();
+        st.pop_back();
+        return true;
+    };
+    auto k = [](int id, int q) { return "v" + std::to_string(id) + "k" + std::to_string(q); };
+    auto c = [](int id) { return "v" + std::to_string(id) + "c"; };
+    auto emit = [&](const std::string& t) { s += "  " + t + ";\n"; };
+    for (size_t pc = 0; pc < P.code.size(); ++pc) {
+        const Insn& in = P.code[pc];
+        const int id = (int)pc;
+        PadVal& V = val[pc];
+        switch (in.op) {
+            case I_LD:
+                snprintf(line, sizeof line, "double %s = F%d[%d]", c(id).c_str(), in.a, in.b);
+                emit(line);
+                break;
+            case I_CONST: {
+                if (in.a) {
+                    snprintf(line, sizeof line, "double %s = a.cst[%d]", c(id).c_str(), in.b);
+                } else {
+                    long long bits;
+                    std::memcpy(&bits, &in.imm, 8);
+                    snprintf(line, sizeof line, "double %s = __longlong_as_double(%lldLL)", c(id).c_str(), bits);
+                }
+                emit(line);
+            } break;
+            case I_DX:
+            case I_X: {
+                const char* tb = in.op == I_DX ? "dx" : "x";
+                const char* cc = in.a == 0 ? "ci" : in.a == 1 ? "cj" : "ck";
+                if (in.a < 0 || in.a > 2) return OPF_ERR_ARG;
+                snprintf(line, sizeof line, "double %s = a.%s[%d][%s + %d - a.morg[%d]]", c(id).c_str(), tb, in.a, cc,
+                         in.b, in.a);
+                emit(line);
+            } break;
+            case I_LDU: {
+                const int u = in.a, ox = (in.c & 31) - 8, oy = ((in.c >> 5) & 31) - 8, oz = ((in.c >> 10) & 31) - 8;
+                int sl = -1;
+                if (ox == 0 && oy == 0 && oz == 0) sl = 0;
+                else if (oy == 0 && oz == 0) sl = ox == -1 ? 1 : ox == 1 ? 2 : -1;
+                else if (ox == 0 && oz == 0) sl = oy == -1 ? 3 : oy == 1 ? 4 : -1;
+                else if (ox == 0 && oy == 0) sl = oz == -1 ? 5 : oz == 1 ? 6 : -1;
+                if (sl < 0 || u < 0 || u >= EQS_MAX) return OPF_ERR_UNSUPPORTED;
+                V.lin = true;
+                V.mask = 1u << sl;
+                snprintf(line, sizeof line, "double %s = (a.live < 0 || a.live == %d) ? 1.0 : 0.0", k(id, sl).c_str(),
+                         u);
+                emit(line);
+                snprintf(line, sizeof line, "double %s = 0.0", c(id).c_str());
+                emit(line);
+                const int off[3] = {ox, oy, oz};
+                const char* cc[3] = {"ci", "cj", "ck"};
+                for (int d = 0; d < 3; ++d) {  // the boundary constant of the last dim outside, as the interpreter
+                    snprintf(line, sizeof line,
+                             "{ const int p = %s + %d; if (!(p >= a.tl[%d] && p < a.te[%d]) && !a.bc[%d].per[%d]) "
+                             "%s = a.bc[%d].beta[%d][p < a.tl[%d] ? 0 : 1]; }",
+                             cc[d], off[d], d, d, u, d, c(id).c_str(), u, d, d);
+                    emit(line);
+                }
+            } break;
+            case I_ADD:
+            case I_SUB: {
+                int x, y;
+                if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                const char o = in.op == I_ADD ? '+' : '-';
+                V.lin = val[x].lin || val[y].lin;
+                V.mask = val[x].mask | val[y].mask;
+                for (int q = 0; q < PAD_K; ++q) {
+                    if (!(V.mask >> q & 1)) continue;
+                    const std::string l = (val[x].mask >> q & 1) ? k(x, q) : "0.0";
+                    const std::string r = (val[y].mask >> q & 1) ? k(y, q) : "0.0";
+                    emit("double " + k(id, q) + " = " + l + " " + o + " " + r);
+                }
+                emit("double " + c(id) + " = " + c(x) + " " + o + " " + c(y));
+            } break;
+            case I_MUL: {
+                int x, y;
+                if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                if (val[x].lin && val[y].lin) return OPF_ERR_UNSUPPORTED;
+                if (val[y].lin) {  // const * pad: every entry of y times x's constant
+                    V = val[y];
+                    for (int q = 0; q < PAD_K; ++q)
+                        if (V.mask >> q & 1) emit("double " + k(id, q) + " = " + c(x) + " * " + k(y, q));
+                    emit("double " + c(id) + " = " + c(x) + " * " + c(y));
+                } else {  // pad * const (or const * const)
+                    V = val[x];
+                    for (int q = 0; q < PAD_K; ++q)
+                        if (V.mask >> q & 1) emit("double " + k(id, q) + " = " + k(x, q) + " * " + c(y));
+                    emit("double " + c(id) + " = " + c(x) + " * " + c(y));
+                }
+            } break;
+            case I_DIV: {
+                int x, y;
+                if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                if (val[y].lin) return OPF_ERR_UNSUPPORTED;
+                V = val[x];
+                for (int q = 0; q < PAD_K; ++q)
+                    if (V.mask >> q & 1) emit("double " + k(id, q) + " = " + k(x, q) + " / " + c(y));
+                emit("double " + c(id) + " = " + c(x) + " / " + c(y));
+            } break;
+            case I_NEG: {
+                int x;
+                if (!pop(x)) return OPF_ERR_ARG;
+                V = val[x];
+                for (int q = 0; q < PAD_K; ++q)
+                    if (V.mask >> q & 1) emit("double " + k(id, q) + " = -" + k(x, q));
+                emit("double " + c(id) + " = -" + c(x));
+            } break;
+            case I_ABS:
+            case I_FN1: {
+                int x;
+                if (!pop(x)) return OPF_ERR_ARG;
+                if (val[x].lin) return OPF_ERR_UNSUPPORTED;
+                if (in.op == I_ABS) emit("double " + c(id) + " = fabs(" + c(x) + ")");
+                else emit("double " + c(id) + " = opf_fn1<double>(" + std::to_string(in.a) + ", " + c(x) + ")");
+            } break;
+            case I_MIN:
+            case I_MAX:
+            case I_GT:
+            case I_LT:
+            case I_GE:
+            case I_LE:
+            case I_FN2: {
+                int x, y;
+                if (!pop(y) || !pop(x)) return OPF_ERR_ARG;
+                if (val[x].lin || val[y].lin) return OPF_ERR_UNSUPPORTED;
+                const std::string X = c(x), Y = c(y);
+                std::string e;
+                switch (in.op) {
+                    case I_MIN: e = Y + " < " + X + " ? " + Y + " : " + X; break;
+                    case I_MAX: e = X + " < " + Y + " ? " + Y + " : " + X; break;
+                    case I_GT: e = X + " > " + Y + " ? 1.0 : 0.0"; break;
+                    case I_LT: e = X + " < " + Y + " ? 1.0 : 0.0"; break;
+                    case I_GE: e = X + " >= " + Y + " ? 1.0 : 0.0"; break;
+                    case I_LE: e = X + " <= " + Y + " ? 1.0 : 0.0"; break;
+                    default: e = "opf_fn2<double>(" + std::to_string(in.a) + ", " + X + ", " + Y + ")";
+                }
+                emit("double " + c(id) + " = " + e);
+            } break;
+            case I_SEL: {
+                int cnd, t_, f_;
+                if (!pop(f_) || !pop(t_) || !pop(cnd)) return OPF_ERR_ARG;
+                if (val[cnd].lin || val[t_].lin || val[f_].lin) return OPF_ERR_UNSUPPORTED;
+                emit("double " + c(id) + " = " + c(cnd) + " != 0.0 ? " + c(t_) + " : " + c(f_));
+            } break;
+            default:  // WENO on pads and anything else: the interpreter
+                return OPF_ERR_UNSUPPORTED;
+        }
+        st.push_back(id);
+    }
+    if (st.size() != 1) return OPF_ERR_ARG;
+    const int r = st.back();
+    for (int q = 0; q < PAD_K; ++q) {
+        snprintf(line, sizeof line, "  a.coef[(long long)%d * N + row] = %s;\n", q,
+                 (val[r].mask >> q & 1) ? k(r, q).c_str() : "0.0");
+        s += line;
+    }
+    s += "  a.bias[row] = " + c(r) + ";\n}\n";
+    return OPF_OK;
+}
+
+std::unordered_map<uint64_t, uint64_t> g_pad_plans;  // program structure -> module key (0: not compiled)
+}  // namespace
+
+int jit_launch_pad(hipStream_t st, const LoweredProgram& P, const PadArgs& a) {
+    int dev = 0;
+    OPF_HIP(hipGetDevice(&dev));
+    uint64_t pk = 1469598103934665603ULL;
+    for (const Insn& in : P.code) {
+        const int w[4] = {in.op, in.a, in.b, in.c};
+        pk = fnv1a64(w, sizeof w, pk);
+        if (!(in.op == I_CONST && in.a)) pk = fnv1a64(&in.imm, sizeof in.imm, pk);
+    }
+    pk = fnv1a64(&dev, sizeof dev, pk);
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_pad_plans.find(pk);
+        if (it != g_pad_plans.end()) {
+            if (it->second == 0) return OPF_ERR_UNSUPPORTED;  // known not to compile
+            auto m = g_fns.find(it->second);
+            if (m != g_fns.end()) {
+                m->second.used = ++g_tick;
+                fn = m->second.fn;
+            }
+        }
+        if (g_pad_plans.size() > 4 * kMaxModules) g_pad_plans.clear();
+    }
+    if (!fn) {
+        std::string src;
+        const int rc = gen_pad_source(P, src);
+        if (rc != OPF_OK) {
+            std::lock_guard<std::mutex> lock(g_mu);
+            g_pad_plans[pk] = 0;
+            return OPF_ERR_UNSUPPORTED;
+        }
+        uint64_t mk = 0;
+        OPF_TRY(function_for_source(src, &fn, &mk, "opf_pad_jit"));
+        std::lock_guard<std::mutex> lock(g_mu);
+        g_pad_plans[pk] = mk;
+    }
+    const long long N = (long long)a.n[0] * a.n[1] * a.n[2];
+    PadArgs args = a;
+    void* params[] = {&args};
+    OPF_HIP(hipModuleLaunchKernel(fn, (unsigned)((N + 255) / 256), 1, 1, 256, 1, 1, 0, st, params, nullptr));
+    return OPF_OK;
 }
 
 }  // namespace opf

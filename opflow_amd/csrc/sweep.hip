@@ -531,6 +531,13 @@ int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields,
 #include "math_device.inc"
 #undef OPF_SHARED_SRC
 
+// the interpreter calls the math functions out of line: inlined, every one of them
+// would sit in the interpreter loop and raise its register count for every program
+template <typename T>
+__device__ __noinline__ T interp_fn1(int f, T x) { return opf_fn1<T>(f, x); }
+template <typename T>
+__device__ __noinline__ T interp_fn2(int f, T x, T y) { return opf_fn2<T>(f, x, y); }
+
 template <typename T>
 __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
     __shared__ T stk[SWEEP_MAX_DEPTH][256];
@@ -580,8 +587,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                 sp -= 2;
                 tos = stk[sp][tid] != (T)0 ? stk[sp + 1][tid] : tos;
             } break;
-            case I_FN1: tos = opf_fn1<T>(in.a, tos); break;
-            case I_FN2: tos = opf_fn2<T>(in.a, stk[--sp][tid], tos); break;
+            case I_FN1: tos = interp_fn1<T>(in.a, tos); break;
+            case I_FN2: tos = interp_fn2<T>(in.a, stk[--sp][tid], tos); break;
             case I_WENO_UP:
             case I_WENO_DOWN: {
                 sp -= 6;

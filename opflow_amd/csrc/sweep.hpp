@@ -80,6 +80,10 @@ int launch_sweep(opf_ctx* ctx, int dtype, const SweepArgs& a, const LoweredProgr
 // the compiled kernel of P (hipRTC, cached) and its launch; form 0 = by grid size,
 // 1 = direct (loads from HBM), 2 = staged in LDS where the program allows
 int jit_launch_sweep(hipStream_t st, const LoweredProgram& P, int dtype, const SweepArgs& a, int form = 0);
+// the StencilPads of a pad-mode program, compiled (eqn.hip's PadArgs); OPF_ERR_UNSUPPORTED
+// when the program is not a linear form the compiled kernel covers (the interpreter runs it)
+struct PadArgs;
+int jit_launch_pad(hipStream_t st, const LoweredProgram& P, const PadArgs& a);
 // process-wide: hipRTC compilations, compiled kernels loaded, kernels loaded now
 void jit_stats(long long* compiles, long long* loaded, long long* resident);
 int check_bounds(const LoweredProgram& P, opf_field* const* fields, int nfields, const opf_mesh* mesh, const int s[3],

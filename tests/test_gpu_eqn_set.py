@@ -321,3 +321,17 @@ def test_solve_eqns_argument_errors(ctx):
     with pytest.raises(of._lib.OpfError, match="5/7-point"):  # a diagonal neighbour of b
         of.SolveEqns(lambda a, b: of.Equation(1.0, _lap(a) + of.dx(UP, of.dy(UP, b))),
                      lambda a, b: of.Equation(1.0, _lap(b)), u, v, None, None)
+
+
+def test_pads_compiled_equal_interpreter_set(ctx):
+    """The compiled StencilPad kernel (jit.hip) and the pad interpreter build the same coupled
+    system bit for bit: every coefficient block and the right-hand side of the set's CSR."""
+    mesh, om, u, v, eqs = _coupled_case(ctx, 19, 14)
+    out = []
+    for mode in (0, 2):
+        ctx.set_jit(mode)
+        m = of.CSRMatrixGenerator.generate(of.makeStencilHolder(of.makeEqnHolder(eqs, [u, v])))
+        out.append((np.array(m.row), np.array(m.col), np.array(m.val), np.array(m.rhs)))
+    ctx.set_jit(1)
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
