@@ -7,6 +7,7 @@
 // coefficients, the per-cell shift, the symmetry and uniformity tests and every
 // coarse level are kernels; the host reads back one small record per level build
 // and the coarsest level (at most 1024 cells) for its dense inverse.
+#include <algorithm>
 #include <cmath>
 
 #include "gmg.hpp"
@@ -16,6 +17,7 @@ namespace {
 
 constexpr int kNT = 256;
 constexpr int kDiagBlocks = 1024;
+constexpr int kFlagBlocks = 2048;
 
 // results of one rows -> level build that the host needs
 struct LevelMeta {
@@ -38,6 +40,13 @@ __device__ double blk_sum(double v) {
     double r = 0;
     for (int q = 0; q < kNT / 64; ++q) r += sh[q];
     return r;
+}
+
+// sets *flag when pred holds on any thread of the block; every thread of the
+// block calls it (one atomic per block, not per thread: a flag that most
+// threads raise would otherwise serialise the whole grid on one address)
+__device__ __forceinline__ void block_flag(bool pred, int* flag) {
+    if (__syncthreads_or(pred) && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
 __device__ __forceinline__ bool differ(double u, double v) { return fabs(u - v) > 1e-12 * fmax(fabs(u), fabs(v)); }
@@ -78,26 +87,29 @@ __global__ __launch_bounds__(kNT) void k_faces(int d, long long N, const double*
     const int fn[3] = {e.n[0] + (d == 0), e.n[1] + (d == 1), e.n[2] + (d == 2)};
     const long long nf = (long long)fn[0] * fn[1] * fn[2];
     const long long f = (long long)blockIdx.x * kNT + threadIdx.x;
-    if (f >= nf) return;
-    int p[3];
-    p[0] = (int)(f % fn[0]);
-    p[1] = (int)((f / fn[0]) % fn[1]);
-    p[2] = (int)(f / ((long long)fn[0] * fn[1]));
-    const double s = m->sign;
-    const bool has_lo = p[d] < e.n[d], has_hi = p[d] > 0;
-    double alo = 0, ahi = 0;
-    if (has_lo) alo = -s * c[(size_t)(1 + 2 * d) * N + cell_lin(p, e.n)];
-    if (has_hi) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[d] -= 1;
-        ahi = -s * c[(size_t)(2 + 2 * d) * N + cell_lin(q, e.n)];
+    bool asym = false;
+    if (f < nf) {
+        int p[3];
+        p[0] = (int)(f % fn[0]);
+        p[1] = (int)((f / fn[0]) % fn[1]);
+        p[2] = (int)(f / ((long long)fn[0] * fn[1]));
+        const double s = m->sign;
+        const bool has_lo = p[d] < e.n[d], has_hi = p[d] > 0;
+        double alo = 0, ahi = 0;
+        if (has_lo) alo = -s * c[(size_t)(1 + 2 * d) * N + cell_lin(p, e.n)];
+        if (has_hi) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] -= 1;
+            ahi = -s * c[(size_t)(2 + 2 * d) * N + cell_lin(q, e.n)];
+        }
+        double a = has_lo ? alo : ahi;
+        if (has_lo && has_hi && differ(alo, ahi)) {
+            a = 0.5 * (alo + ahi);
+            asym = true;
+        }
+        af[f] = a;
     }
-    double a = has_lo ? alo : ahi;
-    if (has_lo && has_hi && differ(alo, ahi)) {
-        a = 0.5 * (alo + ahi);
-        atomicOr(&m->asym, 1);
-    }
-    af[f] = a;
+    block_flag(asym, &m->asym);
 }
 
 // a periodic dim's faces 0 and n are the same physical face
@@ -106,19 +118,22 @@ __global__ __launch_bounds__(kNT) void k_faces_periodic(int d, Ext e, LevelMeta*
     int pn[3] = {fn[0], fn[1], fn[2]};
     pn[d] = 1;
     const long long t = (long long)blockIdx.x * kNT + threadIdx.x;
-    if (t >= (long long)pn[0] * pn[1] * pn[2]) return;
-    int p[3];
-    p[0] = (int)(t % pn[0]);
-    p[1] = (int)((t / pn[0]) % pn[1]);
-    p[2] = (int)(t / ((long long)pn[0] * pn[1]));
-    int q[3] = {p[0], p[1], p[2]};
-    q[d] = e.n[d];
-    const long long i0 = cell_lin(p, fn), i1 = cell_lin(q, fn);
-    const double f0 = af[i0], f1 = af[i1];
-    if (differ(f0, f1)) {
-        af[i0] = af[i1] = 0.5 * (f0 + f1);
-        atomicOr(&m->asym, 1);
+    bool asym = false;
+    if (t < (long long)pn[0] * pn[1] * pn[2]) {
+        int p[3];
+        p[0] = (int)(t % pn[0]);
+        p[1] = (int)((t / pn[0]) % pn[1]);
+        p[2] = (int)(t / ((long long)pn[0] * pn[1]));
+        int q[3] = {p[0], p[1], p[2]};
+        q[d] = e.n[d];
+        const long long i0 = cell_lin(p, fn), i1 = cell_lin(q, fn);
+        const double f0 = af[i0], f1 = af[i1];
+        if (differ(f0, f1)) {
+            af[i0] = af[i1] = 0.5 * (f0 + f1);
+            asym = true;
+        }
     }
+    block_flag(asym, &m->asym);
 }
 
 // sigma = oriented diagonal minus the cell's face coefficients (dims in order, lo
@@ -143,13 +158,18 @@ __global__ __launch_bounds__(kNT) void k_sigma(int dim, long long N, const doubl
     sg[r] = v;
 }
 
-// uniformity (every entry equals the first) and, for sigma, any nonzero entry
+// uniformity (every entry equals the first) and, for sigma, any nonzero entry;
+// grid-stride over at most kFlagBlocks blocks
 __global__ __launch_bounds__(kNT) void k_flags(long long n, const double* __restrict__ v, LevelMeta* m, int is_sg) {
-    const long long t = (long long)blockIdx.x * kNT + threadIdx.x;
-    if (t >= n) return;
-    const double x = v[t];
-    if (x != v[0]) atomicOr(&m->nonconst, 1);
-    if (is_sg && x != 0.0) atomicOr(&m->shift, 1);
+    const double v0 = v[0];
+    bool nonconst = false, shift = false;
+    for (long long t = (long long)blockIdx.x * kNT + threadIdx.x; t < n; t += (long long)gridDim.x * kNT) {
+        const double x = v[t];
+        nonconst |= x != v0;
+        shift |= x != 0.0;
+    }
+    block_flag(nonconst, &m->nonconst);
+    if (is_sg) block_flag(shift, &m->shift);
 }
 
 __global__ void k_firsts(const double* __restrict__ af0, const double* __restrict__ af1,
@@ -206,11 +226,15 @@ __global__ __launch_bounds__(kNT) void k_to_f32(long long n, const double* __res
 
 __global__ __launch_bounds__(kNT) void k_neq(long long n, const double* __restrict__ a, const double* __restrict__ b,
                                              int* flag) {
-    const long long t = (long long)blockIdx.x * kNT + threadIdx.x;
-    if (t < n && a[t] != b[t]) atomicOr(flag, 1);
+    bool ne = false;
+    for (long long t = (long long)blockIdx.x * kNT + threadIdx.x; t < n; t += (long long)gridDim.x * kNT)
+        ne |= a[t] != b[t];
+    block_flag(ne, flag);
 }
 
 inline unsigned blocks(long long n) { return (unsigned)((n + kNT - 1) / kNT); }
+// grid of the grid-stride flag kernels (k_flags, k_neq)
+inline unsigned flag_blocks(long long n) { return std::min(blocks(n), (unsigned)kFlagBlocks); }
 
 inline Ext ext_of(const int n[3]) { return Ext{{n[0], n[1], n[2]}}; }
 
@@ -262,9 +286,9 @@ int rows_to_level_dev(hipStream_t st, const double* coef, long long N, const int
         hipLaunchKernelGGL(k_sigma, dim3(blocks(N)), dim3(kNT), 0, st, dim, N, coef, e, (const LevelMeta*)m,
                            (const double*)L.af[0], (const double*)L.af[1], (const double*)L.af[2], L.sg);
         for (int d = 0; d < dim; ++d)
-            hipLaunchKernelGGL(k_flags, dim3(blocks(level_face_count(nn, d))), dim3(kNT), 0, st,
+            hipLaunchKernelGGL(k_flags, dim3(flag_blocks(level_face_count(nn, d))), dim3(kNT), 0, st,
                                level_face_count(nn, d), (const double*)L.af[d], m, 0);
-        hipLaunchKernelGGL(k_flags, dim3(blocks(N)), dim3(kNT), 0, st, N, (const double*)L.sg, m, 1);
+        hipLaunchKernelGGL(k_flags, dim3(flag_blocks(N)), dim3(kNT), 0, st, N, (const double*)L.sg, m, 1);
         hipLaunchKernelGGL(k_firsts, dim3(1), dim3(1), 0, st, (const double*)L.af[0], (const double*)L.af[1],
                            (const double*)L.af[2], (const double*)L.sg, m);
         if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&hm, m, sizeof(hm), hipMemcpyDeviceToHost, st) ||
@@ -309,10 +333,10 @@ int dev_levels_equal(hipStream_t st, const DevLevel& a, const DevLevel& b, int d
     if (hipMemcpyAsync(flag, &h, sizeof(int), hipMemcpyHostToDevice, st)) rc = OPF_ERR_HIP;
     for (int d = 0; d < dim && !rc; ++d) {
         const long long nf = level_face_count(a.n, d);
-        hipLaunchKernelGGL(k_neq, dim3(blocks(nf)), dim3(kNT), 0, st, nf, (const double*)a.af[d],
+        hipLaunchKernelGGL(k_neq, dim3(flag_blocks(nf)), dim3(kNT), 0, st, nf, (const double*)a.af[d],
                            (const double*)b.af[d], flag);
     }
-    if (!rc) hipLaunchKernelGGL(k_neq, dim3(blocks(a.N)), dim3(kNT), 0, st, a.N, (const double*)a.sg,
+    if (!rc) hipLaunchKernelGGL(k_neq, dim3(flag_blocks(a.N)), dim3(kNT), 0, st, a.N, (const double*)a.sg,
                                 (const double*)b.sg, flag);
     if (!rc && (hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)))
         rc = OPF_ERR_HIP;
