@@ -244,7 +244,12 @@ __global__ __launch_bounds__(64 * NTY) void ftcs3d_lds_kernel(const T* __restric
     static_assert(128 + 2 * TY <= 64 * NTY, "one halo element per thread at most");
     __shared__ T lds[2][TY + 2][FT3_TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * 64 + tx;
-    const int x0 = blockIdx.x * FT3_TX, y0 = blockIdx.y * TY, k0 = blockIdx.z * ZC;
+    // tile of this block: the flat id through xcd_block, so that each XCD's L2 holds a
+    // contiguous slab whose x / y halo lines its own tiles share
+    const uint32_t L = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                 gridDim.x * gridDim.y * gridDim.z);
+    const int x0 = (int)(L % gridDim.x) * FT3_TX, y0 = (int)((L / gridDim.x) % gridDim.y) * TY,
+              k0 = (int)(L / (gridDim.x * gridDim.y)) * ZC;
     const int kend = min(k0 + ZC, a.n[2]);
     const long long sy = a.e0, sz = (long long)a.e0 * a.e1;
     const int i = x0 + tx;

@@ -108,6 +108,17 @@ struct Known {
 };
 
 // the straight-line kernel of program P in precision T, writing out (op) value
+// the block's tile (bx, by, bz) from its hardware id: workgroups are dispatched round-robin
+// over the 8 XCDs (each with its own L2), so the flat id is remapped to give every XCD one
+// contiguous range of tiles (xcd_block, fastdiv.hpp): x and y neighbours, whose halo rows
+// are the same cache lines, then share one L2 instead of each fetching them from HBM
+const char* const kXcdBlockSrc =
+    "  const unsigned gx_ = gridDim.x, gy_ = gridDim.y, G_ = gx_ * gy_ * gridDim.z;\n"
+    "  const unsigned lin_ = blockIdx.x + gx_ * (blockIdx.y + gy_ * blockIdx.z);\n"
+    "  const unsigned q_ = G_ >> 3, r_ = G_ & 7u, xb_ = lin_ & 7u;\n"
+    "  const unsigned L_ = xb_ * q_ + (xb_ < r_ ? xb_ : r_) + (lin_ >> 3);\n"
+    "  const int bx = (int)(L_ % gx_), by = (int)((L_ / gx_) % gy_), bz = (int)(L_ / (gx_ * gy_));\n";
+
 int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArgs& a, std::string& s) {
     s.clear();
     s += dtype == OPF_F64 ? "typedef double T;\n" : "typedef float T;\n";
@@ -116,14 +127,15 @@ int gen_source(const LoweredProgram& P, int dtype, int op, int R, const SweepArg
     s += kWenoSrc;
     s += kMathSrc;
     {
-        char hdr[512];
+        char hdr[2048];
         snprintf(hdr, sizeof hdr,
                  "\nextern \"C\" __global__ __launch_bounds__(256) void opf_sweep_jit(SweepArgs a) {\n"
-                 "  const int i = a.s[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
-                 "  const int j = a.s[1] + (int)blockIdx.y * 4 + (int)threadIdx.y;\n"
+                 "%s"
+                 "  const int i = a.s[0] + bx * 64 + (int)threadIdx.x;\n"
+                 "  const int j = a.s[1] + by * 4 + (int)threadIdx.y;\n"
                  "  if (i >= a.s[0] + a.n[0] || j >= a.s[1] + a.n[1]) return;\n"
-                 "  const int k = a.s[2] + (int)blockIdx.z * %d;\n",
-                 R);
+                 "  const int k = a.s[2] + bz * %d;\n",
+                 kXcdBlockSrc, R);
         s += hdr;
     }
     bool fu[SWEEP_MAX_FIELDS] = {}, dxu[3] = {}, xu[3] = {};
@@ -357,16 +369,17 @@ int gen_source_tile(const LoweredProgram& P, int dtype, int op, int RY, bool pla
     s += kWenoSrc;
     s += kMathSrc;
     const int TY = kTileNTY * RY;
-    char line[640];
+    char line[2048];
     snprintf(line, sizeof line,
              "\nextern \"C\" __global__ __launch_bounds__(256) void opf_sweep_jit(SweepArgs a) {\n"
+             "%s"
              "  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;\n"
-             "  const int x0 = a.s[0] + (int)blockIdx.x * 64, y0 = a.s[1] + (int)blockIdx.y * %d;\n"
-             "  const int kb = a.s[2] + (int)blockIdx.z * a.zc;\n"
+             "  const int x0 = a.s[0] + bx * 64, y0 = a.s[1] + by * %d;\n"
+             "  const int kb = a.s[2] + bz * a.zc;\n"
              "  const int ke = min(kb + a.zc, a.s[2] + a.n[2]);\n"
              "  const int xe = a.s[0] + a.n[0], ye = a.s[1] + a.n[1];\n"
              "  const int i = x0 + tx, jb = y0 + ty * %d;\n",
-             TY, RY);
+             kXcdBlockSrc, TY, RY);
     s += line;
     bool dxu[3] = {}, xu[3] = {};
     for (const Insn& in : P.code) {
