@@ -102,7 +102,7 @@ struct Bat {
     (void)vo;
 
 __device__ double gmg_block_sum(double v) {
-    __shared__ double sh[8];
+    __shared__ double sh[16];  // blocks of up to 1024 threads
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (l == 0) sh[w] = v;
@@ -779,16 +779,16 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
 //       over each 2 x 2 block; x2 is stored.
 // up:   x3 = x2 + P x_c, x4 = x3 + w (b - A x3) / D, x5 = x4 + w (b - A x4) / D -> out,
 //       on the finest level with the partials of (b, x5), (x5, q), sum(x5).
-// A 256-thread workgroup owns a 32 x 32 tile; the intermediate vectors live in LDS on
-// the tile plus a halo of 2 (x1, x3), 1 (x2, x4) and are recomputed by neighbouring
-// tiles instead of going through memory, so a level costs two launches instead of
-// four or five and each fine vector crosses HBM once per phase. Halo positions of a
-// periodic dim hold the wrapped cells; those outside a non-periodic boundary are
-// never read (the boundary cell folds its ghost into D, as apply_op does). Blocks
-// stride over the tiles (a fixed grid: the partial sums are reproducible).
-constexpr int F2_T = 32;                // tile side
-constexpr int F2_W2 = F2_T + 4;         // region with halo 2
-constexpr int F2_W1 = F2_T + 2;         // region with halo 1
+// A workgroup owns a TS x TS tile (32 x 32 with 256 threads on levels of >= 2^20
+// cells, 16 x 16 with 256 threads down to 2^16, 8 x 8 with one wave below: the small
+// levels are latency-bound, so fewer cells per thread and more workgroups pay); the
+// intermediate vectors live in LDS on the tile plus a halo of 2 (x1, x3), 1 (x2, x4)
+// and are recomputed by neighbouring tiles instead of going through memory, so a
+// level costs two launches instead of four or five and each fine vector crosses HBM
+// once per phase. Halo positions of a periodic dim hold the wrapped cells; those
+// outside a non-periodic boundary are never read (the boundary cell folds its ghost
+// into D, as apply_op does). Blocks stride over the tiles (a fixed grid: the partial
+// sums are reproducible).
 
 struct F2Tile {
     int tx, ty, tiles;
@@ -831,77 +831,220 @@ __device__ __forceinline__ T f2_apply(const GOp<T>& o, int i, int j, const T* v,
     return D * v[c] - acc;
 }
 
+// w r / D at cell (i, j): off the boundary as o.wdi r (the interior scaling of the
+// per-level kernels: no division on the cells of a wave that touches no boundary)
 template <typename T>
-__global__ __launch_bounds__(256) void k2_down(GOp<T> o, const T* __restrict__ b, T* __restrict__ x2o,
-                                               T* __restrict__ bc, int c0, int c1, T w, F2Tile ft, Bat bt) {
-    __shared__ T sb[F2_W2 * F2_W2], s1[F2_W2 * F2_W2], s2[F2_W1 * F2_W1], sr[F2_T * F2_T];
+__device__ __forceinline__ T f2_corr(const GOp<T>& o, int i, int j, T r, T D, T w) {
+    if (i > 0 && i < o.n0 - 1 && j > 0 && j < o.n1 - 1) return o.wdi * r;
+    return w * r / D;
+}
+
+// A tile whose region with halo 2 lies inside the level and whose cells are all off
+// the boundary: no wrapped or missing neighbour, D = dint everywhere. Such tiles take
+// the straight-line path (f2_apply_in, prolong_in, o.wdi), which gives the general
+// path's bits for those cells; tiles along the boundary take the general path.
+template <typename T, int TS>
+__device__ __forceinline__ bool f2_inner(const GOp<T>& o, int x0, int y0) {
+    return x0 >= 3 && y0 >= 3 && x0 + TS + 3 <= o.n0 && y0 + TS + 3 <= o.n1;
+}
+
+// (A v)_c of an interior cell (f2_apply's order, D = dint)
+template <typename T>
+__device__ __forceinline__ T f2_apply_in(const GOp<T>& o, const T* v, int c, int W) {
+    T acc = 0;
+    acc += o.a[0] * v[c - 1];
+    acc += o.a[0] * v[c + 1];
+    acc += o.a[1] * v[c - W];
+    acc += o.a[1] * v[c + W];
+    return o.dint * v[c] - acc;
+}
+
+// (P x_c) at fine cell (i, j) whose four coarse cells lie inside the coarse level
+// (prolong_at's weights and order, no boundary factor)
+template <typename T>
+__device__ __forceinline__ T prolong_in(const T* __restrict__ xc, int c0, int i, int j) {
+    const int pi = i >> 1, pj = j >> 1, oi = (i & 1) ? pi + 1 : pi - 1, oj = (j & 1) ? pj + 1 : pj - 1;
+    T s = 0;
+    s += (T)0.5625 * xc[(uint32_t)pi + (uint32_t)c0 * (uint32_t)pj];
+    s += (T)0.1875 * xc[(uint32_t)oi + (uint32_t)c0 * (uint32_t)pj];
+    s += (T)0.1875 * xc[(uint32_t)pi + (uint32_t)c0 * (uint32_t)oj];
+    s += (T)0.0625 * xc[(uint32_t)oi + (uint32_t)c0 * (uint32_t)oj];
+    return s;
+}
+
+// one TS x TS tile of k2_down (IN: f2_inner holds for the tile)
+template <typename T, int TS, bool IN>
+__device__ __forceinline__ void k2_down_tile(const GOp<T>& o, const T* __restrict__ b, T* __restrict__ x2o,
+                                             T* __restrict__ bc, int c0, int c1, T w, int x0, int y0, T* sb, T* s1,
+                                             T* s2) {
+    constexpr int W2 = TS + 4, W1 = TS + 2;
+    // b and x1 = w b / D on the tile + 2
+    for (int p = threadIdx.x; p < W2 * W2; p += blockDim.x) {
+        int i, j;
+        bool ok = true;
+        if (IN) {
+            i = x0 - 2 + p % W2;
+            j = y0 - 2 + p / W2;
+        } else {
+            ok = f2_map(x0 - 2 + p % W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / W2, o.n1, o.per[1], j);
+        }
+        T bv = 0, x1 = 0;
+        if (ok) {
+            bv = b[(uint32_t)i + o.s1 * (uint32_t)j];
+            x1 = IN ? o.wdi * bv : f2_corr<T>(o, i, j, bv, diag_const<T, 2>(o, i, j, 0), w);
+        }
+        sb[p] = bv;
+        s1[p] = x1;
+    }
+    __syncthreads();
+    // x2 on the tile + 1
+    for (int p = threadIdx.x; p < W1 * W1; p += blockDim.x) {
+        const int li = p % W1, lj = p / W1, c = (lj + 1) * W2 + li + 1;
+        T x2 = 0;
+        if (IN) {
+            x2 = s1[c] + o.wdi * (sb[c] - f2_apply_in<T>(o, s1, c, W2));
+        } else {
+            int i, j;
+            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
+                T D;
+                const T ax = f2_apply<T>(o, i, j, s1, c, W2, D);
+                x2 = s1[c] + f2_corr<T>(o, i, j, sb[c] - ax, D, w);
+            }
+        }
+        s2[p] = x2;
+    }
+    __syncthreads();
+    // thread q: coarse cell q of the tile; r = b - A x2 on its four children (x2
+    // stored), coarse b = the children's r summed as k_restrict sums them
+    if (threadIdx.x < (TS / 2) * (TS / 2)) {
+        const int q = threadIdx.x, ci = q % (TS / 2), cj = q / (TS / 2);
+        T s = 0;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int li = 2 * ci + di, lj = 2 * cj + dj, i = x0 + li, j = y0 + lj;
+                if (IN || (i < o.n0 && j < o.n1)) {
+                    const int c = (lj + 1) * W1 + li + 1;
+                    T ax;
+                    if (IN) {
+                        ax = f2_apply_in<T>(o, s2, c, W1);
+                    } else {
+                        T D;
+                        ax = f2_apply<T>(o, i, j, s2, c, W1, D);
+                    }
+                    s += sb[(lj + 2) * W2 + li + 2] - ax;
+                    x2o[(uint32_t)i + o.s1 * (uint32_t)j] = s2[c];
+                } else {
+                    s += (T)0;
+                }
+            }
+        const int I = x0 / 2 + ci, J = y0 / 2 + cj;
+        if (IN || (I < c0 && J < c1)) bc[(uint32_t)I + (uint32_t)c0 * (uint32_t)J] = s * (T)0.25;
+    }
+    __syncthreads();  // the LDS tiles are reused by the next tile
+}
+
+template <typename T, int TS, int NT>
+__global__ __launch_bounds__(NT) void k2_down(GOp<T> o, const T* __restrict__ b, T* __restrict__ x2o,
+                                              T* __restrict__ bc, int c0, int c1, T w, F2Tile ft, Bat bt) {
+    static_assert((TS / 2) * (TS / 2) <= NT, "one coarse cell of the tile per thread");
+    constexpr int W2 = TS + 4, W1 = TS + 2;
+    __shared__ T sb[W2 * W2], s1[W2 * W2], s2[W1 * W1];
     BAT_SLOT(bt);
     b += vo;
     x2o += vo;
     bc += (long long)slot * ((long long)c0 * c1);
     for (int tile = (int)xcd_block(blockIdx.x, gridDim.x); tile < ft.tiles; tile += gridDim.x) {
-        const int x0 = (tile % ft.tx) * F2_T, y0 = (tile / ft.tx) * F2_T;
-        // b and x1 = w b / D on the tile + 2
-        for (int p = threadIdx.x; p < F2_W2 * F2_W2; p += blockDim.x) {
-            int i, j;
-            const bool ok = f2_map(x0 - 2 + p % F2_W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / F2_W2, o.n1, o.per[1], j);
-            T bv = 0, x1 = 0;
-            if (ok) {
-                bv = b[(uint32_t)i + o.s1 * (uint32_t)j];
-                x1 = w * bv / diag_const<T, 2>(o, i, j, 0);
-            }
-            sb[p] = bv;
-            s1[p] = x1;
-        }
-        __syncthreads();
-        // x2 on the tile + 1
-        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
-            const int li = p % F2_W1, lj = p / F2_W1, c = (lj + 1) * F2_W2 + li + 1;
-            int i, j;
-            T x2 = 0;
-            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
-                T D;
-                const T ax = f2_apply<T>(o, i, j, s1, c, F2_W2, D);
-                x2 = s1[c] + w * (sb[c] - ax) / D;
-            }
-            s2[p] = x2;
-        }
-        __syncthreads();
-        // r = b - A x2 on the tile; x2 stored
-        for (int p = threadIdx.x; p < F2_T * F2_T; p += blockDim.x) {
-            const int li = p % F2_T, lj = p / F2_T, i = x0 + li, j = y0 + lj;
-            const int c = (lj + 1) * F2_W1 + li + 1;
-            T r = 0;
-            if (i < o.n0 && j < o.n1) {
-                T D;
-                r = sb[(lj + 2) * F2_W2 + li + 2] - f2_apply<T>(o, i, j, s2, c, F2_W1, D);
-                x2o[(uint32_t)i + o.s1 * (uint32_t)j] = s2[c];
-            }
-            sr[p] = r;
-        }
-        __syncthreads();
-        // coarse b: the children summed as k_restrict sums them
-        {
-            const int q = threadIdx.x, I = x0 / 2 + q % (F2_T / 2), J = y0 / 2 + q / (F2_T / 2);
-            if (I < c0 && J < c1) {
-                const int li = 2 * (q % (F2_T / 2)), lj = 2 * (q / (F2_T / 2));
-                T s = 0;
-                s += sr[lj * F2_T + li];
-                s += sr[lj * F2_T + li + 1];
-                s += sr[(lj + 1) * F2_T + li];
-                s += sr[(lj + 1) * F2_T + li + 1];
-                bc[(uint32_t)I + (uint32_t)c0 * (uint32_t)J] = s * (T)0.25;
-            }
-        }
-        __syncthreads();  // the LDS tiles are reused by the next tile
+        const int x0 = (tile % ft.tx) * TS, y0 = (tile / ft.tx) * TS;
+        if (f2_inner<T, TS>(o, x0, y0)) k2_down_tile<T, TS, true>(o, b, x2o, bc, c0, c1, w, x0, y0, sb, s1, s2);
+        else k2_down_tile<T, TS, false>(o, b, x2o, bc, c0, c1, w, x0, y0, sb, s1, s2);
     }
 }
 
-template <typename T, bool DOTS>
-__global__ __launch_bounds__(256) void k2_up(GOp<T> o, const T* __restrict__ x2, const T* __restrict__ xc, int c0,
-                                             int c1, const T* __restrict__ b, T* __restrict__ out, T w,
-                                             const T* __restrict__ q, double* __restrict__ part, F2Tile ft, Bat bt) {
-    __shared__ T s3[F2_W2 * F2_W2], sb[F2_W1 * F2_W1], s4[F2_W1 * F2_W1];
+// one TS x TS tile of k2_up (IN: f2_inner holds for the tile)
+template <typename T, int TS, bool DOTS, bool IN>
+__device__ __forceinline__ void k2_up_tile(const GOp<T>& o, const T* __restrict__ x2, const T* __restrict__ xc,
+                                           int c0, int c1, const T* __restrict__ b, T* __restrict__ out, T w,
+                                           const T* __restrict__ q, int x0, int y0, T* s3, T* sb, T* s4, double& d0,
+                                           double& d1, double& d2) {
+    constexpr int W2 = TS + 4, W1 = TS + 2;
+    const int cn[3] = {c0, c1, 1};
+    // x3 = x2 + P x_c on the tile + 2
+    for (int p = threadIdx.x; p < W2 * W2; p += blockDim.x) {
+        int i, j;
+        T x3 = 0;
+        if (IN) {
+            i = x0 - 2 + p % W2;
+            j = y0 - 2 + p / W2;
+            x3 = x2[(uint32_t)i + o.s1 * (uint32_t)j] + prolong_in<T>(xc, c0, i, j);
+        } else if (f2_map(x0 - 2 + p % W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / W2, o.n1, o.per[1], j)) {
+            x3 = x2[(uint32_t)i + o.s1 * (uint32_t)j] + prolong_at<T, 2>(o, xc, cn, i, j, 0);
+        }
+        s3[p] = x3;
+    }
+    // b on the tile + 1
+    for (int p = threadIdx.x; p < W1 * W1; p += blockDim.x) {
+        int i, j;
+        bool ok = true;
+        if (IN) {
+            i = x0 - 1 + p % W1;
+            j = y0 - 1 + p / W1;
+        } else {
+            ok = f2_map(x0 - 1 + p % W1, o.n0, o.per[0], i) && f2_map(y0 - 1 + p / W1, o.n1, o.per[1], j);
+        }
+        sb[p] = ok ? b[(uint32_t)i + o.s1 * (uint32_t)j] : (T)0;
+    }
+    __syncthreads();
+    // x4 on the tile + 1
+    for (int p = threadIdx.x; p < W1 * W1; p += blockDim.x) {
+        const int li = p % W1, lj = p / W1, c = (lj + 1) * W2 + li + 1;
+        T x4 = 0;
+        if (IN) {
+            x4 = s3[c] + o.wdi * (sb[p] - f2_apply_in<T>(o, s3, c, W2));
+        } else {
+            int i, j;
+            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
+                T D;
+                const T ax = f2_apply<T>(o, i, j, s3, c, W2, D);
+                x4 = s3[c] + f2_corr<T>(o, i, j, sb[p] - ax, D, w);
+            }
+        }
+        s4[p] = x4;
+    }
+    __syncthreads();
+    // x5 on the tile -> out (+ the dot products)
+    for (int p = threadIdx.x; p < TS * TS; p += blockDim.x) {
+        const int li = p % TS, lj = p / TS, i = x0 + li, j = y0 + lj;
+        if (!IN && (i >= o.n0 || j >= o.n1)) continue;
+        const int c = (lj + 1) * W1 + li + 1;
+        const T bv = sb[c];
+        T x5;
+        if (IN) {
+            x5 = s4[c] + o.wdi * (bv - f2_apply_in<T>(o, s4, c, W1));
+        } else {
+            T D;
+            const T ax = f2_apply<T>(o, i, j, s4, c, W1, D);
+            x5 = s4[c] + f2_corr<T>(o, i, j, bv - ax, D, w);
+        }
+        const uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j;
+        out[t] = x5;
+        if (DOTS) {
+            const double zv = (double)x5;
+            d0 += (double)bv * zv;
+            d1 += zv * (double)q[t];
+            d2 += zv;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int TS, int NT, bool DOTS>
+__global__ __launch_bounds__(NT) void k2_up(GOp<T> o, const T* __restrict__ x2, const T* __restrict__ xc, int c0,
+                                            int c1, const T* __restrict__ b, T* __restrict__ out, T w,
+                                            const T* __restrict__ q, double* __restrict__ part, F2Tile ft, Bat bt) {
+    constexpr int W2 = TS + 4, W1 = TS + 2;
+    __shared__ T s3[W2 * W2], sb[W1 * W1], s4[W1 * W1];
     BAT_SLOT(bt);
     x2 += vo;
     b += vo;
@@ -911,59 +1054,13 @@ __global__ __launch_bounds__(256) void k2_up(GOp<T> o, const T* __restrict__ x2,
         q += vo;
         part += (long long)slot * bt.ps;
     }
-    const int cn[3] = {c0, c1, 1};
     double d0 = 0, d1 = 0, d2 = 0;
     for (int tile = (int)xcd_block(blockIdx.x, gridDim.x); tile < ft.tiles; tile += gridDim.x) {
-        const int x0 = (tile % ft.tx) * F2_T, y0 = (tile / ft.tx) * F2_T;
-        // x3 = x2 + P x_c on the tile + 2
-        for (int p = threadIdx.x; p < F2_W2 * F2_W2; p += blockDim.x) {
-            int i, j;
-            T x3 = 0;
-            if (f2_map(x0 - 2 + p % F2_W2, o.n0, o.per[0], i) && f2_map(y0 - 2 + p / F2_W2, o.n1, o.per[1], j))
-                x3 = x2[(uint32_t)i + o.s1 * (uint32_t)j] + prolong_at<T, 2>(o, xc, cn, i, j, 0);
-            s3[p] = x3;
-        }
-        // b on the tile + 1
-        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
-            int i, j;
-            T bv = 0;
-            if (f2_map(x0 - 1 + p % F2_W1, o.n0, o.per[0], i) && f2_map(y0 - 1 + p / F2_W1, o.n1, o.per[1], j))
-                bv = b[(uint32_t)i + o.s1 * (uint32_t)j];
-            sb[p] = bv;
-        }
-        __syncthreads();
-        // x4 on the tile + 1
-        for (int p = threadIdx.x; p < F2_W1 * F2_W1; p += blockDim.x) {
-            const int li = p % F2_W1, lj = p / F2_W1, c = (lj + 1) * F2_W2 + li + 1;
-            int i, j;
-            T x4 = 0;
-            if (f2_map(x0 - 1 + li, o.n0, o.per[0], i) && f2_map(y0 - 1 + lj, o.n1, o.per[1], j)) {
-                T D;
-                const T ax = f2_apply<T>(o, i, j, s3, c, F2_W2, D);
-                x4 = s3[c] + w * (sb[p] - ax) / D;
-            }
-            s4[p] = x4;
-        }
-        __syncthreads();
-        // x5 on the tile -> out (+ the dot products)
-        for (int p = threadIdx.x; p < F2_T * F2_T; p += blockDim.x) {
-            const int li = p % F2_T, lj = p / F2_T, i = x0 + li, j = y0 + lj;
-            if (i >= o.n0 || j >= o.n1) continue;
-            const int c = (lj + 1) * F2_W1 + li + 1;
-            T D;
-            const T ax = f2_apply<T>(o, i, j, s4, c, F2_W1, D);
-            const T bv = sb[c];
-            const T x5 = s4[c] + w * (bv - ax) / D;
-            const uint32_t t = (uint32_t)i + o.s1 * (uint32_t)j;
-            out[t] = x5;
-            if (DOTS) {
-                const double zv = (double)x5;
-                d0 += (double)bv * zv;
-                d1 += zv * (double)q[t];
-                d2 += zv;
-            }
-        }
-        __syncthreads();
+        const int x0 = (tile % ft.tx) * TS, y0 = (tile / ft.tx) * TS;
+        if (f2_inner<T, TS>(o, x0, y0))
+            k2_up_tile<T, TS, DOTS, true>(o, x2, xc, c0, c1, b, out, w, q, x0, y0, s3, sb, s4, d0, d1, d2);
+        else
+            k2_up_tile<T, TS, DOTS, false>(o, x2, xc, c0, c1, b, out, w, q, x0, y0, s3, sb, s4, d0, d1, d2);
     }
     if (DOTS) {
         d0 = gmg_block_sum(d0);
@@ -1017,15 +1114,18 @@ __device__ void tail_sweep(const GOp<T>& o, int N, const T* xi, const T* b, T* x
         int i, j, k;
         cell_of((uint32_t)t, o.f0, o.f1, i, j, k);
         T D;
+        // constant mode off the boundary: the interior scaling o.wdi, as the per-level kernels
+        const bool inner = !VAR && interior3<T, DIM>(o, i, j, k);
         if (zero) {
             auto zv = [&](uint32_t) { return (T)0; };
             apply_op<T, VAR, DIM>(o, i, j, k, (uint32_t)t, (T)0, zv, D);
-            xo[t] = w * b[t] / D;
+            xo[t] = inner ? o.wdi * b[t] : w * b[t] / D;
         } else {
             auto xv = [&](uint32_t l) { return xi[l]; };
             const T xc = xi[t];
             const T ax = apply_op<T, VAR, DIM>(o, i, j, k, (uint32_t)t, xc, xv, D);
-            xo[t] = xc + w * (b[t] - ax) / D;
+            const T r = b[t] - ax;
+            xo[t] = xc + (inner ? o.wdi * r : w * r / D);
         }
     }
     __syncthreads();
@@ -1133,7 +1233,8 @@ __global__ __launch_bounds__(GMG_TAIL_NT) void k_vtail(TailArgs<T> ta, Bat bt) {
     }
 }
 
-__global__ void k_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ s, double invN,
+__global__ __launch_bounds__(1024) void k_finish(const double* __restrict__ part, int nb, int mode,
+                                                 double* __restrict__ s, double invN,
                          int singular, Bat bt, int* __restrict__ mk_it, int* __restrict__ mk_rep, int maxit) {
     BAT_SLOT(bt);
     if (mode == FIN_RR_DECIDE && !mk_it[slot]) {  // launched for every slot: a stopped one is not updated
@@ -1142,23 +1243,31 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int mode, doub
     }
     part += (long long)slot * bt.ps;
     s += (long long)slot * bt.ss;
-    // up to 16384 partials per value: independent loads, four accumulators per
-    // value per thread (a fixed order: the sums are reproducible)
+    // up to 16384 partials per value: four accumulators per value per thread, the
+    // loads of every value issued together (one latency round per 4 x blockDim
+    // partials); a fixed order, so the sums are reproducible
     const int nv = (mode == FIN_DOT3 || mode == FIN_DOT3_FIRST) ? 3 : 1;
     double v[3] = {0, 0, 0};
-    for (int q = 0; q < nv; ++q) {
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        const double* pq = part + q * GMG_MAXB;
+    {
+        double a[3][4] = {};
         const int step = blockDim.x;
         int t = threadIdx.x;
-        for (; t + 3 * step < nb; t += 4 * step) {
-            a0 += pq[t];
-            a1 += pq[t + step];
-            a2 += pq[t + 2 * step];
-            a3 += pq[t + 3 * step];
-        }
-        for (; t < nb; t += step) a0 += pq[t];
-        v[q] = (a0 + a1) + (a2 + a3);
+        for (; t + 3 * step < nb; t += 4 * step)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q < nv) {
+                    const double* pq = part + q * GMG_MAXB;
+                    a[q][0] += pq[t];
+                    a[q][1] += pq[t + step];
+                    a[q][2] += pq[t + 2 * step];
+                    a[q][3] += pq[t + 3 * step];
+                }
+        for (; t < nb; t += step)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q < nv) a[q][0] += part[q * GMG_MAXB + t];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[q] = (a[q][0] + a[q][1]) + (a[q][2] + a[q][3]);
     }
     for (int q = 0; q < nv; ++q) v[q] = gmg_block_sum(v[q]);
     if (threadIdx.x != 0) return;
@@ -1936,14 +2045,17 @@ static void ph_vtail(const Hierarchy& H, hipStream_t st, const int* mask, int l0
                                                   st, ta, bt));
 }
 
-// 2-D constant-coefficient V(2,2) levels run as k2_down / k2_up: one block per 32 x 32
-// tile (at most GMG_MAXB, then striding); on the finest level k2_up writes the partials
+// 2-D constant-coefficient V(2,2) levels run as k2_down / k2_up: one block per tile
+// (at most GMG_MAXB, then striding); on the finest level k2_up writes the partials
 // of the three dot products, one per block
 static bool fused2d(const Hierarchy& H) { return H.dim == 2 && !H.var && H.pre == 2 && H.post == 2; }
+// tile side of a level's k2 kernels (threads: 256 for 32 and 16, 64 for 8)
+static int f2_ts(const GLevelHost& L) { return L.N >= (1LL << 20) ? 32 : L.N >= (1LL << 16) ? 16 : 8; }
 static F2Tile f2tile(const GLevelHost& L) {
+    const int ts = f2_ts(L);
     F2Tile ft;
-    ft.tx = (L.n[0] + F2_T - 1) / F2_T;
-    ft.ty = (L.n[1] + F2_T - 1) / F2_T;
+    ft.tx = (L.n[0] + ts - 1) / ts;
+    ft.ty = (L.n[1] + ts - 1) / ts;
     ft.tiles = ft.tx * ft.ty;
     return ft;
 }
@@ -1980,15 +2092,28 @@ void ph_vcycle(const Hierarchy& H, hipStream_t st, const int* mask, int l, const
         const F2Tile ft = f2tile(L);
         const unsigned G = (unsigned)std::min(ft.tiles, GMG_MAXB);
         const Bat bt = bat(H, mask, N);
-        hipLaunchKernelGGL(k2_down<T>, dim3(G, B), dim3(256), 0, st, o, rhs, (T*)D.t, (T*)C.b, C.n[0], C.n[1], w, ft,
-                           bt);
+        const int ts = f2_ts(L);
+        auto by_ts = [&](auto launch) {
+            if (ts == 32) launch(std::integral_constant<int, 32>{}, std::integral_constant<int, 256>{});
+            else if (ts == 16) launch(std::integral_constant<int, 16>{}, std::integral_constant<int, 256>{});
+            else launch(std::integral_constant<int, 8>{}, std::integral_constant<int, 64>{});
+        };
+        by_ts([&](auto TS_, auto NT_) {
+            constexpr int TS = decltype(TS_)::value, NT = decltype(NT_)::value;
+            hipLaunchKernelGGL((k2_down<T, TS, NT>), dim3(G, B), dim3(NT), 0, st, o, rhs, (T*)D.t, (T*)C.b, C.n[0],
+                               C.n[1], w, ft, bt);
+        });
         ph_vcycle<T>(H, st, mask, l + 1, (const T*)C.b, (T*)C.x);
-        if (dot_q)  // one partial per block (dot3_parts)
-            hipLaunchKernelGGL((k2_up<T, true>), dim3(G, B), dim3(256), 0, st, o, (const T*)D.t, (const T*)C.x,
-                               C.n[0], C.n[1], rhs, out, w, dot_q, H.part, ft, bt);
-        else
-            hipLaunchKernelGGL((k2_up<T, false>), dim3(G, B), dim3(256), 0, st, o, (const T*)D.t, (const T*)C.x,
-                               C.n[0], C.n[1], rhs, out, w, (const T*)nullptr, (double*)nullptr, ft, bt);
+        by_ts([&](auto TS_, auto NT_) {
+            constexpr int TS = decltype(TS_)::value, NT = decltype(NT_)::value;
+            if (dot_q)  // one partial per block (dot3_parts)
+                hipLaunchKernelGGL((k2_up<T, TS, NT, true>), dim3(G, B), dim3(NT), 0, st, o, (const T*)D.t,
+                                   (const T*)C.x, C.n[0], C.n[1], rhs, out, w, dot_q, H.part, ft, bt);
+            else
+                hipLaunchKernelGGL((k2_up<T, TS, NT, false>), dim3(G, B), dim3(NT), 0, st, o, (const T*)D.t,
+                                   (const T*)C.x, C.n[0], C.n[1], rhs, out, w, (const T*)nullptr, (double*)nullptr,
+                                   ft, bt);
+        });
         return;
     }
     // Jacobi sweeps ping-pong between `out` and the level temp; start on the
@@ -2098,7 +2223,7 @@ static void ph_finish(const Hierarchy& H, hipStream_t st, const int* mask, int m
                       int* m_it = nullptr, int* m_rep = nullptr, int maxit = 0) {
     const long long N = H.host[0].N;
     const int nb = mode == FIN_DOT3 || mode == FIN_DOT3_FIRST ? dot3_parts<T>(H) : red_count<T>(H);
-    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(512), 0, st, H.part, nb, mode, H.scal, 1.0 / N,
+    hipLaunchKernelGGL(k_finish, dim3(1, (unsigned)H.B), dim3(1024), 0, st, H.part, nb, mode, H.scal, 1.0 / N,
                        singular, bat(H, mask, N), m_it, m_rep, maxit);
 }
 
@@ -2235,14 +2360,18 @@ void ph_rel_iter(const Hierarchy& H, hipStream_t st, double* x64, const double* 
         GMG_DISPATCH(H.var, H.dim, hipLaunchKernelGGL((k_spmv_p<T, V_, D_>), dim3(G, B), dim3(256), 0, st, o,
                                                       (const T*)z, (const T*)pold, pnew, q, H.scal, H.part, bt));
     ph_finish<T>(H, st, m_it, FIN_PQ, H.singular);
-    hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, e, (const T*)pnew, r, (const T*)q, H.scal, H.part,
-                       bt);
+    // FP64: x advances in place (no correction vector, no fold-in launch); FP32: the
+    // correction e accumulates in FP32 and is folded into x at each reliable update
+    constexpr bool direct = std::is_same<T, double>::value;
+    hipLaunchKernelGGL(k_axpy2<T>, dim3(G, B), dim3(256), 0, st, N, direct ? reinterpret_cast<T*>(x64) : e,
+                       (const T*)pnew, r, (const T*)q, H.scal, H.part, bt);
     // (r, r), the iteration counted and the reliable-update decision (sets m_rep)
     ph_finish<T>(H, st, nullptr, FIN_RR_DECIDE, H.singular, m_it, m_rep, maxit);
-    // reliable update: x += e, e = 0, r = b - A x in FP64 (the direction p is kept); its
-    // finish stops the slots that meet the tolerance (clears m_it)
-    hipLaunchKernelGGL(k_add_zero<T>, dim3((unsigned)rblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e,
-                       bat(H, m_rep, N));
+    // reliable update: x += e, e = 0 (FP32), r = b - A x in FP64 (the direction p is
+    // kept); its finish stops the slots that meet the tolerance (clears m_it)
+    if (!direct)
+        hipLaunchKernelGGL(k_add_zero<T>, dim3((unsigned)rblk(N), (unsigned)B), dim3(256), 0, st, N, x64, e,
+                           bat(H, m_rep, N));
     ph_resid64<T>(H, st, m_rep, x64, b64, r, FIN_REP_DONE, m_it, maxit);
     if (H.singular) ph_project<T>(H, st, m_rep, r);
     // next preconditioned residual for the slots still iterating

@@ -253,6 +253,21 @@ def test_sweep_weno_periodic(ctx, engine, kind):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("op", [0, 2])
+def test_sweep_3d_periodic_ghosts(ctx, engine, op):
+    """Every dim periodic, the whole range assigned, 3 and 2 ghost layers: the whole
+    storage (ghost faces, edges and corners) equals the oracle's sweep followed by
+    updatePadding bit for bit, in place and compound into another field."""
+    u_spec = (1, [(PER, PER)] * 3, None, (3, 3))
+    w_spec = (1, [(PER, PER)] * 3, None, (2, 2))
+    W, D2 = of.D1WENO53Upwind, of.D2SecondOrderCentered
+
+    def fn(u, w):
+        return u - 0.001 * (of.dx(W, u) + of.dz(W, u)) + 0.01 * of.d2y(D2, w) * w
+    got, ref = _sweep_case(ctx, fn, [u_spec, w_spec], [13, 9, 21], None if op == 0 else w_spec, SEED + 31, op=op)
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("nodes", [[25, 25], [33, 17]], ids=["h1/24", "pow2"])
 def test_sweep_lid_driven_terms(ctx, engine, nodes):
     """Convective terms of examples/LidDriven/LidDriven2D.cpp:30-41 on a MAC grid; on a mesh
