@@ -463,9 +463,9 @@ __device__ __forceinline__ T apply_rule(const GhostRule& r, T v) {
 // non-periodic BC rules of its ghost dims in ascending dim order. Positions are
 // enumerated as a disjoint union of slabs: slab k has coords j<k interior,
 // coord k ghost, coords j>k anywhere in the ghost box.
-template <typename T>
-__global__ void ghost_kernel(T* __restrict__ data, GhostInfo gi, long long n0, long long n1, long long n2) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, typename I>
+__global__ void ghost_kernel(T* __restrict__ data, GhostInfo gi, I n0, I n1, I n2) {
+    const I t = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
     int p[3], src[3], mir[3], lay[3], side[3];
     int act = ghost_locate(gi, t, n0, n1, n2, p, src, mir, lay, side);
     if (act == GHOST_SKIP) return;
@@ -526,9 +526,12 @@ static int ghost_fill_t(const opf_field* f, void* data) {
     }
     long long n0 = slab_count(gi, 0), n1 = d.dim > 1 ? slab_count(gi, 1) : 0, n2 = d.dim > 2 ? slab_count(gi, 2) : 0;
     long long n = n0 + n1 + n2;
-    if (n > 0)
-        hipLaunchKernelGGL(ghost_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)data, gi, n0, n1,
-                           n2);
+    if (n > 0 && n + 256 < (1LL << 32))  // 32-bit thread indices
+        hipLaunchKernelGGL((ghost_kernel<T, uint32_t>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)data,
+                           gi, (uint32_t)n0, (uint32_t)n1, (uint32_t)n2);
+    else if (n > 0)
+        hipLaunchKernelGGL((ghost_kernel<T, long long>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)data,
+                           gi, n0, n1, n2);
     OPF_HIP(hipGetLastError());
     return OPF_OK;
 }

@@ -44,7 +44,10 @@ OPF_HD long long ghost_lin(const GhostInfo& gi, const int p[3]) {
 // the interior mirror `mir` and per-dim (layer, side) of the rules to apply.
 // Returns GHOST_SKIP (nothing to do), GHOST_COPY (copy src -> p: a never-filled
 // position reached by a periodic wrap) or GHOST_RULE.
-OPF_HD int ghost_locate(const GhostInfo& gi, long long t, long long n0, long long n1, long long n2, int p[3],
+// I: the thread-index type (32-bit unsigned where the launch allows: its division is
+// several times cheaper than a 64-bit one)
+template <typename I>
+OPF_HD int ghost_locate(const GhostInfo& gi, I t, I n0, I n1, I n2, int p[3],
                         int src[3], int mir[3], int lay[3], int side[3]) {
     int k;
     if (t < n0) k = 0;

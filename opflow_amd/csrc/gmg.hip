@@ -779,7 +779,7 @@ __global__ __launch_bounds__(256) void k_dense(const double* __restrict__ cinv, 
 //       over each 2 x 2 block; x2 is stored.
 // up:   x3 = x2 + P x_c, x4 = x3 + w (b - A x3) / D, x5 = x4 + w (b - A x4) / D -> out,
 //       on the finest level with the partials of (b, x5), (x5, q), sum(x5).
-// A workgroup owns a TS x TS tile (32 x 32 with 256 threads on levels of >= 2^20
+// A workgroup owns a TS x TS tile (32 x 32 with 256 threads on levels of >= 2^21
 // cells, 16 x 16 with 256 threads down to 2^16, 8 x 8 with one wave below: the small
 // levels are latency-bound, so fewer cells per thread and more workgroups pay); the
 // intermediate vectors live in LDS on the tile plus a halo of 2 (x1, x3), 1 (x2, x4)
@@ -2049,8 +2049,11 @@ static void ph_vtail(const Hierarchy& H, hipStream_t st, const int* mask, int l0
 // (at most GMG_MAXB, then striding); on the finest level k2_up writes the partials
 // of the three dot products, one per block
 static bool fused2d(const Hierarchy& H) { return H.dim == 2 && !H.var && H.pre == 2 && H.post == 2; }
-// tile side of a level's k2 kernels (threads: 256 for 32 and 16, 64 for 8)
-static int f2_ts(const GLevelHost& L) { return L.N >= (1LL << 20) ? 32 : L.N >= (1LL << 16) ? 16 : 8; }
+// tile side of a level's k2 kernels (threads: 256 for 32 and 16, 64 for 8); measured on
+// the 2048^2 hierarchy: 16 x 16 tiles took the 1024^2 level's pair 26.2 -> 25.1 us and
+// 8 x 8 the 128^2 and 64^2 levels' 8-10 -> 5.5 us each, while a 32^2 level stays in the
+// coarse tail (its 8 x 8 pair costs 11 us, the tail 5 us more than without it)
+static int f2_ts(const GLevelHost& L) { return L.N >= (1LL << 21) ? 32 : L.N >= (1LL << 16) ? 16 : 8; }
 static F2Tile f2tile(const GLevelHost& L) {
     const int ts = f2_ts(L);
     F2Tile ft;

@@ -2,6 +2,7 @@
 // position of random ghost boxes is enumerated exactly once, every read
 // (mirror / wrap source) lands in the interior or in storage, and every write
 // lands in storage. Built and run by tests/test_index_math.py.
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -63,6 +64,17 @@ int main() {
         for (long long t = 0; t < tot + 64; ++t) {
             int p[3], src[3], mir[3], lay[3], side[3];
             int act = ghost_locate(gi, t, n[0], n[1], n[2], p, src, mir, lay, side);
+            {  // the 32-bit index form the kernel launches with gives the same answer
+                int p2[3], src2[3], mir2[3], lay2[3], side2[3];
+                const int act2 = ghost_locate<uint32_t>(gi, (uint32_t)t, (uint32_t)n[0], (uint32_t)n[1],
+                                                        (uint32_t)n[2], p2, src2, mir2, lay2, side2);
+                if (act2 != act) return fail("32-bit ghost_locate disagrees", trial);
+                for (int j = 0; j < 3 && act != GHOST_SKIP; ++j)
+                    if (p2[j] != p[j] || src2[j] != src[j] || (act == GHOST_RULE && (mir2[j] != mir[j] ||
+                                                                                      lay2[j] != lay[j] ||
+                                                                                      side2[j] != side[j])))
+                        return fail("32-bit ghost_locate disagrees", trial);
+            }
             if (t >= tot) {
                 if (act != GHOST_SKIP) return fail("thread beyond the slabs did work", trial);
                 continue;
