@@ -420,3 +420,352 @@ __global__ __launch_bounds__(256) void zp_pass(GOp<T> o, const T* __restrict__ i
         }
     }
 }
+
+// ---------------------------------------------------------------------------------------
+// The up half of a V(2,2) level in one pass (same tiling and chunking as zp_pass, so the
+// same partial count): x3 = x2 + P x_c, x4 = x3 + (w / D)(b - A x3), x5 = x4 + (w / D)(b -
+// A x4) stored; Z2_UP_DOT3 also the partials of (b, x5), (x5, q), sum(x5).
+// Two LDS rings of 4 planes: level 0 (x3) on the tile plus a halo of 2 in x and y, level 1
+// (x4) on the tile plus 1. Step zz stores level-0 plane zz, computes level 1 at plane
+// zz - 1 and the output at plane zz - 3 (its level-1 planes were written in earlier
+// steps), so one barrier per step separates every store from its readers. Ghost values
+// follow zp_pass: a non-periodic side's ghost of either level is alpha times the level's
+// boundary cell, so a level-1 ghost position computes the sweep at its mirror cell
+// (inside the tile, its level-0 neighbours in the halo-2 region) times alpha, and the
+// ghost planes -1 / n of level 1 are alpha times its planes 0 / n - 1. Every cell gets
+// the arithmetic of ZP_PJAC followed by ZP_JAC(_DOT3), so the results are the same bits
+// with one pass fewer over the level's vectors. Used in FP64 on levels of >= 8-plane
+// chunks (256^3: 237 -> 191 us per V-cycle for the pair); in FP32, and with the down half
+// (ZP_JAC2Z + ZP_RESTRICT) fused the same way, the two-level kernel measured slower than
+// the separate passes (its registers allow 2 waves per SIMD).
+enum { Z2_UP = 0, Z2_UP_DOT3 };
+
+// cell whose value stands at coordinate g (region coordinates run past the ghost layer
+// by one in halo-2 regions): periodic dims wrap any g, a non-periodic ghost is alpha times
+// the boundary cell, positions further out (never used) clamp to a boundary cell
+template <typename T>
+__device__ __forceinline__ int zp2_mirror(const GOp<T>& o, int d, int g, int n, T& f) {
+    if (o.per[d]) return ((g % n) + n) % n;
+    if (g < 0) {
+        if (g == -1) f *= zp_alpha(o, d, 0);
+        return 0;
+    }
+    if (g >= n) {
+        if (g == n) f *= zp_alpha(o, d, 1);
+        return n - 1;
+    }
+    return g;
+}
+
+// level-1 position at coordinate g: the coordinate whose sweep it holds (itself, or the
+// mirror cell of a non-periodic ghost, factor alpha; further out: a boundary cell, unused)
+template <typename T>
+__device__ __forceinline__ int zp2_src(const GOp<T>& o, int d, int g, int n, T& f) {
+    if (o.per[d]) return g;
+    if (g < 0) {
+        f *= zp_alpha(o, d, 0);
+        return 0;
+    }
+    if (g >= n) {
+        if (g == n) f *= zp_alpha(o, d, 1);
+        return n - 1;
+    }
+    return g;
+}
+
+template <typename T, int M2>
+__global__ __launch_bounds__(256) void zp2_pass(GOp<T> o, const T* __restrict__ xin, const T* __restrict__ xc,
+                                                const T* __restrict__ cb, const T* __restrict__ cq,
+                                                T* __restrict__ out0, double* __restrict__ part, T w, ZpGrid g,
+                                                long long cvs, Bat bt) {
+    using C = ZpCfg<T>;
+    // prefetch distance 2 in both precisions: the second level and the b / q prefetch
+    // take the registers a deeper level-0 prefetch would
+    constexpr int TX = C::TX, TY = C::TY, NT = C::NT, PF = 2;
+    constexpr int RX0 = TX + 4, RY0 = TY + 4, RP0 = RX0 * RY0, NL0 = (RP0 + NT - 1) / NT;
+    constexpr int RX1 = TX + 2, RY1 = TY + 2, RP1 = RX1 * RY1, NL1 = (RP1 + NT - 1) / NT;
+    constexpr int CPT = TX * TY / NT;  // output cells per thread and plane
+    constexpr int RS = NT / TX;        // rows between a thread's output cells
+    constexpr bool DOTS = M2 == Z2_UP_DOT3;
+    static_assert((TX * TY) % NT == 0 && NT % TX == 0 && TX % 2 == 0 && TY % 2 == 0, "tile shape");
+    static_assert(PF % 2 == 0, "the coarse window shifts every second plane: PF must be even");
+    __shared__ T ring0[4][RP0];
+    __shared__ T ring1[4][RP1];
+
+    BAT_SLOT(bt);
+    xin += vo;
+    xc += (long long)slot * cvs;
+    cb += vo;
+    if (DOTS) {
+        cq += vo;
+        part += (long long)slot * bt.ps;
+    }
+    out0 += vo;
+
+    const int tid = threadIdx.x;
+    int bi = (int)xcd_block(blockIdx.x, gridDim.x);  // neighbouring tiles share an XCD's L2
+    const int x0 = (bi % g.tx) * TX;
+    bi /= g.tx;
+    const int y0 = (bi % g.ty) * TY;
+    const int z0 = (bi / g.ty) * g.zc;
+    const int z1 = min(z0 + g.zc, o.n2);
+    const bool zper = o.per[2] != 0;
+
+    // level-0 elements: in-plane offset of the cell they hold, ghost factor, the cell
+    // (x | y << 16: its coarse cells are recomputed per use)
+    uint32_t off0[NL0];
+    T fac0[NL0];
+    uint32_t mxy[NL0];
+    const int cn0 = o.n0 >> 1, cn1 = o.n1 >> 1, cn2 = o.n2 >> 1;
+#pragma unroll
+    for (int m = 0; m < NL0; ++m) {
+        int e = tid + m * NT;
+        if (e >= RP0) e = 0;
+        T f = 1;
+        const int mx = zp2_mirror(o, 0, x0 - 2 + e % RX0, o.n0, f);
+        const int my = zp2_mirror(o, 1, y0 - 2 + e / RX0, o.n1, f);
+        off0[m] = (uint32_t)mx + o.s1 * (uint32_t)my;
+        fac0[m] = f;
+        mxy[m] = (uint32_t)mx | ((uint32_t)my << 16);
+    }
+    // coarse cell (parent or other) of the x-y interpolation at fine coordinate mm
+    // in dim d, and the weight of the other (a non-periodic coarse ghost is alpha * parent)
+    auto coarse_of = [&](int d, int mm, int cn, bool other, T& gf) {
+        const int par = mm >> 1;
+        int oth = (mm & 1) ? par + 1 : par - 1;
+        gf = (T)0.25;
+        if (oth < 0 || oth >= cn) {
+            if (o.per[d]) oth = oth < 0 ? oth + cn : oth - cn;
+            else {
+                gf *= zp_alpha(o, d, oth < 0 ? 0 : 1);
+                oth = par;
+            }
+        }
+        return other ? oth : par;
+    };
+    // level-1 elements: the level-0 ring index of the cell whose sweep they hold, its
+    // in-plane offset (b and the diagonal), the ghost factor
+    int src1[NL1];
+    uint32_t off1[NL1];
+    T fac1[NL1], dx1[NL1];
+#pragma unroll
+    for (int m = 0; m < NL1; ++m) {
+        int e = tid + m * NT;
+        if (e >= RP1) e = 0;
+        T f = 1;
+        const int sx = zp2_src(o, 0, x0 - 1 + e % RX1, o.n0, f);
+        const int sy = zp2_src(o, 1, y0 - 1 + e / RX1, o.n1, f);
+        src1[m] = (sx - (x0 - 2)) + RX0 * (sy - (y0 - 2));
+        T fd = 1;
+        const int cx = zp2_mirror(o, 0, sx, o.n0, fd), cy = zp2_mirror(o, 1, sy, o.n1, fd);
+        off1[m] = (uint32_t)cx + o.s1 * (uint32_t)cy;
+        fac1[m] = f;
+        dx1[m] = zp_dadj(o, 0, cx, o.n0) + zp_dadj(o, 1, cy, o.n1);
+    }
+    // output cells of this thread: (lx, ly0 + m * RS)
+    const int lx = tid % TX, ly0 = tid / TX;
+    uint32_t coff[CPT];
+    bool cval[CPT];
+    T dxc[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int gx = x0 + lx, gy = y0 + ly0 + m * RS;
+        cval[m] = gx < o.n0 && gy < o.n1;
+        const int cx = min(gx, o.n0 - 1), cy = min(gy, o.n1 - 1);
+        coff[m] = (uint32_t)cx + o.s1 * (uint32_t)cy;
+        dxc[m] = zp_dadj(o, 0, cx, o.n0) + zp_dadj(o, 1, cy, o.n1);
+    }
+    // w / D of a cell whose boundary diagonal part is dd
+    auto wdiag = [&](T dd) { return dd != (T)0 ? w / (o.dint - dd) : o.wdi; };
+    // plane of the vectors a level-1 / output plane p reads (periodic z wraps; a
+    // non-periodic plane outside [0, n) is never computed from them: clamped)
+    auto plane_of = [&](int p) { return zper ? ((p % o.n2) + o.n2) % o.n2 : min(max(p, 0), o.n2 - 1); };
+
+    const int zb = z0 - 2, zl0 = z1 + 1;  // level-0 planes zb .. zl0; steps zz = zb .. z1 + 2
+    T pre[PF][NL0];
+    T fzp[PF];
+    int mzp[PF];
+    auto l0_load = [&](T(&r)[NL0], T& fz, int& mzo, int z) {
+        T f = 1;
+        const int mz = zp2_mirror(o, 2, z, o.n2, f);
+        fz = f;
+        mzo = mz;
+        const T* p = xin + (size_t)o.s2 * (uint32_t)mz;
+#pragma unroll
+        for (int m = 0; m < NL0; ++m) r[m] = p[off0[m]];
+    };
+    // x-y interpolated coarse planes Q(c) at the level-0 elements for the pair
+    // (cA, cA + 1), cA = floor((zz - 1) / 2), that plane zz interpolates between in z;
+    // coarse planes outside [0, cn2) follow the coarse ghost rule; the raw values of the
+    // next coarse plane are loaded two fine planes ahead
+    T qa[NL0], qb[NL0], raw[4][NL0];
+    T rawf = 1;
+    auto coarse_load = [&](int c) {
+        if (!o.per[2]) c = min(max(c, -1), cn2);  // a periodic z wraps any plane
+        T f = 1;
+        const int mc = zp2_mirror(o, 2, c, cn2, f);
+        rawf = f;
+        const T* p = xc + (size_t)cn0 * (size_t)cn1 * (uint32_t)mc;
+#pragma unroll
+        for (int m = 0; m < NL0; ++m) {
+            const int mx = (int)(mxy[m] & 0xffffu), my = (int)(mxy[m] >> 16);
+            T gx, gy;
+            const uint32_t px = (uint32_t)coarse_of(0, mx, cn0, false, gx), ox = (uint32_t)coarse_of(0, mx, cn0, true, gx);
+            const uint32_t py = (uint32_t)coarse_of(1, my, cn1, false, gy), oy = (uint32_t)coarse_of(1, my, cn1, true, gy);
+            raw[0][m] = p[px + (uint32_t)cn0 * py];
+            raw[1][m] = p[ox + (uint32_t)cn0 * py];
+            raw[2][m] = p[px + (uint32_t)cn0 * oy];
+            raw[3][m] = p[ox + (uint32_t)cn0 * oy];
+        }
+    };
+    auto coarse_q = [&](T(&qo)[NL0]) {
+#pragma unroll
+        for (int m = 0; m < NL0; ++m) {
+            T gx, gy;
+            coarse_of(0, (int)(mxy[m] & 0xffffu), cn0, true, gx);
+            coarse_of(1, (int)(mxy[m] >> 16), cn1, true, gy);
+            const T ry0 = (T)0.75 * raw[0][m] + gx * raw[1][m];
+            const T ry1 = (T)0.75 * raw[2][m] + gx * raw[3][m];
+            qo[m] = rawf * ((T)0.75 * ry0 + gy * ry1);
+        }
+    };
+    // plane zb = z0 - 2 (even) interpolates between coarse planes z0 / 2 - 2 and z0 / 2 - 1
+    coarse_load(z0 / 2 - 2);
+    coarse_q(qa);
+    coarse_load(z0 / 2 - 1);
+    coarse_q(qb);
+    coarse_load(z0 / 2);
+    // b of step zz's level-1 plane (zz - 1) and output plane (zz - 3), q of the output
+    // plane: loaded PF steps ahead like the level-0 planes
+    T bl1p[PF][NL1], bop[PF][CPT], qop[DOTS ? PF : 1][CPT];
+    auto b_load = [&](T(&bl)[NL1], T(&bo)[CPT], T(&qo)[CPT], int zz) {
+        const size_t po1 = (size_t)o.s2 * (uint32_t)plane_of(zz - 1);
+#pragma unroll
+        for (int m = 0; m < NL1; ++m) bl[m] = cb[po1 + off1[m]];
+        const size_t po2 = (size_t)o.s2 * (uint32_t)plane_of(zz - 3);
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            bo[m] = cb[po2 + coff[m]];
+            if (DOTS) qo[m] = cq[po2 + coff[m]];
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        l0_load(pre[u], fzp[u], mzp[u], min(zb + u, zl0));
+        b_load(bl1p[u], bop[u], qop[DOTS ? u : 0], zb + u);
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int zz0 = zb; zz0 <= z1 + 2; zz0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int zz = zz0 + u;
+            if (zz > z1 + 2) break;  // uniform over the block
+            const int p1 = zz - 1, p2 = zz - 3;
+            const T(&bl1)[NL1] = bl1p[u];
+            const T(&bo)[CPT] = bop[u];
+            const T(&qo)[CPT] = qop[DOTS ? u : 0];
+            // level-0 plane zz
+            if (zz <= zl0) {
+                T* rs = ring0[zz & 3];
+                const T fz = fzp[u];
+                if (u % 2 == 1) {  // odd plane (zb is even): the pair moves up by one coarse plane
+#pragma unroll
+                    for (int m = 0; m < NL0; ++m) qa[m] = qb[m];
+                    coarse_q(qb);
+                    coarse_load((zz + 1) / 2 + 1);
+                }
+                // weights by the parity of the cell interpolated at (the mirror plane)
+                const bool odd = mzp[u] & 1;
+                const T wa = odd ? (T)0.75 : (T)0.25, wb = odd ? (T)0.25 : (T)0.75;
+#pragma unroll
+                for (int m = 0; m < NL0; ++m) {
+                    const int e = tid + m * NT;
+                    if (e < RP0) rs[e] = (fac0[m] * fz) * (pre[u][m] + (wa * qa[m] + wb * qb[m]));
+                }
+            }
+            l0_load(pre[u], fzp[u], mzp[u], min(zz + PF, zl0));
+            __syncthreads();
+            // level 1 at plane p1 (level-0 planes p1 - 1 .. p1 + 1)
+            if (p1 >= z0 - 1 && p1 <= z1 && (zper || p1 >= 0)) {
+                T* r1 = ring1[p1 & 3];
+                if (!zper && p1 == o.n2) {  // ghost plane n: alpha * plane n - 1
+                    const T az = zp_alpha(o, 2, 1);
+                    const T* r1m = ring1[(p1 - 1) & 3];
+#pragma unroll
+                    for (int m = 0; m < NL1; ++m) {
+                        const int e = tid + m * NT;
+                        if (e < RP1) r1[e] = az * r1m[e];
+                    }
+                } else {
+                    const T* pm = ring0[(p1 - 1) & 3];
+                    const T* pc = ring0[p1 & 3];
+                    const T* pp = ring0[(p1 + 1) & 3];
+                    const T dz = zp_dadj(o, 2, plane_of(p1), o.n2);
+                    const bool g0 = !zper && p1 == 0;  // also the ghost plane -1
+                    T* r1g = ring1[(p1 - 1) & 3];
+                    const T az0 = zp_alpha(o, 2, 0);
+#pragma unroll
+                    for (int m = 0; m < NL1; ++m) {
+                        const int e = tid + m * NT;
+                        const int s = src1[m];
+                        const T c = pc[s];
+                        T acc = o.a[0] * pc[s - 1];
+                        acc += o.a[0] * pc[s + 1];
+                        acc += o.a[1] * pc[s - RX0];
+                        acc += o.a[1] * pc[s + RX0];
+                        acc += o.a[2] * pm[s];
+                        acc += o.a[2] * pp[s];
+                        const T av = o.dint * c - acc;
+                        const T v = fac1[m] * (c + wdiag(dx1[m] + dz) * (bl1[m] - av));
+                        if (e < RP1) {
+                            r1[e] = v;
+                            if (g0) r1g[e] = az0 * v;
+                        }
+                    }
+                }
+            }
+            // output plane p2 (level-1 planes p2 - 1 .. p2 + 1, written in earlier steps)
+            if (p2 >= z0 && p2 < z1) {
+                const T* pm = ring1[(p2 - 1) & 3];
+                const T* pc = ring1[p2 & 3];
+                const T* pp = ring1[(p2 + 1) & 3];
+                const size_t po = (size_t)o.s2 * (uint32_t)p2;
+                const T dz = zp_dadj(o, 2, p2, o.n2);
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int ly = ly0 + m * RS;
+                    const int e = (lx + 1) + RX1 * (ly + 1);
+                    const T c = pc[e];
+                    T acc = o.a[0] * pc[e - 1];
+                    acc += o.a[0] * pc[e + 1];
+                    acc += o.a[1] * pc[e - RX1];
+                    acc += o.a[1] * pc[e + RX1];
+                    acc += o.a[2] * pm[e];
+                    acc += o.a[2] * pp[e];
+                    const T av = o.dint * c - acc;
+                    const T xo = c + wdiag(dxc[m] + dz) * (bo[m] - av);
+                    if (cval[m]) {
+                        out0[po + coff[m]] = xo;
+                        if (DOTS) {
+                            const double zd = (double)xo;
+                            s0 += (double)bo[m] * zd;
+                            s1 += zd * (double)qo[DOTS ? m : 0];
+                            s2 += zd;
+                        }
+                    }
+                }
+            }
+            b_load(bl1p[u], bop[u], qop[DOTS ? u : 0], zz + PF);
+        }
+    }
+    if (DOTS) {
+        s0 = gmg_block_sum(s0);
+        s1 = gmg_block_sum(s1);
+        s2 = gmg_block_sum(s2);
+        if (tid == 0) {
+            part[blockIdx.x] = s0;
+            part[GMG_MAXB + blockIdx.x] = s1;
+            part[2 * GMG_MAXB + blockIdx.x] = s2;
+        }
+    }
+}
