@@ -758,10 +758,62 @@ class RawBinaryIStream:
         return self
 
 
+StreamIn, StreamOut, StreamASCII, StreamBinary = 0x1, 0x2, 0x4, 0x8  # Core/Constants.hpp
+
+
+class IOGroup:
+    """Utils::IOGroup (Utils/Writers/IOGroup.hpp:23-167) over the raw binary streams: one
+    stream per field at root + the field's name (files <dir of root>/<name>_<count>.cart);
+    dump(t) writes every field stamped t (RawBinaryOStream), read(t) reads each field's
+    next record (RawBinaryIStream: records are numbered, not indexed by time). One file
+    per field: setAllInOne(True) with several fields is refused."""
+
+    def __init__(self, stream, root, mode, *fields):
+        self.stream_cls, self.root, self.mode, self.fields = stream, str(root), mode, list(fields)
+        self.streams = None
+
+    def _init(self):
+        if self.streams is None:
+            self.streams = [self.stream_cls(self.root + f.name) for f in self.fields]
+
+    def dump(self, t):
+        if self.stream_cls is not RawBinaryOStream:
+            raise TypeError("IOGroup.dump needs an output stream")
+        self._init()
+        for s, f in zip(self.streams, self.fields):
+            s << TimeStamp(float(t)) << f
+
+    def read(self, t=None):
+        if self.stream_cls is not RawBinaryIStream:
+            raise TypeError("IOGroup.read needs an input stream")
+        self._init()
+        for s, f in zip(self.streams, self.fields):
+            s >> f
+
+    def setAllInOne(self, o):
+        if o and len(self.fields) > 1:
+            raise ValueError("IOGroup: the raw binary streams write one file per field")
+
+    def fixedMesh(self):
+        pass
+
+    def dumpToSeparateFile(self):
+        pass
+
+
+def makeIOGroup(stream, root, *args):
+    """Utils::makeIOGroup<Stream>(root, [mode,] fields...)"""
+    if args and isinstance(args[0], int):
+        return IOGroup(stream, root, args[0], *args[1:])
+    return IOGroup(stream, root, StreamOut, *args)
+
+
 class Utils:  # the reference's OpFlow::Utils namespace
     TimeStamp = TimeStamp
     RawBinaryOStream = RawBinaryOStream
     RawBinaryIStream = RawBinaryIStream
+    IOGroup = IOGroup
+    makeIOGroup = staticmethod(makeIOGroup)
 
 
 from .solve import (StructSolverParams, EqnSolveState, Solve, makeEqnSolveHandler,  # noqa: E402
