@@ -72,6 +72,9 @@ extern "C" int opf_ctx_create(int device, opf_ctx** out) {
         return OPF_ERR_HIP;
     }
     c->own_stream = c->stream;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cus = cus;
     if (int rc = scratch_setup(device, &c->noalloc_scratch); rc != OPF_OK) {
         hipStreamDestroy(c->stream);
         delete c;

@@ -488,18 +488,19 @@ inline int tb2_steps(const int n[3]) {
 }
 
 // 3-D blocked launch shape per precision (swept at 512^3 on MI355X): TX x TY
-// columns, NT threads, z chunks of ZC planes, K steps per launch (a remainder of
-// two steps takes one K = 2 launch of the same tile, one step the single-step kernel)
+// columns, NT threads, K steps per launch (a remainder of two steps takes one K = 2
+// launch of the same tile, one step the single-step kernel); the z chunk is chosen per
+// launch (ftcs_t)
 template <typename T>
 struct Ftcs3dCfg;
 template <>
 struct Ftcs3dCfg<double> {
-    static constexpr int TX = 64, TY = 32, NT = 1024, ZC = 64, K = 3;
+    static constexpr int TX = 64, TY = 32, NT = 1024, K = 3;
     static constexpr bool XCD = true;
 };
 template <>
 struct Ftcs3dCfg<float> {
-    static constexpr int TX = 128, TY = 32, NT = 1024, ZC = 64, K = 3;
+    static constexpr int TX = 128, TY = 32, NT = 1024, K = 3;
     static constexpr bool XCD = true;
 };
 
@@ -630,23 +631,33 @@ int ftcs_t(opf_field* f, double coef, int nsteps, int tblock, void* ext_spare = 
             });
         } else if (d.dim == 3 && k > 1) {
             using C3 = Ftcs3dCfg<T>;
-            int nb = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY) * ((a.n[2] + C3::ZC - 1) / C3::ZC);
-            static_assert(C3::K == 3, "remainder launches assume K = 3");
             const bool allper = a.per[0] && a.per[1] && a.per[2];
+            // z chunk: periodic fields take the longest (each chunk recomputes 2K halo
+            // planes) that still gives every CU a workgroup (one 147 KB workgroup fills a
+            // CU). 512^3 FP64: 256 planes, 256 workgroups, one round: 750 -> 800 G
+            // cell-updates/s against 64-plane chunks (four rounds); FP32 (128-wide tiles)
+            // 128 planes: 1515 -> 1610. With Dirichlet / Neumann sides the workgroups that
+            // rebuild ghosts take longer, and 64-plane chunks (more workgroups than CUs,
+            // balanced by the dispatcher) stay faster (512^3 FP64: 376 vs 330 G/s)
+            const int tiles = ((a.n[0] + C3::TX - 1) / C3::TX) * ((a.n[1] + C3::TY - 1) / C3::TY);
+            int zc = allper ? 256 : 64;
+            while (zc > 16 && (long long)tiles * ((a.n[2] + zc - 1) / zc) < f->ctx->cus) zc /= 2;
+            const int nb = tiles * ((a.n[2] + zc - 1) / zc);
+            static_assert(C3::K == 3, "remainder launches assume K = 3");
             by_mode([&](auto M) {
                 constexpr int MD = decltype(M)::value;
                 if (k == C3::K && allper)
                     hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD, true>),
-                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, zc);
                 else if (k == C3::K)
                     hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, C3::K, C3::XCD, false>),
-                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, zc);
                 else if (allper)
                     hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, 2, C3::XCD, true>),
-                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, zc);
                 else
                     hipLaunchKernelGGL((ftcs3d_tb_kernel<T, MD, C3::TX, C3::TY, C3::NT, 2, C3::XCD, false>),
-                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, C3::ZC);
+                                       dim3(nb), dim3(C3::NT), 0, st, in, out, a, zc);
             });
             ghosts_stale = true;
         } else if (d.dim == 3) {

@@ -44,6 +44,7 @@ inline size_t dtype_size(int dt) { return dt == OPF_F32 ? 4 : 8; }
 
 struct opf_ctx {
     int device = 0;
+    int cus = 256;  // compute units of the device (launch shapes that want one workgroup per CU)
     hipStream_t stream = nullptr;      // where the context's work goes
     hipStream_t own_stream = nullptr;  // the stream the context created (destroyed with it)
     // scratch for deterministic reductions (partials + result)
