@@ -654,3 +654,34 @@ def test_sweep_conv_bitexact(ctx, engine, dim):
     got, ref = _sweep_case(ctx, lambda u, w: 0.25 * of.conv(u * w - 1.0, ker) + u, [spec, spec], nodes,
                            spec, SEED + 130 + dim)
     np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kind", ["ftcs", "weno"])
+def test_sweep_f32_error_against_fp64_oracle(ctx, engine, kind):
+    """FP32 sweeps (both engines) against the FP64 oracle on the same input (the FP32
+    field holds the FP64 input rounded to float): the FP32 path computes in float from
+    float spacings, so the bound is a small multiple of float's epsilon (6e-8) relative
+    to the largest value: 1e-6."""
+    n = [37, 29, 24]
+    mesh, om = _mesh(ctx, [k + 1 for k in n], ranges=[(0.0, 1.0), (0.0, 0.75), (0.0, 0.5)])
+    W, D2 = of.D1WENO53Upwind, of.D2SecondOrderCentered
+    if kind == "ftcs":
+        fn = lambda u: u + 1e-5 * (of.d2x(D2, u) + of.d2y(D2, u) + of.d2z(D2, u))
+    else:
+        fn = lambda u: u - 1e-3 * (of.dx(W, u) + of.dy(W, u) + of.dz(W, u))
+    u64 = build_field(ctx, mesh, 3, 1, [(PER, PER)] * 3, ext=(3, 3), dtype="f64", name="u")
+    a = oracle.update_padding(u64.desc, om, random_storage(u64, SEED + 51, 0.5, 1.5))
+    a32 = a.astype(np.float32).astype(np.float64)  # the FP32 field's input, exactly
+    expr = fn(u64)
+    nodes_, flds, root = expr.flatten()
+    ref = oracle.assign(nodes_, root, [(a32, u64.desc)], om, a32.copy(), u64.desc, 0,
+                        update_padding=lambda r: oracle.update_padding(u64.desc, om, r))
+    u32 = build_field(ctx, mesh, 3, 1, [(PER, PER)] * 3, ext=(3, 3), dtype="f32", name="u")
+    u32.upload(a32.astype(np.float32))
+    u32.assign(fn(u32))
+    got = u32.numpy().astype(np.float64)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(kind, "max rel err", err)
+    assert err <= 1e-6, err
+    u64.close()
+    u32.close()
