@@ -156,14 +156,15 @@ def generic_sweeps(of, ctx, ev, n):
 
 def cpp_programs(n):
     """The FTCS time loops written as OpFlow programs (tests/cpp/ftcs2d_example.cpp, ftcs3d_example.cpp)
-    on the C++ front-end, which defers the repeated assignment into fused launches: loop time from
-    the first assignment to the device finishing the last (2-D: until the host read of the result)."""
+    on the C++ front-end, which defers the repeated assignment into fused launches. 3-D: after the
+    first assignment and the same untimed clock-settling steps as the headline, the time from the
+    next assignment to the device finishing the last; 2-D: until the host read of the result."""
     import subprocess
     bind = os.path.join(ROOT, "opflow_amd", "bin")
     out = {}
     for dt_ in ("f64", "f32"):
-        r = subprocess.run([os.path.join(bind, "ftcs3d_example"), str(n), str(CONFIG_STEPS), dt_],
-                           capture_output=True, text=True, timeout=300)
+        r = subprocess.run([os.path.join(bind, "ftcs3d_example"), str(n), str(CONFIG_STEPS), dt_, "",
+                            str(SETTLE_STEPS)], capture_output=True, text=True, timeout=300)
         if r.returncode == 0:
             j = json.loads(r.stdout.strip().splitlines()[-1])
             out[f"cpp_ftcs3d_{n}_{dt_}_{CONFIG_STEPS}steps"] = {"Mcell_updates_per_s": j["Mcell_updates_per_s"],
