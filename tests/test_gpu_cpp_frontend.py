@@ -72,6 +72,28 @@ def test_ftcs3d_example_deferred_steps(tmp_path, n, steps, dtype):
         assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+def test_ftcs3d_example_settle_steps(tmp_path):
+    """The untimed settle assignments (argv[5]) are ordinary time steps: 9 steps after 65
+    settle steps (a deferred flush falls inside them) leave the field the oracle's 74-step
+    FTCS sequence gives, bit for bit, and the JSON line reports them."""
+    import json
+    n, steps, settle = 24, 9, 65
+    out = str(tmp_path / "u.bin")
+    r = _run(["ftcs3d_example", str(n), str(steps), "f64", out, str(settle)])
+    assert r.returncode == 0
+    j = json.loads(r.stdout.strip().splitlines()[-1])
+    assert j["settle"] == settle and j["timed_steps"] == steps - 1
+    d = desc(3, [n + 1] * 3, 1, [(3, 3)] * 3, ext=(1, 1))
+    om = oracle.Mesh([n + 1] * 3)
+    import opflow_amd as of
+    cnt = d.extent[0] * d.extent[1] * d.extent[2]
+    init = of.gen_uniform(0xC0FFEE + 2, cnt, -1.0, 1.0).reshape(d.extent[2], d.extent[1], d.extent[0])
+    init = oracle.update_padding(d, om, init)
+    h = 1.0 / n
+    ref = oracle.ftcs(d, om, init, 0.2 * h * h * 1.0, steps + settle)
+    np.testing.assert_array_equal(np.fromfile(out, dtype=np.float64).reshape(ref.shape), ref)
+
+
 def test_conv1d_example_against_oracle(tmp_path):
     """examples/CONV1D/CONV1D.cpp on the front-end: 100 steps of first-order upwind advection of
     a square pulse, and the conservative Superbee-limited variant, equal to the oracle's
